@@ -1,0 +1,85 @@
+/*
+ * The oracle: a scalar CPU restatement of NxStereo's semi-global matching
+ * path and of the stages around it, used ONLY as the checker by the tests,
+ * by the smoke check and by the benchmark's CPU-baseline leg. Nothing in the
+ * product library links or calls it.
+ *
+ * Every function names the reference file:line it restates. Volumes are
+ * pixel-major: volume[(y * width + x) * disparities + d].
+ *
+ * Pinning (what checks each function against something other than itself):
+ *   orc_census       - census.rs:95-151 closed forms (bit order, thresholds)
+ *   orc_costs        - closed forms: identical features cost 0, x-d<0 reads 0
+ *   orc_path         - the textbook O(D^2) penalty-matrix recurrence
+ *                      (Hirschmueller 2008) and paths.rs:19-42's u8/sentinel
+ *                      form, on random steps at (p1,p2) in {(0,1),(8,64),(78,79)}
+ *   orc_select       - select.rs:247-261 closed forms, hand-worked subpixel cases
+ *   orc_consistency  - check.rs:133-145 tie case; scatter-min vs stacked min
+ *   orc_median3      - sorted middle of the clamped window
+ *   orc_sgm          - lib.rs:630-644 shifted scene; shifted noise; unrelated
+ *                      images with uniqueness 1000 (nx_stereo_test.exs:58-75)
+ *   orc_rectify      - rectify.rs:173-211 and rectify_test.exs:39-162 cases
+ *   orc_despeckle    - filter_test.exs:7-56, filters.rs:359-403
+ *   orc_depth        - depth_test.exs:6-28, depth.rs:95-107
+ *   orc_cloud        - depth_test.exs:30-98, depth.rs:109-154
+ *   orc_temporal     - filter_test.exs:69-100, filters.rs:406-458
+ *   orc_quality      - measures.rs:241-252
+ */
+#ifndef NXS_ORACLE_H
+#define NXS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_INVALID_KEY 0xFFFFFFFFu
+
+/* reference.rs:20-38 */
+void orc_census(const uint8_t *image, int width, int height, uint64_t *features);
+/* reference.rs:41-63; costs are 0..48, stored as bytes */
+void orc_costs(const uint64_t *left, const uint64_t *right, int width, int height,
+               int disparities, uint8_t *costs);
+/* reference.rs:65-121; one direction (dy, dx); values 0..p2+48 as i32 */
+void orc_path(const uint8_t *costs, int width, int height, int disparities, int dy, int dx,
+              int p1, int p2, int32_t *out);
+/* reference.rs:123-146; sums over 3 or 8 directions, 0..1016 */
+void orc_aggregate(const uint8_t *costs, int width, int height, int disparities, int paths,
+                   int p1, int p2, uint16_t *sums);
+/* reference.rs:148-186 */
+void orc_select(const uint16_t *sums, int width, int height, int disparities, float uniqueness,
+                float *disparity, uint32_t *keys);
+/* reference.rs:188-235 */
+void orc_consistency(const float *disparity, const uint32_t *keys, int width, int height,
+                     int disparities, float *out);
+/* reference.rs:237-254 */
+void orc_median3(const float *image, int width, int height, float *out);
+/* reference.rs:256-277 then the Disparity stage's "> 0 else -1" (segment.rs:407-418).
+ * Returns 0, or -1 when memory runs out. */
+int orc_sgm(const uint8_t *left, const uint8_t *right, int width, int height, int disparities,
+            int paths, int p1, int p2, float uniqueness, float *disparity);
+
+/* rectify.rs:34-128: undistort, rectify and shrink one frame. camera = k[9], d[5], inverse_pr[9]. */
+void orc_rectify(const uint8_t *frame, int frame_width, int frame_height, const float *k,
+                 const float *d, const float *inverse_pr, int width, int height, uint8_t *out);
+/* filters.rs:222-268 */
+void orc_despeckle(const float *disparity, int width, int height, float min_support,
+                   float max_difference, int radius, float *out);
+/* depth.rs:12-44 */
+void orc_depth(const float *disparity, int pixels, float focal_length_px, float baseline_m,
+               uint16_t *depth_mm, float *depth_m);
+/* depth.rs:58-89; returns the kept count */
+int orc_cloud(const float *depth_m, int width, int height, int decimation, float focal_length_px,
+              float cx, float cy, float nearest_m, float farthest_m, float *points);
+/* filters.rs:296-318; disparity filtered in place, previous/missed carried */
+void orc_temporal(float *disparity, float *previous, float *missed, int pixels, float alpha,
+                  float delta, float hold, float max_missed);
+/* measures.rs:125-154; previous carried */
+void orc_quality(const float *disparity, float *previous, int width, int height, float *sums5);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

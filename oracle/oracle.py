@@ -1,0 +1,226 @@
+"""ctypes bindings of the oracle and of the CPU baseline.
+
+Test and benchmark infrastructure only: the tests, the smoke check and the
+benchmark's CPU-baseline leg load these libraries as the checker. The
+product package never imports this module.
+
+The C sources (oracle.c, cpu_baseline.c) cite the reference file:line each
+function restates.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+_P = ctypes.c_void_p
+
+
+def build() -> None:
+    """Compiles liboracle.so and libcpubaseline.so beside their sources."""
+    subprocess.run(["make", "-s", "-C", _DIR], check=True)
+
+
+def _load(name: str) -> ctypes.CDLL:
+    path = os.path.join(_DIR, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run `make -C oracle`")
+    return ctypes.CDLL(path)
+
+
+_lib = None
+_cpu = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = _load("liboracle.so")
+        i, f = ctypes.c_int, ctypes.c_float
+        _lib.orc_census.argtypes = [_P, i, i, _P]
+        _lib.orc_costs.argtypes = [_P, _P, i, i, i, _P]
+        _lib.orc_path.argtypes = [_P, i, i, i, i, i, i, i, _P]
+        _lib.orc_aggregate.argtypes = [_P, i, i, i, i, i, i, _P]
+        _lib.orc_select.argtypes = [_P, i, i, i, f, _P, _P]
+        _lib.orc_consistency.argtypes = [_P, _P, i, i, i, _P]
+        _lib.orc_median3.argtypes = [_P, i, i, _P]
+        _lib.orc_sgm.argtypes = [_P, _P, i, i, i, i, i, i, f, _P]
+        _lib.orc_sgm.restype = i
+        _lib.orc_rectify.argtypes = [_P, i, i, _P, _P, _P, i, i, _P]
+        _lib.orc_despeckle.argtypes = [_P, i, i, f, f, i, _P]
+        _lib.orc_depth.argtypes = [_P, i, f, f, _P, _P]
+        _lib.orc_cloud.argtypes = [_P, i, i, i, f, f, f, f, f, _P]
+        _lib.orc_cloud.restype = i
+        _lib.orc_temporal.argtypes = [_P, _P, _P, i, f, f, f, f]
+        _lib.orc_quality.argtypes = [_P, _P, i, i, _P]
+    return _lib
+
+
+def cpu() -> ctypes.CDLL:
+    global _cpu
+    if _cpu is None:
+        _cpu = _load("libcpubaseline.so")
+        i, f = ctypes.c_int, ctypes.c_float
+        _cpu.cpu_sgm.argtypes = [_P, _P, i, i, i, i, i, i, f, _P, i]
+        _cpu.cpu_sgm.restype = i
+        _cpu.cpu_max_threads.restype = i
+    return _cpu
+
+
+def _ptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_P)
+
+
+def _c(a, dtype) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def census(image):
+    image = _c(image, np.uint8)
+    h, w = image.shape
+    out = np.empty((h, w), np.uint64)
+    lib().orc_census(_ptr(image), w, h, _ptr(out))
+    return out
+
+
+def costs(fl, fr, disparities):
+    fl, fr = _c(fl, np.uint64), _c(fr, np.uint64)
+    h, w = fl.shape
+    out = np.empty((h, w, disparities), np.uint8)
+    lib().orc_costs(_ptr(fl), _ptr(fr), w, h, disparities, _ptr(out))
+    return out
+
+
+def path(costs_volume, dy, dx, p1, p2):
+    c = _c(costs_volume, np.uint8)
+    h, w, d = c.shape
+    out = np.empty((h, w, d), np.int32)
+    lib().orc_path(_ptr(c), w, h, d, dy, dx, p1, p2, _ptr(out))
+    return out
+
+
+def aggregate(costs_volume, paths, p1, p2):
+    c = _c(costs_volume, np.uint8)
+    h, w, d = c.shape
+    out = np.empty((h, w, d), np.uint16)
+    lib().orc_aggregate(_ptr(c), w, h, d, paths, p1, p2, _ptr(out))
+    return out
+
+
+def select(sums, uniqueness):
+    s = _c(sums, np.uint16)
+    h, w, d = s.shape
+    disp = np.empty((h, w), np.float32)
+    keys = np.empty((h, w), np.uint32)
+    lib().orc_select(_ptr(s), w, h, d, float(np.float32(uniqueness)), _ptr(disp), _ptr(keys))
+    return disp, keys
+
+
+def consistency(disparity, keys, disparities):
+    disparity, keys = _c(disparity, np.float32), _c(keys, np.uint32)
+    h, w = disparity.shape
+    out = np.empty((h, w), np.float32)
+    lib().orc_consistency(_ptr(disparity), _ptr(keys), w, h, disparities, _ptr(out))
+    return out
+
+
+def median3(image):
+    image = _c(image, np.float32)
+    h, w = image.shape
+    out = np.empty((h, w), np.float32)
+    lib().orc_median3(_ptr(image), w, h, _ptr(out))
+    return out
+
+
+def sgm(left, right, disparities=64, paths=8, p1=8, p2=64, uniqueness=1.0526315789473684):
+    left, right = _c(left, np.uint8), _c(right, np.uint8)
+    h, w = left.shape
+    out = np.empty((h, w), np.float32)
+    rc = lib().orc_sgm(_ptr(left), _ptr(right), w, h, disparities, paths, p1, p2,
+                       float(np.float32(uniqueness)), _ptr(out))
+    if rc:
+        raise MemoryError("oracle sgm ran out of memory")
+    return out
+
+
+def rectify(frame, k, d, inverse_pr, size):
+    frame = _c(frame, np.uint8)
+    fh, fw = frame.shape
+    height, width = size
+    k, d, ipr = _c(k, np.float32), _c(d, np.float32), _c(inverse_pr, np.float32)
+    out = np.empty((height, width), np.uint8)
+    lib().orc_rectify(_ptr(frame), fw, fh, _ptr(k), _ptr(d), _ptr(ipr), width, height, _ptr(out))
+    return out
+
+
+def despeckle(disparity, min_support, max_diff, radius=3):
+    disparity = _c(disparity, np.float32)
+    h, w = disparity.shape
+    out = np.empty((h, w), np.float32)
+    lib().orc_despeckle(_ptr(disparity), w, h, float(np.float32(min_support)),
+                        float(np.float32(max_diff)), radius, _ptr(out))
+    return out
+
+
+def depth(disparity, focal_length_px, baseline_m):
+    disparity = _c(disparity, np.float32)
+    mm = np.empty(disparity.shape, np.uint16)
+    m = np.empty(disparity.shape, np.float32)
+    lib().orc_depth(_ptr(disparity), disparity.size, float(np.float32(focal_length_px)),
+                    float(np.float32(baseline_m)), _ptr(mm), _ptr(m))
+    return mm, m
+
+
+def cloud(depth_m, decimation, focal_length_px, cx, cy, nearest=0.05, farthest=20.0):
+    depth_m = _c(depth_m, np.float32)
+    h, w = depth_m.shape
+    points = np.empty((h // decimation, w // decimation, 3), np.float32)
+    kept = lib().orc_cloud(_ptr(depth_m), w, h, decimation, *(float(np.float32(v)) for v in (
+        focal_length_px, cx, cy, nearest, farthest)), _ptr(points))
+    return points, kept
+
+
+class Temporal:
+    """filters.rs:270-319: the carried previous disparity and missed counts."""
+
+    def __init__(self, shape, max_missed=255.0):
+        self.previous = np.full(shape, -1.0, np.float32)
+        self.missed = np.zeros(shape, np.float32)
+        self.max_missed = max_missed
+
+    def filter(self, disparity, alpha, delta, hold):
+        out = _c(disparity, np.float32).copy()
+        lib().orc_temporal(_ptr(out), _ptr(self.previous), _ptr(self.missed), out.size,
+                           *(float(np.float32(v)) for v in (alpha, delta, hold, self.max_missed)))
+        return out
+
+
+class Quality:
+    """measures.rs:110-155: the carried previous disparity."""
+
+    def __init__(self, shape):
+        self.previous = np.full(shape, -1.0, np.float32)
+
+    def sums(self, disparity):
+        disparity = _c(disparity, np.float32)
+        h, w = disparity.shape
+        out = np.empty(5, np.float32)
+        lib().orc_quality(_ptr(disparity), _ptr(self.previous), w, h, _ptr(out))
+        return out
+
+
+def cpu_sgm(left, right, disparities=64, paths=8, p1=8, p2=64, uniqueness=1.0526315789473684,
+            threads=0):
+    """The multi-threaded stereo_match-structured baseline; threads=0 takes every core."""
+    left, right = _c(left, np.uint8), _c(right, np.uint8)
+    h, w = left.shape
+    out = np.empty((h, w), np.float32)
+    rc = cpu().cpu_sgm(_ptr(left), _ptr(right), w, h, disparities, paths, p1, p2,
+                       float(np.float32(uniqueness)), _ptr(out), threads)
+    if rc:
+        raise MemoryError("cpu baseline ran out of memory")
+    return out
