@@ -1,0 +1,143 @@
+/*
+ * nxs_hip: the C ABI of the MI355X (gfx950) NxStereo segment backend.
+ *
+ * It replaces the native stereo_match process that NxStereo.NativeSegment
+ * drives over an Erlang Port (lib/nx_stereo/native_segment.ex,
+ * lib/nx_stereo/matcher.ex:248-251), in-process: the records are those of
+ * native/stereo_match/src/protocol.rs:1-23, byte for byte, so the Elixir
+ * side's encoder and decoder (native_segment.ex:256-386) serve unchanged.
+ *
+ *   request  1 configure  -> reply tag 7 (configured) or 9 (refused, UTF-8 reason)
+ *   request  2 run        -> reply tag 8 (member times in us, output tensors) or 9
+ *   request  3 release    -> no reply record (the ticket completes with an empty reply)
+ *
+ * Tensors: name, type (0 u8, 1 u16, 2 f32, 3 s32), rank, u32 dimensions,
+ * row-major little-endian elements (tensor.rs:25-40).
+ *
+ * Threading: every function may be called from any thread. nxs_submit copies
+ * the record and returns at once; a context-owned thread computes requests
+ * in submission order per segment and posts completions. The process never
+ * aborts on a request: configuration errors and run errors become tag 9.
+ */
+#ifndef NXS_HIP_H
+#define NXS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nxs_ctx nxs_ctx;
+
+/* Status codes of the calls below. */
+#define NXS_OK 0
+#define NXS_TIMEOUT 1
+#define NXS_EINVAL (-1)
+#define NXS_EDEVICE (-2)
+#define NXS_ENOMEM (-3)
+#define NXS_EUNKNOWN (-4)
+
+/* A completed request's reply record, valid only during the call. */
+typedef void (*nxs_done_fn)(void *user, int64_t ticket, const uint8_t *reply, size_t len);
+
+/* Opens a context on HIP device `device`: the stream owner and device-memory
+ * owner of every segment configured through it. Replaces opening the Port
+ * (matcher.ex:248-255). On failure writes a reason into err. */
+int nxs_open(int device, nxs_ctx **out, char *err, size_t errlen);
+
+/* Fails every pending ticket, releases every segment, frees the device
+ * memory and joins the context's thread (matcher.ex:148-160: the owner's
+ * exit releases its segments). */
+void nxs_close(nxs_ctx *ctx);
+
+/* Queues one request record (protocol.rs:107-168). Returns a ticket >= 1, or
+ * a negative status. With `done`, the reply is posted to done(user, ...)
+ * from the context's thread; without, it is kept for nxs_wait/nxs_reply.
+ * Replaces Matcher.send_run/6 + the Port write (matcher.ex:66-115). */
+int64_t nxs_submit(nxs_ctx *ctx, const uint8_t *record, size_t len, nxs_done_fn done,
+                   void *user);
+
+/* Waits for `ticket` (submitted without a callback): NXS_OK once done,
+ * NXS_TIMEOUT after timeout_ms (< 0 waits forever), NXS_EINVAL if unknown. */
+int nxs_wait(nxs_ctx *ctx, int64_t ticket, int timeout_ms);
+
+/* The kept reply of a completed ticket; valid until nxs_release_reply. */
+int nxs_reply(nxs_ctx *ctx, int64_t ticket, const uint8_t **reply, size_t *len);
+void nxs_release_reply(nxs_ctx *ctx, int64_t ticket);
+
+/* How many kept replies are complete and not yet released (non-blocking). */
+int nxs_poll(nxs_ctx *ctx, int max);
+
+/* ---- The matcher on device buffers (the hot path, without records) ----
+ *
+ * Stage.Disparity's computation (stage/disparity.ex:135-163, full search;
+ * Matcher::disparity, native/stereo_match/src/lib.rs:107-144): u8 left/right
+ * {height,width} in device memory -> f32 disparity {height,width}, -1.0 where
+ * unmatched, enqueued on `stream` (a hipStream_t; NULL for the null stream).
+ * disparities in {64, 128, 192, 256} (the reference takes 64 and 128),
+ * paths in {3, 8}, 0 <= p1 < p2 <= 79, uniqueness >= 1, 8 < width <= 4096,
+ * height > 6. `workspace` is device memory of nxs_match_workspace_size bytes,
+ * owned by the caller; calls that overlap must not share one. */
+size_t nxs_match_workspace_size(int width, int height, int disparities, int paths);
+int nxs_match_device(const uint8_t *left, const uint8_t *right, int width, int height,
+                     int disparities, int paths, int p1, int p2, float uniqueness,
+                     float *disparity, void *workspace, size_t workspace_bytes, void *stream);
+
+/* The same on host buffers, returning the intermediate products for
+ * inspection (any output pointer may be NULL): census features u64 {H,W}
+ * of each image, matching costs u8 {H,W,D}, aggregated path costs u16
+ * {H,W,D}, the selected disparity f32 and key u32 {H,W} before the
+ * left-right check, the checked disparity f32 {H,W}, and the stage's output.
+ * Synchronous; allocates and frees its own device memory. */
+int nxs_match_host(const uint8_t *left, const uint8_t *right, int width, int height,
+                   int disparities, int paths, int p1, int p2, float uniqueness,
+                   uint64_t *census_left, uint64_t *census_right, uint8_t *costs,
+                   uint16_t *sums, float *selected, uint32_t *keys, float *checked,
+                   float *disparity);
+
+/* ---- A configured segment on device-resident entries ----
+ *
+ * Segment::run (native/stereo_match/src/segment.rs:206-239) with the entry
+ * tensors already in device memory: `record` is a run record (kind 2) of a
+ * segment configured through nxs_submit, holding each member's live inputs
+ * (its entry count may be 0); `entries` are device tensors read in place
+ * (a disparity entry is copied first, since filters change it in place).
+ * Every member is enqueued on `stream` (NULL: the segment's own stream) and
+ * the call returns without waiting. `outputs` receives up to max_outputs
+ * filled tensors in first-filled order, as device memory of the segment,
+ * valid until its next run; *n_outputs is their count. Run errors return
+ * NXS_EINVAL with the reason in nxs_last_error. No counterpart in the
+ * reference, whose runs always cross a pipe: it serves callers whose frames
+ * are already on the GPU (the benchmark, the batched and served forms). */
+typedef struct nxs_tensor {
+    const char *name;
+    uint8_t type; /* 0 u8, 1 u16, 2 f32, 3 s32 */
+    uint8_t rank;
+    uint32_t shape[4];
+    void *data;
+} nxs_tensor;
+
+int nxs_run_device(nxs_ctx *ctx, const uint8_t *record, size_t len, const nxs_tensor *entries,
+                   int n_entries, nxs_tensor *outputs, int max_outputs, int *n_outputs,
+                   void *stream);
+
+/* ---- Kernel timing ----
+ * Between start and stop, every matcher launch is bracketed by HIP events on
+ * its own stream (up to `capacity` launches). nxs_profile_stop waits for
+ * them and adds up milliseconds and launch counts per kernel, in the order
+ * census, cost, path (the aggregation), select, consistency, median. No
+ * counterpart in the reference (test/bench/pipeline.exs --kernels times its
+ * Vulkan kernels one by one). */
+int nxs_profile_start(int capacity);
+int nxs_profile_stop(double *total_ms, int *launches, int kernels);
+
+/* Why the last failing call on this thread failed (a static buffer per thread). */
+const char *nxs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

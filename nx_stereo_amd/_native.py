@@ -1,0 +1,134 @@
+"""ctypes bindings of libnxs_hip.so (include/nxs_hip.h).
+
+The library must be built (``python -m nx_stereo_amd.build``); there is no
+fallback. Loading fails loudly when it is missing, and every entry point that
+reaches the GPU fails loudly when the HIP runtime finds no device.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libnxs_hip.so")
+
+NXS_OK, NXS_TIMEOUT, NXS_EINVAL, NXS_EDEVICE, NXS_ENOMEM, NXS_EUNKNOWN = 0, 1, -1, -2, -3, -4
+
+# Every symbol include/nxs_hip.h declares.
+SYMBOLS = (
+    "nxs_open", "nxs_close", "nxs_submit", "nxs_wait", "nxs_reply", "nxs_release_reply",
+    "nxs_poll", "nxs_match_workspace_size", "nxs_match_device", "nxs_match_host",
+    "nxs_run_device", "nxs_profile_start", "nxs_profile_stop", "nxs_last_error",
+)
+
+DONE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int64,
+                           ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t)
+
+_lib = None
+
+
+class NxsTensor(ctypes.Structure):
+    """struct nxs_tensor: a named device tensor."""
+
+    _fields_ = [("name", ctypes.c_char_p), ("type", ctypes.c_uint8), ("rank", ctypes.c_uint8),
+                ("shape", ctypes.c_uint32 * 4), ("data", ctypes.c_void_p)]
+
+
+class NativeError(RuntimeError):
+    """A failing call into libnxs_hip, with the library's reason."""
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBRARY):
+        raise NativeError(f"{LIBRARY} is not built: run `python -m nx_stereo_amd.build`")
+    lib = ctypes.CDLL(LIBRARY)
+    P, i, f, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_size_t
+    lib.nxs_open.argtypes = [i, ctypes.POINTER(P), ctypes.c_char_p, sz]
+    lib.nxs_open.restype = i
+    lib.nxs_close.argtypes = [P]
+    lib.nxs_close.restype = None
+    lib.nxs_submit.argtypes = [P, P, sz, DONE_FN, P]
+    lib.nxs_submit.restype = ctypes.c_int64
+    lib.nxs_wait.argtypes = [P, ctypes.c_int64, i]
+    lib.nxs_wait.restype = i
+    lib.nxs_reply.argtypes = [P, ctypes.c_int64, ctypes.POINTER(P), ctypes.POINTER(sz)]
+    lib.nxs_reply.restype = i
+    lib.nxs_release_reply.argtypes = [P, ctypes.c_int64]
+    lib.nxs_release_reply.restype = None
+    lib.nxs_poll.argtypes = [P, i]
+    lib.nxs_poll.restype = i
+    lib.nxs_match_workspace_size.argtypes = [i, i, i, i]
+    lib.nxs_match_workspace_size.restype = sz
+    lib.nxs_match_device.argtypes = [P, P, i, i, i, i, i, i, f, P, P, sz, P]
+    lib.nxs_match_device.restype = i
+    lib.nxs_match_host.argtypes = [P, P, i, i, i, i, i, i, f] + [P] * 8
+    lib.nxs_match_host.restype = i
+    lib.nxs_run_device.argtypes = [P, P, sz, ctypes.POINTER(NxsTensor), i,
+                                   ctypes.POINTER(NxsTensor), i, ctypes.POINTER(i), P]
+    lib.nxs_run_device.restype = i
+    lib.nxs_profile_start.argtypes = [i]
+    lib.nxs_profile_start.restype = i
+    lib.nxs_profile_stop.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i), i]
+    lib.nxs_profile_stop.restype = i
+    lib.nxs_last_error.argtypes = []
+    lib.nxs_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().nxs_last_error().decode("utf-8", "replace")
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != NXS_OK:
+        raise NativeError(f"{what} failed ({rc}): {last_error()}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def match_host(left: np.ndarray, right: np.ndarray, disparities: int = 64, paths: int = 8,
+               p1: int = 8, p2: int = 64, uniqueness: float = 1.0526315789473684,
+               intermediates: bool = False) -> dict:
+    """Stage.Disparity on the GPU from host images; with `intermediates`, every product."""
+    left = np.ascontiguousarray(left, np.uint8)
+    right = np.ascontiguousarray(right, np.uint8)
+    h, w = left.shape
+    out = {"disparity": np.empty((h, w), np.float32)}
+    if intermediates:
+        out.update(
+            census_left=np.empty((h, w), np.uint64), census_right=np.empty((h, w), np.uint64),
+            costs=np.empty((h, w, disparities), np.uint8),
+            sums=np.empty((h, w, disparities), np.uint16),
+            selected=np.empty((h, w), np.float32), keys=np.empty((h, w), np.uint32),
+            checked=np.empty((h, w), np.float32))
+    g = out.get
+    rc = load().nxs_match_host(
+        _ptr(left), _ptr(right), w, h, disparities, paths, p1, p2,
+        float(np.float32(uniqueness)), _ptr(g("census_left")), _ptr(g("census_right")),
+        _ptr(g("costs")), _ptr(g("sums")), _ptr(g("selected")), _ptr(g("keys")),
+        _ptr(g("checked")), _ptr(out["disparity"]))
+    _check(rc, "nxs_match_host")
+    return out
+
+
+def workspace_size(width: int, height: int, disparities: int, paths: int) -> int:
+    return int(load().nxs_match_workspace_size(width, height, disparities, paths))
+
+
+def match_device(left_ptr: int, right_ptr: int, width: int, height: int, disparities: int,
+                 paths: int, p1: int, p2: int, uniqueness: float, out_ptr: int,
+                 workspace_ptr: int, workspace_bytes: int, stream: int = 0) -> None:
+    """Stage.Disparity on device buffers (raw device addresses), enqueued on `stream`."""
+    rc = load().nxs_match_device(
+        ctypes.c_void_p(left_ptr), ctypes.c_void_p(right_ptr), width, height, disparities,
+        paths, p1, p2, float(np.float32(uniqueness)), ctypes.c_void_p(out_ptr),
+        ctypes.c_void_p(workspace_ptr), workspace_bytes, ctypes.c_void_p(stream or None))
+    _check(rc, "nxs_match_device")

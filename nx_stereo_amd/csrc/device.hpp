@@ -1,0 +1,50 @@
+// Device-side helpers shared by the gfx950 kernels: packed u16 arithmetic,
+// DPP row exchanges and reductions over a 16-lane DPP row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nxs {
+
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ u16x2 as_v(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
+__device__ __forceinline__ uint32_t as_u(u16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+
+// v_pk_min_u16 / v_pk_add_u16 / v_pk_sub_u16 on two u16 lanes held in one VGPR.
+__device__ __forceinline__ uint32_t pk_min(uint32_t a, uint32_t b) {
+    return as_u(__builtin_elementwise_min(as_v(a), as_v(b)));
+}
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) { return as_u(as_v(a) + as_v(b)); }
+__device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) { return as_u(as_v(a) - as_v(b)); }
+
+// DPP controls (GFX9): row_shl:1, row_shr:1, row_ror:n. A DPP row is 16 lanes.
+constexpr int DPP_ROW_SHL1 = 0x101;
+constexpr int DPP_ROW_SHR1 = 0x111;
+constexpr int dpp_row_ror(int n) { return 0x120 + n; }
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+// Minimum over the 16 lanes of a DPP row, left in every lane of the row.
+__device__ __forceinline__ uint32_t row_min_u32(uint32_t v) {
+    v = min(v, dpp<dpp_row_ror(8)>(v));
+    v = min(v, dpp<dpp_row_ror(4)>(v));
+    v = min(v, dpp<dpp_row_ror(2)>(v));
+    v = min(v, dpp<dpp_row_ror(1)>(v));
+    return v;
+}
+
+// Lane-wise u16 minimum of two halves over the 16 lanes of a DPP row.
+__device__ __forceinline__ uint32_t row_min_pk(uint32_t v) {
+    v = pk_min(v, dpp<dpp_row_ror(8)>(v));
+    v = pk_min(v, dpp<dpp_row_ror(4)>(v));
+    v = pk_min(v, dpp<dpp_row_ror(2)>(v));
+    v = pk_min(v, dpp<dpp_row_ror(1)>(v));
+    return v;
+}
+
+}  // namespace nxs

@@ -1,0 +1,70 @@
+// Host-side launchers of the gfx950 kernels. Every launcher enqueues on the
+// given stream and returns the launch status; none synchronises.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace nxs {
+
+constexpr uint32_t INVALID_KEY = 0xFFFFFFFFu;
+
+// Device buffers of one matching at W x H x D. `paired[o]` holds, per line
+// orientation o (rows, columns, x-y diagonals, x+y diagonals), the byte sum
+// of its two directions' path costs (one direction for columns with 3 paths).
+struct MatchBuffers {
+    uint64_t *census_left = nullptr, *census_right = nullptr;  // W*H
+    uint8_t *costs = nullptr;                                  // W*H*D
+    uint8_t *paired[4] = {nullptr, nullptr, nullptr, nullptr}; // W*H*D each
+    float *selected = nullptr;                                 // W*H
+    uint32_t *keys = nullptr;                                  // W*H
+    float *checked = nullptr;                                  // W*H
+};
+
+// Bytes MatchBuffers needs at this size, and carving them from one allocation.
+size_t match_bytes(int width, int height, int disparities, int paths);
+void match_carve(MatchBuffers &b, void *base, int width, int height, int disparities, int paths);
+
+struct MatchParams {
+    int width, height, disparities, paths, p1, p2;
+    float uniqueness;
+};
+
+// Disparity stage: left/right u8 {H,W} -> disparity f32 {H,W}, -1 where unmatched.
+// `sums` (optional, W*H*D u16) receives the aggregated path costs.
+hipError_t launch_match(const MatchParams &p, const uint8_t *left, const uint8_t *right,
+                        MatchBuffers &b, float *disparity, uint16_t *sums, hipStream_t stream);
+
+// Camera model of the rectification stage: k[9], plumb_bob d[5], inverse_pr[9], all f32.
+struct Camera {
+    float k[9], d[5], inverse_pr[9];
+};
+
+hipError_t launch_rectify(const uint8_t *left_frame, const uint8_t *right_frame, int frame_width,
+                          int frame_height, const Camera &left, const Camera &right, int width,
+                          int height, uint8_t *left_out, uint8_t *right_out, hipStream_t stream);
+
+hipError_t launch_despeckle(const float *in, float *out, int width, int height, float min_support,
+                            float max_diff, int radius, hipStream_t stream);
+
+hipError_t launch_depth(const float *disparity, int pixels, float focal_length_px,
+                        float baseline_m, uint16_t *depth_mm, float *depth_m, hipStream_t stream);
+
+// `partials` holds at least cloud_partials(width, height, decimation) ints.
+int cloud_partials(int width, int height, int decimation);
+hipError_t launch_cloud(const float *depth_m, int width, int height, int decimation,
+                        float focal_length_px, float cx, float cy, float nearest, float farthest,
+                        float *points, int32_t *partials, int32_t *kept, hipStream_t stream);
+
+hipError_t launch_temporal(float *disparity, float *previous, float *missed, int pixels,
+                           float alpha, float delta, float hold, float max_missed,
+                           hipStream_t stream);
+
+// `columns` holds 5 * width floats of scratch.
+hipError_t launch_quality(const float *disparity, float *previous, int width, int height,
+                          float *columns, float *sums5, hipStream_t stream);
+
+hipError_t launch_fill_f32(float *dst, float value, size_t count, hipStream_t stream);
+
+}  // namespace nxs

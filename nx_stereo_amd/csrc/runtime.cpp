@@ -1,0 +1,1061 @@
+// The segment runtime behind the C ABI (include/nxs_hip.h): the request
+// records of native/stereo_match/src/protocol.rs, the configured stage lists
+// of segment.rs, device arenas and streams per segment, and a context thread
+// that computes requests in order and posts replies.
+#include "../../include/nxs_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "kernels.hpp"
+#include "profile.hpp"
+
+namespace nxs {
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string &why) {
+    g_last_error = why;
+    return code;
+}
+
+// ------------------------------------------------------------------------------------------
+// Kernel timing (profile.hpp): a pool of event pairs, filled in launch order.
+
+struct Profiler {
+    std::mutex mu;
+    bool on = false;
+    int used = 0;
+    std::vector<hipEvent_t> begin, end;
+    std::vector<int> kernel;
+};
+static Profiler g_profiler;
+
+int profile_begin(ProfiledKernel k, hipStream_t stream) {
+    std::lock_guard<std::mutex> lock(g_profiler.mu);
+    if (!g_profiler.on || g_profiler.used >= (int)g_profiler.begin.size()) return -1;
+    const int slot = g_profiler.used++;
+    g_profiler.kernel[slot] = k;
+    if (hipEventRecord(g_profiler.begin[slot], stream) != hipSuccess) return -1;
+    return slot;
+}
+
+void profile_end(int slot, hipStream_t stream) {
+    if (slot >= 0) (void)hipEventRecord(g_profiler.end[slot], stream);
+}
+
+// ------------------------------------------------------------------------------------------
+// Records (protocol.rs:1-23, 55-168; tensor.rs:25-75)
+
+enum : uint8_t { CONFIGURE = 1, RUN = 2, RELEASE = 3 };
+enum : uint8_t { TAG_CONFIGURED = 7, TAG_OUTPUTS = 8, TAG_REFUSED = 9 };
+enum : uint8_t { TYPE_U8 = 0, TYPE_U16 = 1, TYPE_F32 = 2, TYPE_S32 = 3 };
+
+static int element_bytes(uint8_t type) {
+    switch (type) {
+        case TYPE_U8: return 1;
+        case TYPE_U16: return 2;
+        case TYPE_F32:
+        case TYPE_S32: return 4;
+        default: return -1;
+    }
+}
+
+struct Error {
+    std::string reason;
+};
+
+struct Reader {
+    const uint8_t *p;
+    size_t n;
+
+    const uint8_t *take(size_t count) {
+        if (count > n) {
+            std::ostringstream s;
+            s << "the request ends " << (count - n) << " bytes early";
+            throw Error{s.str()};
+        }
+        const uint8_t *t = p;
+        p += count;
+        n -= count;
+        return t;
+    }
+    uint8_t u8() { return *take(1); }
+    uint16_t u16() {
+        const uint8_t *b = take(2);
+        return (uint16_t)(b[0] | (b[1] << 8));
+    }
+    uint32_t u32() {
+        const uint8_t *b = take(4);
+        return (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);
+    }
+    double f64() {
+        double v;
+        std::memcpy(&v, take(8), 8);
+        return v;
+    }
+    std::string name() {
+        size_t length = u8();
+        const uint8_t *b = take(length);
+        return std::string(reinterpret_cast<const char *>(b), length);
+    }
+};
+
+struct HostTensor {
+    uint8_t type = 0;
+    std::vector<uint32_t> shape;
+    const uint8_t *bytes = nullptr;  // into the request record
+    size_t size = 0;
+};
+
+using Options = std::map<std::string, double>;
+using Inputs = std::map<std::string, std::vector<float>>;
+
+struct Geometry {
+    int width, height, frame_width, frame_height;
+};
+
+struct Request {
+    uint8_t kind;
+    uint32_t segment;
+    Geometry geometry{};
+    std::vector<std::pair<std::string, Options>> members;
+    std::vector<Inputs> inputs;
+    std::vector<std::pair<std::string, HostTensor>> entries;
+};
+
+static Request parse(const uint8_t *record, size_t len) {
+    Reader r{record, len};
+    Request q;
+    q.kind = r.u8();
+    q.segment = r.u32();
+    if (q.kind == CONFIGURE) {
+        q.geometry.width = r.u16();
+        q.geometry.height = r.u16();
+        q.geometry.frame_width = r.u16();
+        q.geometry.frame_height = r.u16();
+        int count = r.u8();
+        for (int m = 0; m < count; m++) {
+            std::string kind = r.name();
+            Options options;
+            int n = r.u8();
+            for (int i = 0; i < n; i++) {
+                std::string key = r.name();
+                options[key] = r.f64();
+            }
+            q.members.emplace_back(kind, options);
+        }
+    } else if (q.kind == RUN) {
+        int count = r.u8();
+        for (int m = 0; m < count; m++) {
+            Inputs inputs;
+            int n = r.u8();
+            for (int i = 0; i < n; i++) {
+                std::string key = r.name();
+                int values = r.u16();
+                const uint8_t *b = r.take(4 * (size_t)values);
+                std::vector<float> v(values);
+                std::memcpy(v.data(), b, 4 * (size_t)values);
+                inputs[key] = std::move(v);
+            }
+            q.inputs.push_back(std::move(inputs));
+        }
+        int entries = r.u8();
+        for (int e = 0; e < entries; e++) {
+            std::string key = r.name();
+            HostTensor t;
+            t.type = r.u8();
+            int rank = r.u8();
+            size_t elements = 1;
+            for (int d = 0; d < rank; d++) {
+                t.shape.push_back(r.u32());
+                elements *= t.shape.back();
+            }
+            int eb = element_bytes(t.type);
+            if (eb < 0) throw Error{"unknown element type " + std::to_string(t.type)};
+            t.size = elements * eb;
+            t.bytes = r.take(t.size);
+            bool replaced = false;
+            for (auto &existing : q.entries)
+                if (existing.first == key) {
+                    existing.second = t;
+                    replaced = true;
+                }
+            if (!replaced) q.entries.emplace_back(key, t);
+        }
+    } else if (q.kind != RELEASE) {
+        throw Error{"unknown request kind " + std::to_string(q.kind)};
+    }
+    if (r.n != 0) throw Error{std::to_string(r.n) + " bytes follow the request"};
+    return q;
+}
+
+static std::vector<uint8_t> refused_record(const std::string &reason) {
+    std::vector<uint8_t> v;
+    v.push_back(TAG_REFUSED);
+    v.insert(v.end(), reason.begin(), reason.end());
+    return v;
+}
+
+static std::string shape_text(const std::vector<uint32_t> &shape) {
+    std::ostringstream s;
+    s << "[";
+    for (size_t i = 0; i < shape.size(); i++) s << (i ? ", " : "") << shape[i];
+    s << "]";
+    return s.str();
+}
+
+static std::string number_text(double v) {
+    std::ostringstream s;
+    if (v == std::floor(v) && std::fabs(v) < 1e15) s << (long long)v;
+    else s << v;
+    return s.str();
+}
+
+// ------------------------------------------------------------------------------------------
+// Stages (segment.rs:244-333): kinds, build settings and the constants they share with Nx.
+
+enum class Kind { Rectification, Disparity, SpeckleFilter, TemporalFilter, Depth, PointCloud, Quality };
+
+struct Member {
+    Kind kind;
+    std::string name;
+    int disparities = 0, paths = 0;
+    int radius = 0;
+    int decimation = 0;
+    float nearest = 0, farthest = 0;
+    float max_missed = 0;
+};
+
+static double option(const Options &o, const std::string &kind, const char *name) {
+    auto it = o.find(name);
+    if (it == o.end()) throw Error{kind + " needs its " + name + " setting"};
+    return it->second;
+}
+
+static void takes_only(const Options &o, const std::string &kind,
+                       std::initializer_list<const char *> names) {
+    std::vector<std::string> unknown;
+    for (auto &kv : o) {
+        bool known = false;
+        for (const char *n : names) known = known || kv.first == n;
+        if (!known) unknown.push_back(kv.first);
+    }
+    if (unknown.empty()) return;
+    std::string list;
+    for (size_t i = 0; i < unknown.size(); i++) list += (i ? ", " : "") + unknown[i];
+    throw Error{kind + " has no setting " + list};
+}
+
+static int count_option(const Options &o, const std::string &kind, const char *name, int lowest,
+                        int highest) {
+    double v = option(o, kind, name);
+    if (v == std::floor(v) && v >= lowest && v <= highest) return (int)v;
+    throw Error{kind + " " + name + " must be an integer from " + std::to_string(lowest) + " to " +
+                std::to_string(highest) + ", got " + number_text(v)};
+}
+
+static int integer_option(const Options &o, const std::string &kind, const char *name,
+                          std::initializer_list<int> allowed) {
+    double v = option(o, kind, name);
+    std::string list;
+    for (int a : allowed) {
+        if ((double)a == v) return a;
+        list += (list.empty() ? "" : ", ") + std::to_string(a);
+    }
+    throw Error{kind + " " + name + " must be one of [" + list + "], got " + number_text(v)};
+}
+
+// Parameters::check (lib.rs:41-73), with 192 and 256 disparities added.
+static void check_matching(int W, int H, int D, int paths, int p1, int p2, float uniqueness) {
+    if (!(W > 8 && H > 6 && W <= 4096))
+        throw Error{"expected an image wider than 8, higher than 6 and at most 4096 wide, got " +
+                    std::to_string(W) + "x" + std::to_string(H)};
+    if (D != 64 && D != 128 && D != 192 && D != 256)
+        throw Error{"disparities must be 64, 128, 192 or 256, got " + std::to_string(D)};
+    if (paths != 3 && paths != 8) throw Error{"paths must be 3 or 8, got " + std::to_string(paths)};
+    if (!(p1 < p2 && p2 + 48 <= 127))
+        throw Error{"expected 0 <= p1 < p2 <= 79, got " + std::to_string(p1) + " and " +
+                    std::to_string(p2)};
+    if (std::isnan(uniqueness) || uniqueness < 1.0f) {
+        std::ostringstream s;
+        s << "uniqueness must be at least 1.0, got " << uniqueness;
+        throw Error{s.str()};
+    }
+}
+
+static Member make_member(const Geometry &g, const std::string &kind, const Options &o) {
+    Member m;
+    m.name = kind;
+    const int W = g.width, H = g.height;
+    if (kind == "rectification") {
+        takes_only(o, kind, {});
+        if (g.frame_width < W || g.frame_height < H)
+            throw Error{"rectification only shrinks: " + std::to_string(g.frame_width) + "x" +
+                        std::to_string(g.frame_height) + " frames are smaller than the " +
+                        std::to_string(W) + "x" + std::to_string(H) + " working size"};
+        m.kind = Kind::Rectification;
+    } else if (kind == "disparity") {
+        takes_only(o, kind, {"num_disparities", "paths"});
+        m.disparities = integer_option(o, kind, "num_disparities", {64, 128, 192, 256});
+        m.paths = integer_option(o, kind, "paths", {3, 8});
+        check_matching(W, H, m.disparities, m.paths, 8, 64, 1.0f);
+        m.kind = Kind::Disparity;
+    } else if (kind == "speckle_filter") {
+        takes_only(o, kind, {"radius"});
+        m.radius = count_option(o, kind, "radius", 1, std::min(W, H));
+        m.kind = Kind::SpeckleFilter;
+    } else if (kind == "temporal_filter") {
+        takes_only(o, kind, {"max_missed"});
+        m.max_missed = (float)option(o, kind, "max_missed");
+        m.kind = Kind::TemporalFilter;
+    } else if (kind == "depth") {
+        takes_only(o, kind, {});
+        m.kind = Kind::Depth;
+    } else if (kind == "point_cloud") {
+        takes_only(o, kind, {"decimation", "min_depth_m", "max_depth_m"});
+        m.decimation = count_option(o, kind, "decimation", 1, std::min(W, H));
+        m.nearest = (float)option(o, kind, "min_depth_m");
+        m.farthest = (float)option(o, kind, "max_depth_m");
+        m.kind = Kind::PointCloud;
+    } else if (kind == "quality") {
+        takes_only(o, kind, {});
+        m.kind = Kind::Quality;
+    } else {
+        throw Error{"no HIP implementation of " + kind};
+    }
+    return m;
+}
+
+static const std::vector<const char *> &member_outputs(Kind k) {
+    static const std::vector<const char *> rect = {"left", "right"}, disp = {"disparity"},
+                                           depth = {"depth_mm"},
+                                           cloud = {"cloud_grid", "cloud_points"},
+                                           quality = {"quality_sums"};
+    switch (k) {
+        case Kind::Rectification: return rect;
+        case Kind::Depth: return depth;
+        case Kind::PointCloud: return cloud;
+        case Kind::Quality: return quality;
+        default: return disp;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Segments: configured stage lists with their device memory and carries.
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) throw Error{std::string("HIP error: ") + hipGetErrorString(e_)}; \
+    } while (0)
+
+struct Slot {
+    uint8_t type;
+    std::vector<uint32_t> shape;
+    void *dev;  // nullptr when the tensor is of a shape or type no stage reads
+};
+
+struct Segment {
+    uint32_t id = 0;
+    Geometry g{};
+    std::vector<Member> members;
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> events;
+    void *arena = nullptr;
+    uint8_t *host = nullptr;  // pinned staging for outputs
+    size_t host_bytes = 0;
+
+    // device buffers carved from the arena
+    uint8_t *frames[2]{}, *images[2]{};
+    float *disparity[2]{};
+    uint16_t *depth_mm = nullptr;
+    float *depth_m = nullptr, *cloud = nullptr, *quality_columns = nullptr, *quality_sums = nullptr;
+    int32_t *cloud_partials = nullptr, *cloud_kept = nullptr;
+    void *match_workspace = nullptr;
+    size_t match_workspace_bytes = 0;
+    std::vector<float *> carry_a, carry_b;  // per member: temporal previous/missed, quality previous
+
+    ~Segment() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto e : events) (void)hipEventDestroy(e);
+        if (arena) (void)hipFree(arena);
+        if (host) (void)hipHostFree(host);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+static size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
+static uint8_t *up256_ptr(uint8_t *p) {
+    return reinterpret_cast<uint8_t *>((reinterpret_cast<uintptr_t>(p) + 255) & ~(uintptr_t)255);
+}
+
+static std::unique_ptr<Segment> configure(uint32_t id, const Geometry &g,
+                                          const std::vector<std::pair<std::string, Options>> &ms) {
+    if (g.width < 2 || g.height < 2)
+        throw Error{"a segment works at 2x2 or more, got " + std::to_string(g.width) + "x" +
+                    std::to_string(g.height)};
+    auto s = std::make_unique<Segment>();
+    s->id = id;
+    s->g = g;
+    for (auto &m : ms) s->members.push_back(make_member(g, m.first, m.second));
+
+    const size_t P = (size_t)g.width * g.height, F = (size_t)g.frame_width * g.frame_height;
+    int max_dec = 0;
+    size_t match = 0, cloud = 0, partials = 0;
+    for (auto &m : s->members) {
+        if (m.kind == Kind::Disparity)
+            match = std::max(match, match_bytes(g.width, g.height, m.disparities, m.paths));
+        if (m.kind == Kind::PointCloud) {
+            max_dec = m.decimation;
+            cloud = std::max(cloud, (size_t)(g.width / m.decimation) * (g.height / m.decimation) * 3);
+            partials = std::max(partials, (size_t)cloud_partials(g.width, g.height, m.decimation));
+        }
+    }
+    (void)max_dec;
+    int carries = 0;
+    for (auto &m : s->members) carries += m.kind == Kind::TemporalFilter ? 2 : (m.kind == Kind::Quality);
+    size_t bytes = up256(F) * 2 + up256(P) * 2 + up256(4 * P) * 2 + up256(2 * P) + up256(4 * P) +
+                   up256(4 * cloud) + up256(4 * (partials + 1)) + up256(4 * (5 * (size_t)g.width + 5)) +
+                   up256(match) + up256(4 * P) * carries + 256;
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIP_TRY(hipMalloc(&s->arena, bytes));
+    uint8_t *p = static_cast<uint8_t *>(s->arena);
+    auto carve = [&](size_t n) {
+        uint8_t *q = p;
+        p += up256(n);
+        return q;
+    };
+    s->frames[0] = carve(F);
+    s->frames[1] = carve(F);
+    s->images[0] = carve(P);
+    s->images[1] = carve(P);
+    s->disparity[0] = reinterpret_cast<float *>(carve(4 * P));
+    s->disparity[1] = reinterpret_cast<float *>(carve(4 * P));
+    s->depth_mm = reinterpret_cast<uint16_t *>(carve(2 * P));
+    s->depth_m = reinterpret_cast<float *>(carve(4 * P));
+    s->cloud = reinterpret_cast<float *>(carve(4 * cloud));
+    s->cloud_partials = reinterpret_cast<int32_t *>(carve(4 * (partials + 1)));
+    s->cloud_kept = s->cloud_partials + partials;
+    s->quality_columns = reinterpret_cast<float *>(carve(4 * (5 * (size_t)g.width + 5)));
+    s->quality_sums = s->quality_columns + 5 * (size_t)g.width;
+    s->match_workspace = carve(match);
+    s->match_workspace_bytes = match;
+    for (auto &m : s->members) {
+        float *a = nullptr, *b = nullptr;
+        if (m.kind == Kind::TemporalFilter) {
+            a = reinterpret_cast<float *>(carve(4 * P));
+            b = reinterpret_cast<float *>(carve(4 * P));
+            HIP_TRY(launch_fill_f32(a, -1.0f, P, s->stream));  // previous: unmatched
+            HIP_TRY(launch_fill_f32(b, 0.0f, P, s->stream));   // missed: none
+        } else if (m.kind == Kind::Quality) {
+            a = reinterpret_cast<float *>(carve(4 * P));
+            HIP_TRY(launch_fill_f32(a, -1.0f, P, s->stream));
+        }
+        s->carry_a.push_back(a);
+        s->carry_b.push_back(b);
+    }
+    s->events.resize(s->members.size() + 1);
+    for (auto &e : s->events) HIP_TRY(hipEventCreate(&e));
+    // Pinned staging large enough for every output of every member.
+    // left and right (u8), disparity (f32), depth_mm (u16), the cloud and two small tensors,
+    // each span rounded up to 16 bytes.
+    s->host_bytes = 2 * P + 4 * P + 2 * P + 4 * cloud + 256;
+    HIP_TRY(hipHostMalloc((void **)&s->host, s->host_bytes));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return s;
+}
+
+static const std::vector<float> &input(const Inputs &in, const std::string &name, size_t length) {
+    auto it = in.find(name);
+    if (it == in.end()) throw Error{"no " + name + " among the stage's inputs"};
+    if (it->second.size() != length)
+        throw Error{name + " has " + std::to_string(it->second.size()) + " values, expected " +
+                    std::to_string(length)};
+    return it->second;
+}
+
+static float scalar(const Inputs &in, const std::string &name) { return input(in, name, 1)[0]; }
+
+// segment.rs:146-156: a penalty is an integer byte.
+static int penalty(const Inputs &in, const std::string &name) {
+    float v = scalar(in, name);
+    if (v == std::floor(v) && v >= 0.0f && v <= 255.0f) return (int)v;
+    std::ostringstream s;
+    s << name << " must be an integer from 0 to 255, got " << v;
+    throw Error{s.str()};
+}
+
+static Camera camera(const Inputs &in, const std::string &side) {
+    Camera c;
+    std::memcpy(c.k, input(in, side + ".k", 9).data(), 36);
+    std::memcpy(c.d, input(in, side + ".d", 5).data(), 20);
+    std::memcpy(c.inverse_pr, input(in, side + ".inverse_pr", 9).data(), 36);
+    return c;
+}
+
+// tensor.rs:123-183: the tensor `name` of `type` and `shape`, or why not.
+static void *shaped(const std::map<std::string, Slot> &tensors, const std::string &name,
+                    uint8_t type, std::vector<uint32_t> shape, const char *what) {
+    auto it = tensors.find(name);
+    if (it == tensors.end()) throw Error{"no " + name + " among the segment's tensors"};
+    const Slot &s = it->second;
+    if (s.shape != shape || s.type != type || !s.dev)
+        throw Error{name + " is a " + shape_text(s.shape != shape ? s.shape : shape) +
+                    " tensor of type " + std::to_string(s.type) + ", expected a " +
+                    shape_text(shape) + " " + what};
+    return s.dev;
+}
+
+// An entry already in device memory (nxs_run_device).
+struct DeviceEntry {
+    std::string name;
+    uint8_t type;
+    std::vector<uint32_t> shape;
+    void *data;
+};
+
+struct Enqueued {
+    std::map<std::string, Slot> tensors;
+    std::vector<const char *> filled;
+};
+
+// Enqueues every member on `stream`: entries go to the segment's buffers (device entries
+// a stage only reads are read in place), then each stage in order (segment.rs:206-239).
+static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
+                        const std::vector<std::pair<std::string, HostTensor>> &entries,
+                        const std::vector<DeviceEntry> &device_entries, hipStream_t stream) {
+    if (inputs.size() != s.members.size())
+        throw Error{std::to_string(s.members.size()) + " stages, got the inputs of " +
+                    std::to_string(inputs.size())};
+    const Geometry &g = s.g;
+    const uint32_t W = g.width, H = g.height;
+    const size_t P = (size_t)W * H;
+    const std::vector<uint32_t> image = {H, W},
+                                frame = {(uint32_t)g.frame_height, (uint32_t)g.frame_width};
+    Enqueued q;
+    std::map<std::string, Slot> &tensors = q.tensors;
+    int free_disparity = 0;
+
+    // Where an entry a stage can read goes, or nullptr.
+    auto target_of = [&](const std::string &name, uint8_t type, const std::vector<uint32_t> &shape) -> void * {
+        if ((name == "left_frame" || name == "right_frame") && type == TYPE_U8 && shape == frame)
+            return s.frames[name == "right_frame"];
+        if ((name == "left" || name == "right") && type == TYPE_U8 && shape == image)
+            return s.images[name == "right"];
+        if (name == "disparity" && type == TYPE_F32 && shape == image) return s.disparity[0];
+        if (name == "depth_m" && type == TYPE_F32 && shape == image) return s.depth_m;
+        return nullptr;
+    };
+    for (auto &e : entries) {
+        const HostTensor &t = e.second;
+        Slot slot{t.type, t.shape, nullptr};
+        if (void *target = target_of(e.first, t.type, t.shape)) {
+            HIP_TRY(hipMemcpyAsync(target, t.bytes, t.size, hipMemcpyHostToDevice, stream));
+            slot.dev = target;
+            if (e.first == "disparity") free_disparity = 1;
+        }
+        tensors[e.first] = slot;
+    }
+    for (auto &e : device_entries) {
+        Slot slot{e.type, e.shape, nullptr};
+        if (void *target = target_of(e.name, e.type, e.shape)) {
+            if (e.name == "disparity") {
+                // Filters may change the disparity in place: work on the segment's copy.
+                HIP_TRY(hipMemcpyAsync(target, e.data, 4 * P, hipMemcpyDeviceToDevice, stream));
+                slot.dev = target;
+                free_disparity = 1;
+            } else {
+                slot.dev = e.data;
+            }
+        }
+        tensors[e.name] = slot;
+    }
+
+    std::vector<const char *> &filled = q.filled;
+    HIP_TRY(hipEventRecord(s.events[0], stream));
+    for (size_t i = 0; i < s.members.size(); i++) {
+        const Member &m = s.members[i];
+        const Inputs &in = inputs[i];
+        switch (m.kind) {
+            case Kind::Rectification: {
+                Camera left = camera(in, "left"), right = camera(in, "right");
+                auto *lf = (const uint8_t *)shaped(tensors, "left_frame", TYPE_U8, frame, "u8 image");
+                auto *rf = (const uint8_t *)shaped(tensors, "right_frame", TYPE_U8, frame, "u8 image");
+                HIP_TRY(launch_rectify(lf, rf, g.frame_width, g.frame_height, left, right, W, H,
+                                       s.images[0], s.images[1], stream));
+                tensors["left"] = Slot{TYPE_U8, image, s.images[0]};
+                tensors["right"] = Slot{TYPE_U8, image, s.images[1]};
+                break;
+            }
+            case Kind::Disparity: {
+                const int p1 = penalty(in, "p1"), p2 = penalty(in, "p2");
+                const float u = scalar(in, "uniqueness");
+                check_matching(W, H, m.disparities, m.paths, p1, p2, u);
+                auto *l = (const uint8_t *)shaped(tensors, "left", TYPE_U8, image, "u8 image");
+                auto *r = (const uint8_t *)shaped(tensors, "right", TYPE_U8, image, "u8 image");
+                float *out = s.disparity[free_disparity];
+                MatchBuffers b;
+                match_carve(b, s.match_workspace, W, H, m.disparities, m.paths);
+                MatchParams mp{(int)W, (int)H, m.disparities, m.paths, p1, p2, u};
+                HIP_TRY(launch_match(mp, l, r, b, out, nullptr, stream));
+                tensors["disparity"] = Slot{TYPE_F32, image, out};
+                free_disparity ^= 1;
+                break;
+            }
+            case Kind::SpeckleFilter: {
+                const float support = scalar(in, "min_support"), diff = scalar(in, "max_diff");
+                auto *d = (const float *)shaped(tensors, "disparity", TYPE_F32, image, "f32 map");
+                float *out = s.disparity[0] == d ? s.disparity[1] : s.disparity[0];
+                HIP_TRY(launch_despeckle(d, out, W, H, support, diff, m.radius, stream));
+                tensors["disparity"] = Slot{TYPE_F32, image, out};
+                free_disparity = s.disparity[0] == out ? 1 : 0;
+                break;
+            }
+            case Kind::TemporalFilter: {
+                const float alpha = scalar(in, "alpha"), delta = scalar(in, "delta"),
+                            hold = scalar(in, "hold");
+                auto *d = (float *)shaped(tensors, "disparity", TYPE_F32, image, "f32 map");
+                HIP_TRY(launch_temporal(d, s.carry_a[i], s.carry_b[i], (int)P, alpha, delta, hold,
+                                        m.max_missed, stream));
+                break;
+            }
+            case Kind::Depth: {
+                const float f = scalar(in, "focal_length_px"), b = scalar(in, "baseline_m");
+                auto *d = (const float *)shaped(tensors, "disparity", TYPE_F32, image, "f32 map");
+                HIP_TRY(launch_depth(d, (int)P, f, b, s.depth_mm, s.depth_m, stream));
+                tensors["depth_mm"] = Slot{TYPE_U16, image, s.depth_mm};
+                tensors["depth_m"] = Slot{TYPE_F32, image, s.depth_m};
+                break;
+            }
+            case Kind::PointCloud: {
+                const float f = scalar(in, "focal_length_px"), cx = scalar(in, "cx"),
+                            cy = scalar(in, "cy");
+                auto *d = (const float *)shaped(tensors, "depth_m", TYPE_F32, image, "f32 map");
+                HIP_TRY(launch_cloud(d, W, H, m.decimation, f, cx, cy, m.nearest, m.farthest,
+                                     s.cloud, s.cloud_partials, s.cloud_kept, stream));
+                tensors["cloud_grid"] = Slot{TYPE_F32, {H / m.decimation, W / m.decimation, 3}, s.cloud};
+                tensors["cloud_points"] = Slot{TYPE_S32, {}, s.cloud_kept};
+                break;
+            }
+            case Kind::Quality: {
+                auto *d = (const float *)shaped(tensors, "disparity", TYPE_F32, image, "f32 map");
+                HIP_TRY(launch_quality(d, s.carry_a[i], W, H, s.quality_columns, s.quality_sums,
+                                       stream));
+                tensors["quality_sums"] = Slot{TYPE_F32, {5}, s.quality_sums};
+                break;
+            }
+        }
+        HIP_TRY(hipEventRecord(s.events[i + 1], stream));
+        for (const char *name : member_outputs(m.kind)) {
+            bool seen = false;
+            for (const char *f : filled) seen = seen || std::strcmp(f, name) == 0;
+            if (!seen) filled.push_back(name);
+        }
+    }
+
+    return q;
+}
+
+static std::vector<uint8_t> run(Segment &s, const std::vector<Inputs> &inputs,
+                                const std::vector<std::pair<std::string, HostTensor>> &entries) {
+    Enqueued q = enqueue(s, inputs, entries, {}, s.stream);
+    std::map<std::string, Slot> &tensors = q.tensors;
+    const std::vector<const char *> &filled = q.filled;
+
+    // Downloads, then the reply: tag, member times, then each filled tensor.
+    size_t offset = 0;
+    std::vector<std::pair<size_t, size_t>> spans;
+    for (const char *name : filled) {
+        const Slot &slot = tensors[name];
+        size_t elements = 1;
+        for (uint32_t d : slot.shape) elements *= d;
+        const size_t bytes = elements * element_bytes(slot.type);
+        if (offset + bytes > s.host_bytes) throw Error{"output staging too small"};
+        HIP_TRY(hipMemcpyAsync(s.host + offset, slot.dev, bytes, hipMemcpyDeviceToHost, s.stream));
+        spans.emplace_back(offset, bytes);
+        offset += (bytes + 15) & ~(size_t)15;
+    }
+    HIP_TRY(hipStreamSynchronize(s.stream));
+
+    std::vector<uint8_t> reply;
+    reply.push_back(TAG_OUTPUTS);
+    reply.push_back((uint8_t)s.members.size());
+    for (size_t i = 0; i < s.members.size(); i++) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, s.events[i], s.events[i + 1]));
+        const uint32_t us = (uint32_t)std::min(4294967295.0, std::floor((double)ms * 1000.0));
+        for (int k = 0; k < 4; k++) reply.push_back((uint8_t)(us >> (8 * k)));
+    }
+    reply.push_back((uint8_t)filled.size());
+    for (size_t k = 0; k < filled.size(); k++) {
+        const Slot &slot = tensors[filled[k]];
+        const std::string name = filled[k];
+        reply.push_back((uint8_t)name.size());
+        reply.insert(reply.end(), name.begin(), name.end());
+        reply.push_back(slot.type);
+        reply.push_back((uint8_t)slot.shape.size());
+        for (uint32_t d : slot.shape)
+            for (int b = 0; b < 4; b++) reply.push_back((uint8_t)(d >> (8 * b)));
+        reply.insert(reply.end(), s.host + spans[k].first, s.host + spans[k].first + spans[k].second);
+    }
+    return reply;
+}
+
+// ------------------------------------------------------------------------------------------
+// The context: a queue of requests, one thread computing them in order.
+
+struct Job {
+    int64_t ticket;
+    std::vector<uint8_t> record;
+    nxs_done_fn done;
+    void *user;
+};
+
+struct Kept {
+    bool complete = false;
+    std::vector<uint8_t> reply;
+};
+
+}  // namespace nxs
+
+struct nxs_ctx {
+    int device = 0;
+    std::mutex mu;
+    std::condition_variable cv_jobs, cv_done;
+    std::deque<nxs::Job> jobs;
+    std::unordered_map<int64_t, nxs::Kept> kept;
+    std::map<uint32_t, std::unique_ptr<nxs::Segment>> segments;
+    int64_t next_ticket = 1;
+    bool stopping = false;
+    std::thread worker;
+    std::mutex segments_mu;  // the segments, between the context thread and nxs_run_device
+
+    std::vector<uint8_t> handle(const std::vector<uint8_t> &record) {
+        using namespace nxs;
+        Request q;
+        try {
+            q = parse(record.data(), record.size());
+        } catch (const Error &e) {
+            return refused_record(e.reason);
+        }
+        try {
+            if (q.kind == CONFIGURE) {
+                // main.rs:146-160: a refused (re)configuration leaves the segment unconfigured.
+                segments.erase(q.segment);
+                segments[q.segment] = configure(q.segment, q.geometry, q.members);
+                return {TAG_CONFIGURED};
+            }
+            if (q.kind == RELEASE) {
+                segments.erase(q.segment);
+                return {};
+            }
+            auto it = segments.find(q.segment);
+            if (it == segments.end())
+                return refused_record("segment " + std::to_string(q.segment) + " is not configured");
+            return run(*it->second, q.inputs, q.entries);
+        } catch (const Error &e) {
+            if (q.kind == CONFIGURE) segments.erase(q.segment);
+            return refused_record(e.reason);
+        } catch (const std::bad_alloc &) {
+            if (q.kind == CONFIGURE) segments.erase(q.segment);
+            return refused_record("out of host memory");
+        }
+    }
+
+    void loop() {
+        (void)hipSetDevice(device);
+        for (;;) {
+            nxs::Job job;
+            {
+                std::unique_lock<std::mutex> lock(mu);
+                cv_jobs.wait(lock, [&] { return stopping || !jobs.empty(); });
+                if (jobs.empty()) return;
+                job = std::move(jobs.front());
+                jobs.pop_front();
+            }
+            std::vector<uint8_t> reply;
+            {
+                std::lock_guard<std::mutex> lock(segments_mu);
+                reply = handle(job.record);
+            }
+            if (job.done) {
+                job.done(job.user, job.ticket, reply.data(), reply.size());
+            } else {
+                std::lock_guard<std::mutex> lock(mu);
+                auto &k = kept[job.ticket];
+                k.reply = std::move(reply);
+                k.complete = true;
+                cv_done.notify_all();
+            }
+        }
+    }
+};
+
+extern "C" {
+
+int nxs_open(int device, nxs_ctx **out, char *err, size_t errlen) {
+    auto report = [&](const std::string &why) {
+        if (err && errlen) {
+            std::strncpy(err, why.c_str(), errlen - 1);
+            err[errlen - 1] = 0;
+        }
+        nxs::g_last_error = why;
+    };
+    if (!out) {
+        report("no output pointer");
+        return NXS_EINVAL;
+    }
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || device < 0 || device >= count) {
+        report(std::string("no HIP device ") + std::to_string(device) +
+               (e != hipSuccess ? std::string(": ") + hipGetErrorString(e) : std::string()));
+        return NXS_EDEVICE;
+    }
+    e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        report(std::string("hipSetDevice: ") + hipGetErrorString(e));
+        return NXS_EDEVICE;
+    }
+    auto *ctx = new nxs_ctx();
+    ctx->device = device;
+    ctx->worker = std::thread([ctx] { ctx->loop(); });
+    *out = ctx;
+    return NXS_OK;
+}
+
+void nxs_close(nxs_ctx *ctx) {
+    if (!ctx) return;
+    std::deque<nxs::Job> pending;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        ctx->stopping = true;
+        pending.swap(ctx->jobs);
+    }
+    ctx->cv_jobs.notify_all();
+    if (ctx->worker.joinable()) ctx->worker.join();
+    const std::vector<uint8_t> closed = nxs::refused_record("the context is closed");
+    for (auto &job : pending)
+        if (job.done) job.done(job.user, job.ticket, closed.data(), closed.size());
+    (void)hipSetDevice(ctx->device);
+    ctx->segments.clear();
+    delete ctx;
+}
+
+int64_t nxs_submit(nxs_ctx *ctx, const uint8_t *record, size_t len, nxs_done_fn done, void *user) {
+    if (!ctx || (!record && len)) return nxs::fail(NXS_EINVAL, "no context or record");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->stopping) return nxs::fail(NXS_EINVAL, "the context is closing");
+    const int64_t ticket = ctx->next_ticket++;
+    ctx->jobs.push_back(nxs::Job{ticket, std::vector<uint8_t>(record, record + len), done, user});
+    if (!done) ctx->kept[ticket];
+    ctx->cv_jobs.notify_one();
+    return ticket;
+}
+
+int nxs_wait(nxs_ctx *ctx, int64_t ticket, int timeout_ms) {
+    if (!ctx) return NXS_EINVAL;
+    std::unique_lock<std::mutex> lock(ctx->mu);
+    auto it = ctx->kept.find(ticket);
+    if (it == ctx->kept.end()) return nxs::fail(NXS_EINVAL, "unknown ticket");
+    auto ready = [&] { return ctx->kept[ticket].complete; };
+    if (timeout_ms < 0) {
+        ctx->cv_done.wait(lock, ready);
+        return NXS_OK;
+    }
+    return ctx->cv_done.wait_for(lock, std::chrono::milliseconds(timeout_ms), ready) ? NXS_OK
+                                                                                       : NXS_TIMEOUT;
+}
+
+int nxs_reply(nxs_ctx *ctx, int64_t ticket, const uint8_t **reply, size_t *len) {
+    if (!ctx || !reply || !len) return NXS_EINVAL;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    auto it = ctx->kept.find(ticket);
+    if (it == ctx->kept.end() || !it->second.complete) return nxs::fail(NXS_EINVAL, "no reply");
+    *reply = it->second.reply.data();
+    *len = it->second.reply.size();
+    return NXS_OK;
+}
+
+void nxs_release_reply(nxs_ctx *ctx, int64_t ticket) {
+    if (!ctx) return;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    auto it = ctx->kept.find(ticket);
+    if (it != ctx->kept.end() && it->second.complete) ctx->kept.erase(it);
+}
+
+int nxs_poll(nxs_ctx *ctx, int max) {
+    if (!ctx) return NXS_EINVAL;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int n = 0;
+    for (auto &kv : ctx->kept)
+        if (kv.second.complete && (max <= 0 || n < max)) n++;
+    return n;
+}
+
+size_t nxs_match_workspace_size(int width, int height, int disparities, int paths) {
+    return nxs::match_bytes(width, height, disparities, paths);
+}
+
+int nxs_match_device(const uint8_t *left, const uint8_t *right, int width, int height,
+                     int disparities, int paths, int p1, int p2, float uniqueness,
+                     float *disparity, void *workspace, size_t workspace_bytes, void *stream) {
+    try {
+        nxs::check_matching(width, height, disparities, paths, p1, p2, uniqueness);
+    } catch (const nxs::Error &e) {
+        return nxs::fail(NXS_EINVAL, e.reason);
+    }
+    if (p1 < 0) return nxs::fail(NXS_EINVAL, "p1 must be at least 0");
+    if (!left || !right || !disparity || !workspace)
+        return nxs::fail(NXS_EINVAL, "a device pointer is null");
+    if (workspace_bytes < nxs::match_bytes(width, height, disparities, paths))
+        return nxs::fail(NXS_EINVAL, "the workspace is smaller than nxs_match_workspace_size");
+    nxs::MatchBuffers b;
+    nxs::match_carve(b, workspace, width, height, disparities, paths);
+    nxs::MatchParams mp{width, height, disparities, paths, p1, p2, uniqueness};
+    hipError_t e = nxs::launch_match(mp, left, right, b, disparity, nullptr, (hipStream_t)stream);
+    if (e != hipSuccess) return nxs::fail(NXS_EDEVICE, hipGetErrorString(e));
+    return NXS_OK;
+}
+
+int nxs_match_host(const uint8_t *left, const uint8_t *right, int width, int height,
+                   int disparities, int paths, int p1, int p2, float uniqueness,
+                   uint64_t *census_left, uint64_t *census_right, uint8_t *costs, uint16_t *sums,
+                   float *selected, uint32_t *keys, float *checked, float *disparity) {
+    try {
+        nxs::check_matching(width, height, disparities, paths, p1, p2, uniqueness);
+    } catch (const nxs::Error &e) {
+        return nxs::fail(NXS_EINVAL, e.reason);
+    }
+    if (p1 < 0) return nxs::fail(NXS_EINVAL, "p1 must be at least 0");
+    if (!left || !right) return nxs::fail(NXS_EINVAL, "no input image");
+    const size_t P = (size_t)width * height, N = P * disparities;
+    const size_t ws = nxs::match_bytes(width, height, disparities, paths);
+    uint8_t *dev = nullptr;
+    const size_t total = ws + 2 * P + 4 * P + 2 * N + 1024;
+    hipError_t e = hipMalloc(&dev, total);
+    if (e != hipSuccess) return nxs::fail(NXS_ENOMEM, hipGetErrorString(e));
+    uint8_t *d_left = dev + ws, *d_right = d_left + P;
+    float *d_out = reinterpret_cast<float *>(nxs::up256_ptr(d_right + P));
+    uint16_t *d_sums = reinterpret_cast<uint16_t *>(nxs::up256_ptr((uint8_t *)(d_out + P)));
+    nxs::MatchBuffers b;
+    nxs::match_carve(b, dev, width, height, disparities, paths);
+    nxs::MatchParams mp{width, height, disparities, paths, p1, p2, uniqueness};
+    int rc = NXS_OK;
+    auto check = [&](hipError_t err) {
+        if (err != hipSuccess && rc == NXS_OK) rc = nxs::fail(NXS_EDEVICE, hipGetErrorString(err));
+    };
+    check(hipMemcpy(d_left, left, P, hipMemcpyHostToDevice));
+    check(hipMemcpy(d_right, right, P, hipMemcpyHostToDevice));
+    if (rc == NXS_OK) check(nxs::launch_match(mp, d_left, d_right, b, d_out, sums ? d_sums : nullptr, nullptr));
+    check(hipDeviceSynchronize());
+    auto down = [&](void *host, const void *devp, size_t n) {
+        if (host && rc == NXS_OK) check(hipMemcpy(host, devp, n, hipMemcpyDeviceToHost));
+    };
+    down(census_left, b.census_left, 8 * P);
+    down(census_right, b.census_right, 8 * P);
+    down(costs, b.costs, N);
+    down(sums, d_sums, 2 * N);
+    down(selected, b.selected, 4 * P);
+    down(keys, b.keys, 4 * P);
+    down(checked, b.checked, 4 * P);
+    down(disparity, d_out, 4 * P);
+    (void)hipFree(dev);
+    return rc;
+}
+
+int nxs_run_device(nxs_ctx *ctx, const uint8_t *record, size_t len, const nxs_tensor *entries,
+                   int n_entries, nxs_tensor *outputs, int max_outputs, int *n_outputs,
+                   void *stream) {
+    using namespace nxs;
+    if (!ctx || !record || (n_entries > 0 && !entries) || !n_outputs)
+        return fail(NXS_EINVAL, "no context, record or output count");
+    std::lock_guard<std::mutex> lock(ctx->segments_mu);
+    try {
+        Request q = parse(record, len);
+        if (q.kind != RUN) return fail(NXS_EINVAL, "nxs_run_device takes a run record");
+        auto it = ctx->segments.find(q.segment);
+        if (it == ctx->segments.end())
+            return fail(NXS_EINVAL, "segment " + std::to_string(q.segment) + " is not configured");
+        Segment &seg = *it->second;
+        std::vector<DeviceEntry> device;
+        for (int e = 0; e < n_entries; e++) {
+            const nxs_tensor &t = entries[e];
+            if (!t.name || t.rank > 4) return fail(NXS_EINVAL, "an entry has no name or rank > 4");
+            device.push_back(DeviceEntry{t.name, t.type, std::vector<uint32_t>(t.shape, t.shape + t.rank), t.data});
+        }
+        hipStream_t s = stream ? (hipStream_t)stream : seg.stream;
+        Enqueued out = enqueue(seg, q.inputs, q.entries, device, s);
+        *n_outputs = (int)out.filled.size();
+        for (int k = 0; k < (int)out.filled.size() && k < max_outputs; k++) {
+            const Slot &slot = out.tensors[out.filled[k]];
+            nxs_tensor &t = outputs[k];
+            t.name = out.filled[k];
+            t.type = slot.type;
+            t.rank = (uint8_t)slot.shape.size();
+            for (int d = 0; d < 4; d++) t.shape[d] = d < t.rank ? slot.shape[d] : 0;
+            t.data = slot.dev;
+        }
+        return NXS_OK;
+    } catch (const Error &e) {
+        return fail(NXS_EINVAL, e.reason);
+    }
+}
+
+int nxs_profile_start(int capacity) {
+    using namespace nxs;
+    std::lock_guard<std::mutex> lock(g_profiler.mu);
+    while ((int)g_profiler.begin.size() < capacity) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+            return fail(NXS_EDEVICE, "hipEventCreate failed");
+        g_profiler.begin.push_back(a);
+        g_profiler.end.push_back(b);
+        g_profiler.kernel.push_back(0);
+    }
+    g_profiler.used = 0;
+    g_profiler.on = true;
+    return NXS_OK;
+}
+
+int nxs_profile_stop(double *total_ms, int *launches, int kernels) {
+    using namespace nxs;
+    std::lock_guard<std::mutex> lock(g_profiler.mu);
+    g_profiler.on = false;
+    for (int k = 0; k < kernels; k++) {
+        if (total_ms) total_ms[k] = 0;
+        if (launches) launches[k] = 0;
+    }
+    for (int i = 0; i < g_profiler.used; i++) {
+        if (hipEventSynchronize(g_profiler.end[i]) != hipSuccess)
+            return fail(NXS_EDEVICE, "hipEventSynchronize failed");
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, g_profiler.begin[i], g_profiler.end[i]) != hipSuccess)
+            return fail(NXS_EDEVICE, "hipEventElapsedTime failed");
+        const int k = g_profiler.kernel[i];
+        if (k < kernels) {
+            if (total_ms) total_ms[k] += ms;
+            if (launches) launches[k]++;
+        }
+    }
+    g_profiler.used = 0;
+    return NXS_OK;
+}
+
+const char *nxs_last_error(void) { return nxs::g_last_error.c_str(); }
+
+}  // extern "C"

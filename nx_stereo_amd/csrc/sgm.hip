@@ -1,0 +1,606 @@
+// Semi-global matching on gfx950: census, Hamming costs, 8-path min-plus
+// aggregation, winner selection, left-right check and the 3x3 median.
+//
+// Semantics follow NxStereo.sgm_n (lib/nx_stereo.ex:475-683) as its
+// step-by-step transcription states it (native/stereo_match/src/reference.rs).
+//
+// Layout in HBM: images u8 {H,W}; census u64 {H,W}; volumes pixel-major,
+// volume[(y*W + x)*D + d], so the D values of one pixel are contiguous and a
+// 16-lane DPP row reads them in one coalesced access.
+//
+// Aggregation works per line orientation: rows (L->R, R->L), columns (down,
+// up), x-y diagonals (down-right, up-left) and x+y diagonals (down-left,
+// up-right). Each line is independent and owned by one 16-lane DPP row: lane
+// i holds disparities [i*K, i*K+K) as K/2 packed u16 pairs, neighbours d-1
+// and d+1 across lanes come from row_shr/row_shl DPP moves, and the minimum
+// over D is a 4-step row_ror DPP reduction. The forward direction writes its
+// byte path costs into the orientation's volume; the backward direction reads
+// them back and writes the byte sum of both (each <= p2+48 <= 127, so the
+// sum fits a byte). Selection sums the orientations.
+#include "device.hpp"
+#include "kernels.hpp"
+#include "profile.hpp"
+
+namespace nxs {
+
+// ------------------------------------------------------------------------------------------
+// Census (reference.rs:20-38; nx_stereo.ex:491-509): bit k, row-major over
+// dy in (-3..3 without 0) x dx in (-4..4 without 0), is set iff the clamped
+// neighbour + 1 < the centre.
+
+__global__ __launch_bounds__(256) void census_kernel(const uint8_t *__restrict__ left,
+                                                     const uint8_t *__restrict__ right, int W,
+                                                     int H, uint64_t *__restrict__ cl,
+                                                     uint64_t *__restrict__ cr) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const uint8_t *img = blockIdx.z ? right : left;
+    uint64_t *out = blockIdx.z ? cr : cl;
+    if (x >= W) return;
+    const int centre = img[(size_t)y * W + x];
+    uint64_t f = 0;
+    int bit = 0;
+#pragma unroll
+    for (int dy = -3; dy <= 3; dy++) {
+        if (dy == 0) continue;
+        const int sy = min(max(y + dy, 0), H - 1);
+        const uint8_t *row = img + (size_t)sy * W;
+#pragma unroll
+        for (int dx = -4; dx <= 4; dx++) {
+            if (dx == 0) continue;
+            const int sx = min(max(x + dx, 0), W - 1);
+            f |= (uint64_t)((int)row[sx] + 1 < centre) << bit;
+            bit++;
+        }
+    }
+    out[(size_t)y * W + x] = f;
+}
+
+// ------------------------------------------------------------------------------------------
+// Costs (reference.rs:41-63): C[y,x,d] = popcount(FL[y,x] ^ FR[y,x-d]), FR = 0 left of
+// the image. One thread per pixel and 16 disparities, one 16-byte store.
+
+__global__ __launch_bounds__(256) void cost_kernel(const uint64_t *__restrict__ cl,
+                                                   const uint64_t *__restrict__ cr, int W,
+                                                   int pixels, int D,
+                                                   uint8_t *__restrict__ costs) {
+    const int chunks = D >> 4;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long pix = t / chunks;
+    if (pix >= pixels) return;
+    const int chunk = (int)(t - pix * chunks);
+    const int x = (int)(pix % W);
+    const uint64_t l = cl[pix];
+    uint32_t words[4];
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int d = chunk * 16 + w * 4 + j;
+            const uint64_t r = x >= d ? cr[pix - d] : 0;
+            word |= (uint32_t)__builtin_popcountll(l ^ r) << (8 * j);
+        }
+        words[w] = word;
+    }
+    *reinterpret_cast<uint4 *>(costs + pix * D + chunk * 16) =
+        make_uint4(words[0], words[1], words[2], words[3]);
+}
+
+// ------------------------------------------------------------------------------------------
+// Path recurrence (reference.rs:65-121; paths.rs:19-42), on a 16-lane row.
+
+template <int K>
+struct Bytes;  // K bytes of one lane, loaded and stored as whole dwords
+template <>
+struct Bytes<4> {
+    uint32_t w[1];
+    __device__ void load(const uint8_t *p) { w[0] = *reinterpret_cast<const uint32_t *>(p); }
+    __device__ void store(uint8_t *p) const { *reinterpret_cast<uint32_t *>(p) = w[0]; }
+};
+template <>
+struct Bytes<8> {
+    uint32_t w[2];
+    __device__ void load(const uint8_t *p) {
+        uint2 v = *reinterpret_cast<const uint2 *>(p);
+        w[0] = v.x;
+        w[1] = v.y;
+    }
+    __device__ void store(uint8_t *p) const { *reinterpret_cast<uint2 *>(p) = make_uint2(w[0], w[1]); }
+};
+template <>
+struct Bytes<12> {
+    uint32_t w[3];
+    __device__ void load(const uint8_t *p) {
+        const uint32_t *q = reinterpret_cast<const uint32_t *>(p);
+        w[0] = q[0];
+        w[1] = q[1];
+        w[2] = q[2];
+    }
+    __device__ void store(uint8_t *p) const {
+        uint32_t *q = reinterpret_cast<uint32_t *>(p);
+        q[0] = w[0];
+        q[1] = w[1];
+        q[2] = w[2];
+    }
+};
+template <>
+struct Bytes<16> {
+    uint32_t w[4];
+    __device__ void load(const uint8_t *p) {
+        uint4 v = *reinterpret_cast<const uint4 *>(p);
+        w[0] = v.x;
+        w[1] = v.y;
+        w[2] = v.z;
+        w[3] = v.w;
+    }
+    __device__ void store(uint8_t *p) const {
+        *reinterpret_cast<uint4 *>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+};
+
+// Bytes b0 b1 b2 b3 of a dword as the u16 pairs (b0, b1) and (b2, b3).
+// v_perm_b32 (S0, S1, sel): result byte i is byte sel[i] of {S0:S1}, 0-3 from S1, 4-7 from
+// S0, and 0x0C reads zero.
+__device__ __forceinline__ uint32_t lo_pair(uint32_t w) { return __builtin_amdgcn_perm(0u, w, 0x0C010C00u); }
+__device__ __forceinline__ uint32_t hi_pair(uint32_t w) { return __builtin_amdgcn_perm(0u, w, 0x0C030C02u); }
+// Two u16 pairs (each value < 256) as one dword of bytes.
+__device__ __forceinline__ uint32_t pack_pairs(uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_perm(b, a, 0x06040200u);
+}
+
+template <int K>
+struct PathState {
+    static constexpr int R = K / 2;
+    uint32_t v[R];  // disparities (2k, 2k+1) of this lane's K in the low and high halves
+    uint32_t pm;    // the minimum over the line's D values, same in all 16 lanes
+
+    __device__ void reset() {
+#pragma unroll
+        for (int k = 0; k < R; k++) v[k] = 0;
+        pm = 0;
+    }
+
+    // One step of L(p,d) = min(a[d], p2, min(a[d-1], a[d+1]) + p1) + C(p,d) with
+    // a = prev - min(prev), written unnormalised: min(prev[d], pm + p2,
+    // min(prev[d-1], prev[d+1]) + p1) - pm + C. Out-of-range neighbours are a
+    // sentinel that only ever loses to the pm + p2 term, as libSGM's p2 - p1
+    // padding does.
+    __device__ __forceinline__ void step(const Bytes<K> &cost, uint32_t p1x2, uint32_t p2,
+                                         int lane16) {
+        constexpr uint32_t SENT = 0x3FFF3FFFu;
+        uint32_t left_in = dpp<DPP_ROW_SHR1>(v[R - 1]);
+        uint32_t right_in = dpp<DPP_ROW_SHL1>(v[0]);
+        left_in = lane16 == 0 ? SENT : left_in;
+        right_in = lane16 == 15 ? SENT : right_in;
+        uint32_t s[R + 1];
+        s[0] = (v[0] << 16) | (left_in >> 16);
+#pragma unroll
+        for (int k = 1; k < R; k++) s[k] = (v[k] << 16) | (v[k - 1] >> 16);
+        s[R] = (right_in << 16) | (v[R - 1] >> 16);
+        const uint32_t PM = pm * 0x10001u;
+        const uint32_t JUMP = (pm + p2) * 0x10001u;
+        uint32_t c[R];
+#pragma unroll
+        for (int k = 0; k < R / 2; k++) {
+            c[2 * k] = lo_pair(cost.w[k]);
+            c[2 * k + 1] = hi_pair(cost.w[k]);
+        }
+        uint32_t mn = 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            uint32_t t = pk_min(s[k], s[k + 1]);
+            t = pk_add(t, p1x2);
+            t = pk_min(t, v[k]);
+            t = pk_min(t, JUMP);
+            t = pk_add(pk_sub(t, PM), c[k]);
+            v[k] = t;
+            mn = pk_min(mn, t);
+        }
+        uint32_t m = min(mn & 0xFFFFu, mn >> 16);
+        pm = row_min_u32(m);
+    }
+
+    __device__ __forceinline__ Bytes<K> bytes() const {
+        Bytes<K> b;
+#pragma unroll
+        for (int k = 0; k < R / 2; k++) b.w[k] = pack_pairs(v[2 * k], v[2 * k + 1]);
+        return b;
+    }
+};
+
+struct LineGeometry {
+    int y0, x0, length;
+};
+
+// Line i of orientation o, its first pixel in the forward direction and its length.
+__device__ __forceinline__ LineGeometry line_geometry(int o, int i, int W, int H) {
+    LineGeometry g;
+    if (o == 0) {  // row i, forward L->R
+        g.y0 = i;
+        g.x0 = 0;
+        g.length = i < H ? W : 0;
+    } else if (o == 1) {  // column i, forward down
+        g.y0 = 0;
+        g.x0 = i;
+        g.length = i < W ? H : 0;
+    } else if (o == 2) {  // x - y = i - (H-1), forward down-right
+        const int c = i - (H - 1);
+        g.y0 = max(0, -c);
+        g.x0 = g.y0 + c;
+        g.length = i < W + H - 1 ? min(H - g.y0, W - g.x0) : 0;
+    } else {  // x + y = i, forward down-left
+        g.y0 = max(0, i - (W - 1));
+        g.x0 = i - g.y0;
+        g.length = i < W + H - 1 ? min(H - g.y0, g.x0 + 1) : 0;
+    }
+    return g;
+}
+
+struct PathArgs {
+    const uint8_t *costs;
+    uint8_t *paired[4];
+    int W, H, p1, p2;
+    int backward_columns;  // 0 with 3 paths: columns go down only
+    int block_start[4];    // first block of each orientation
+};
+
+// Lines per block: one wave, four 16-lane DPP rows, one line each.
+constexpr int PATH_BLOCK = 64;
+constexpr int LINES_PER_BLOCK = 4;
+// Steps of costs (and, going back, forward path costs) loaded ahead of use.
+constexpr int PREFETCH = 4;
+
+template <int K>
+__global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
+    constexpr int D = 16 * K;
+    const int lane16 = threadIdx.x & 15;
+    const int row = threadIdx.x >> 4;
+    const int b = blockIdx.x;
+    int o = 3;
+    if (b < a.block_start[1]) o = 0;
+    else if (b < a.block_start[2]) o = 1;
+    else if (b < a.block_start[3]) o = 2;
+    const int first_line = (b - a.block_start[o]) * LINES_PER_BLOCK;
+    const int W = a.W, H = a.H;
+
+    const LineGeometry g = line_geometry(o, first_line + row, W, H);
+    int longest = 0;
+#pragma unroll
+    for (int r = 0; r < LINES_PER_BLOCK; r++)
+        longest = max(longest, line_geometry(o, first_line + r, W, H).length);
+    if (longest == 0) return;  // wave-uniform
+
+    const int dy = o == 0 ? 0 : 1;
+    const int dx = o == 0 ? 1 : (o == 1 ? 0 : (o == 2 ? 1 : -1));
+    const long stride = (long)(dy * W + dx) * D;  // bytes from one step's pixel to the next
+    // A line past the last one (length 0) reads the first pixel and writes nothing.
+    const size_t first = g.length > 0 ? ((size_t)g.y0 * W + g.x0) * D : 0;
+    const int last = max(g.length - 1, 0);
+    const uint8_t *costs = a.costs + first + (size_t)lane16 * K;
+    uint8_t *paired = a.paired[o] + first + (size_t)lane16 * K;
+    auto offset = [&](int t) { return (long)min(max(t, 0), last) * stride; };
+    const uint32_t p1x2 = (uint32_t)a.p1 * 0x10001u;
+    const uint32_t p2 = (uint32_t)a.p2;
+
+    PathState<K> s;
+    s.reset();
+    Bytes<K> ring[PREFETCH];
+#pragma unroll
+    for (int p = 0; p < PREFETCH; p++) ring[p].load(costs + offset(p));
+    for (int t0 = 0; t0 < longest; t0 += PREFETCH) {
+#pragma unroll
+        for (int p = 0; p < PREFETCH; p++) {
+            const int t = t0 + p;
+            if (t >= longest) break;  // wave-uniform
+            const Bytes<K> c = ring[p];
+            ring[p].load(costs + offset(t + PREFETCH));
+            s.step(c, p1x2, p2, lane16);
+            if (t < g.length) s.bytes().store(paired + offset(t));
+        }
+    }
+    if (o == 1 && !a.backward_columns) return;
+
+    // Back along the line: the other direction, summed with the forward one in bytes
+    // (each <= p2 + 48 <= 127, so no byte carries into the next).
+    s.reset();
+    Bytes<K> forward[PREFETCH];
+    const int top = longest - 1;
+#pragma unroll
+    for (int p = 0; p < PREFETCH; p++) {
+        ring[p].load(costs + offset(top - p));
+        forward[p].load(paired + offset(top - p));
+    }
+    for (int t0 = top; t0 >= 0; t0 -= PREFETCH) {
+#pragma unroll
+        for (int p = 0; p < PREFETCH; p++) {
+            const int t = t0 - p;
+            if (t < 0) break;  // wave-uniform
+            const Bytes<K> c = ring[p];
+            Bytes<K> f = forward[p];
+            ring[p].load(costs + offset(t - PREFETCH));
+            forward[p].load(paired + offset(t - PREFETCH));
+            s.step(c, p1x2, p2, lane16);
+            if (t < g.length) {
+                const Bytes<K> mine = s.bytes();
+#pragma unroll
+                for (int k = 0; k < K / 4; k++) f.w[k] += mine.w[k];
+                f.store(paired + offset(t));
+            } else {
+                s.reset();
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Selection (reference.rs:148-186; nx_stereo.ex:608-641), 16 lanes per pixel.
+
+template <int K>
+__global__ __launch_bounds__(256) void select_kernel(const uint8_t *__restrict__ p0,
+                                                     const uint8_t *__restrict__ p1,
+                                                     const uint8_t *__restrict__ p2,
+                                                     const uint8_t *__restrict__ p3,
+                                                     int orientations, int W, int pixels,
+                                                     float uniqueness, float *__restrict__ selected,
+                                                     uint32_t *__restrict__ keys,
+                                                     uint16_t *__restrict__ sums_out) {
+    constexpr int D = 16 * K;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane16 = threadIdx.x & 15;
+    const long pix_raw = t >> 4;
+    const bool valid = pix_raw < pixels;
+    const long pix = valid ? pix_raw : pixels - 1;
+    const int x = (int)(pix % W);
+    const size_t at = (size_t)pix * D + (size_t)lane16 * K;
+
+    uint32_t sum[K / 2];
+    {
+        Bytes<K> b;
+        b.load(p0 + at);
+#pragma unroll
+        for (int k = 0; k < K / 4; k++) {
+            sum[2 * k] = lo_pair(b.w[k]);
+            sum[2 * k + 1] = hi_pair(b.w[k]);
+        }
+        const uint8_t *more[3] = {p1, p2, p3};
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            if (m + 1 >= orientations) break;
+            b.load(more[m] + at);
+#pragma unroll
+            for (int k = 0; k < K / 4; k++) {
+                sum[2 * k] = pk_add(sum[2 * k], lo_pair(b.w[k]));
+                sum[2 * k + 1] = pk_add(sum[2 * k + 1], hi_pair(b.w[k]));
+            }
+        }
+    }
+    if (sums_out && valid) {
+        uint32_t *out = reinterpret_cast<uint32_t *>(sums_out + at);
+#pragma unroll
+        for (int k = 0; k < K / 2; k++) out[k] = sum[k];
+    }
+
+    // V[d] = S[d] + 0x8000 past the left edge; keys (V << 8) | d order by cost, then d.
+    uint32_t value[K];
+    uint32_t best = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int d = lane16 * K + j;
+        const uint32_t s = (j & 1) ? (sum[j >> 1] >> 16) : (sum[j >> 1] & 0xFFFFu);
+        value[j] = s + (d > x ? 0x8000u : 0u);
+        best = min(best, (value[j] << 8) | (uint32_t)d);
+    }
+    best = row_min_u32(best);
+    const uint32_t minimum = best >> 8;
+    const int winner = (int)(best & 0xFFu);
+    const int below = max(winner - 1, 0), above = min(winner + 1, D - 1);
+    uint32_t second = 0xFFFFFFFFu, neighbours = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int d = lane16 * K + j;
+        if (d < below || d > above) second = min(second, value[j]);
+        if (d == below) neighbours = (neighbours & 0xFFFF0000u) | value[j];
+        if (d == above) neighbours = (neighbours & 0x0000FFFFu) | (value[j] << 16);
+    }
+    second = row_min_u32(second);
+    neighbours = row_min_pk(neighbours);
+    if (lane16 != 0 || !valid) return;
+
+    const float lower = (float)(neighbours & 0xFFFFu);
+    const float upper = (float)(neighbours >> 16);
+    const float minimum_f32 = (float)minimum;
+    // An empty "second" set would be i32::MAX in the reference; D >= 64 never leaves it empty.
+    const bool accepted = (float)second > __fmul_rn(minimum_f32, uniqueness);
+    const float denominator = __fsub_rn(__fadd_rn(lower, upper), __fmul_rn(2.0f, minimum_f32));
+    const float correction = fminf(
+        fmaxf(__fdiv_rn(__fsub_rn(lower, upper), __fmul_rn(2.0f, fmaxf(denominator, 1.0f))), -0.5f),
+        0.5f);
+    const bool refine = denominator > 0.0f && winner > 0 && winner < min(D - 1, x);
+    const float disparity = __fadd_rn((float)winner, refine ? correction : 0.0f);
+    selected[pix] = accepted ? disparity : -1.0f;
+    keys[pix] = accepted ? ((minimum << 16) | (uint32_t)winner) : INVALID_KEY;
+}
+
+// ------------------------------------------------------------------------------------------
+// Left-right check (reference.rs:188-235; check.rs:7-47), one block per row: each
+// right pixel keeps the cheapest left match projected onto it, ties to the larger
+// disparity, via an LDS atomic minimum of (cost << 16) | (65535 - d).
+
+constexpr int MAX_WIDTH = 4096;
+
+__global__ __launch_bounds__(256) void consistency_kernel(const float *__restrict__ selected,
+                                                          const uint32_t *__restrict__ keys, int W,
+                                                          float *__restrict__ checked) {
+    __shared__ uint32_t right[MAX_WIDTH];
+    const int y = blockIdx.x;
+    const float *sel = selected + (size_t)y * W;
+    const uint32_t *row = keys + (size_t)y * W;
+    for (int x = threadIdx.x; x < W; x += blockDim.x) right[x] = INVALID_KEY;
+    __syncthreads();
+    for (int x = threadIdx.x; x < W; x += blockDim.x) {
+        const uint32_t key = row[x];
+        if (key != INVALID_KEY) {
+            const uint32_t w = key & 0xFFFFu;
+            atomicMin(&right[x - (int)w], (key & 0xFFFF0000u) | (65535u - w));
+        }
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < W; x += blockDim.x) {
+        const float value = sel[x];
+        float out = -1.0f;
+        if (value >= 0.0f) {
+            const int lo = (int)floorf(value), hi = (int)ceilf(value);
+            bool rejected = true;
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int d = k ? hi : lo;
+                const int rx = x - d;
+                bool dis = false;
+                if (rx >= 0 && rx < W) {
+                    const uint32_t found = right[rx];
+                    dis = found != INVALID_KEY && abs(65535 - (int)(found & 0xFFFFu) - d) > 1;
+                }
+                rejected = rejected && dis;
+            }
+            if (!rejected) out = value;
+        }
+        checked[(size_t)y * W + x] = out;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// 3x3 median with replicated borders (reference.rs:237-254; filters.rs:10-30 network),
+// on i32 bit patterns, which order -1.0 and non-negative values as f32 does; then the
+// Disparity stage's d > 0 ? d : -1 (stage/disparity.ex:162).
+
+__global__ __launch_bounds__(256) void median_kernel(const float *__restrict__ in, int W, int H,
+                                                     float *__restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W) return;
+    int v[9];
+    int n = 0;
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++) {
+        const int yy = min(max(y + dy, 0), H - 1);
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            const int xx = min(max(x + dx, 0), W - 1);
+            v[n++] = __float_as_int(in[(size_t)yy * W + xx]);
+        }
+    }
+#define CX(a, b)                    \
+    {                               \
+        const int lo = min(v[a], v[b]); \
+        const int hi = max(v[a], v[b]); \
+        v[a] = lo;                  \
+        v[b] = hi;                  \
+    }
+    CX(1, 2) CX(4, 5) CX(7, 8) CX(0, 1) CX(3, 4) CX(6, 7) CX(1, 2) CX(4, 5) CX(7, 8) CX(0, 3)
+    CX(5, 8) CX(4, 7) CX(3, 6) CX(1, 4) CX(2, 5) CX(4, 7) CX(4, 2) CX(6, 4) CX(4, 2)
+#undef CX
+    const float m = __int_as_float(v[4]);
+    out[(size_t)y * W + x] = m > 0.0f ? m : -1.0f;
+}
+
+// ------------------------------------------------------------------------------------------
+
+static int orientations_of(int paths) { return paths == 8 ? 4 : 2; }
+
+size_t match_bytes(int W, int H, int D, int paths) {
+    const size_t P = (size_t)W * H, N = P * D;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return up(8 * P) * 2 + up(N) * (1 + orientations_of(paths)) + up(4 * P) * 3;
+}
+
+void match_carve(MatchBuffers &b, void *base, int W, int H, int D, int paths) {
+    const size_t P = (size_t)W * H, N = P * D;
+    auto up = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    uint8_t *p = static_cast<uint8_t *>(base);
+    b.census_left = reinterpret_cast<uint64_t *>(p);
+    p += up(8 * P);
+    b.census_right = reinterpret_cast<uint64_t *>(p);
+    p += up(8 * P);
+    b.costs = p;
+    p += up(N);
+    for (int o = 0; o < 4; o++) {
+        if (o < orientations_of(paths)) {
+            b.paired[o] = p;
+            p += up(N);
+        } else {
+            b.paired[o] = nullptr;
+        }
+    }
+    b.selected = reinterpret_cast<float *>(p);
+    p += up(4 * P);
+    b.keys = reinterpret_cast<uint32_t *>(p);
+    p += up(4 * P);
+    b.checked = reinterpret_cast<float *>(p);
+}
+
+template <int K>
+static hipError_t launch_paths_and_select(const MatchParams &p, MatchBuffers &b, uint16_t *sums,
+                                          hipStream_t stream) {
+    const int W = p.width, H = p.height;
+    const int n = orientations_of(p.paths);
+    PathArgs a;
+    a.costs = b.costs;
+    for (int o = 0; o < 4; o++) a.paired[o] = b.paired[o] ? b.paired[o] : b.paired[0];
+    a.W = W;
+    a.H = H;
+    a.p1 = p.p1;
+    a.p2 = p.p2;
+    a.backward_columns = p.paths == 8;
+    const int lines[4] = {H, W, W + H - 1, W + H - 1};
+    int blocks = 0;
+    for (int o = 0; o < 4; o++) {
+        a.block_start[o] = blocks;
+        if (o < n) blocks += (lines[o] + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
+    }
+    int slot = profile_begin(PK_PATH, stream);
+    hipLaunchKernelGGL(path_kernel<K>, dim3(blocks), dim3(PATH_BLOCK), 0, stream, a);
+    profile_end(slot, stream);
+    const long threads = (long)W * H * 16;
+    slot = profile_begin(PK_SELECT, stream);
+    hipLaunchKernelGGL(select_kernel<K>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                       stream, b.paired[0], b.paired[1], b.paired[2], b.paired[3], n, W, W * H,
+                       p.uniqueness, b.selected, b.keys, sums);
+    profile_end(slot, stream);
+    return hipGetLastError();
+}
+
+hipError_t launch_match(const MatchParams &p, const uint8_t *left, const uint8_t *right,
+                        MatchBuffers &b, float *disparity, uint16_t *sums, hipStream_t stream) {
+    const int W = p.width, H = p.height, D = p.disparities;
+    if (W > MAX_WIDTH || W < 1 || H < 1) return hipErrorInvalidValue;
+    int slot = profile_begin(PK_CENSUS, stream);
+    hipLaunchKernelGGL(census_kernel, dim3((W + 255) / 256, H, 2), dim3(256), 0, stream, left,
+                       right, W, H, b.census_left, b.census_right);
+    profile_end(slot, stream);
+    const long cost_threads = (long)W * H * (D / 16);
+    slot = profile_begin(PK_COST, stream);
+    hipLaunchKernelGGL(cost_kernel, dim3((unsigned)((cost_threads + 255) / 256)), dim3(256), 0,
+                       stream, b.census_left, b.census_right, W, W * H, D, b.costs);
+    profile_end(slot, stream);
+    hipError_t e;
+    switch (D) {
+        case 64: e = launch_paths_and_select<4>(p, b, sums, stream); break;
+        case 128: e = launch_paths_and_select<8>(p, b, sums, stream); break;
+        case 192: e = launch_paths_and_select<12>(p, b, sums, stream); break;
+        case 256: e = launch_paths_and_select<16>(p, b, sums, stream); break;
+        default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    slot = profile_begin(PK_CONSISTENCY, stream);
+    hipLaunchKernelGGL(consistency_kernel, dim3(H), dim3(256), 0, stream, b.selected, b.keys, W,
+                       b.checked);
+    profile_end(slot, stream);
+    slot = profile_begin(PK_MEDIAN, stream);
+    hipLaunchKernelGGL(median_kernel, dim3((W + 255) / 256, H), dim3(256), 0, stream, b.checked, W,
+                       H, disparity);
+    profile_end(slot, stream);
+    return hipGetLastError();
+}
+
+}  // namespace nxs

@@ -1,0 +1,319 @@
+// The stages around the matcher on gfx950: Rectification, SpeckleFilter, Depth,
+// PointCloud, TemporalFilter and Quality. f32 arithmetic follows the
+// reference's operation order with one rounding per operation (this file is
+// built with -ffp-contract=off; the explicit __f*_rn intrinsics keep it so).
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace nxs {
+
+// ------------------------------------------------------------------------------------------
+// Rectification (rectify.rs:34-128; rectify.ex:133-207): per output pixel the projection
+// through inverse(P3 R), plumb_bob distortion and K, a centre-aligned scale to the frame,
+// and a bilinear sample with 0 outside, rounded half away from zero and clamped.
+
+struct RectifyArgs {
+    Camera camera[2];
+    const uint8_t *frame[2];
+    uint8_t *out[2];
+    int fw, fh, width, height;
+    float scale_x, scale_y;
+};
+
+__global__ __launch_bounds__(256) void rectify_kernel(RectifyArgs a) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    const int side = blockIdx.z;
+    if (u >= a.width) return;
+    const Camera &c = a.camera[side];
+    const float *ipr = c.inverse_pr;
+    const float fu = (float)u, fv = (float)v;
+    float x = __fadd_rn(__fadd_rn(__fmul_rn(ipr[0], fu), __fmul_rn(ipr[1], fv)), ipr[2]);
+    float y = __fadd_rn(__fadd_rn(__fmul_rn(ipr[3], fu), __fmul_rn(ipr[4], fv)), ipr[5]);
+    const float w = __fadd_rn(__fadd_rn(__fmul_rn(ipr[6], fu), __fmul_rn(ipr[7], fv)), ipr[8]);
+    x = __fdiv_rn(x, w);
+    y = __fdiv_rn(y, w);
+    const float k1 = c.d[0], k2 = c.d[1], p1 = c.d[2], p2 = c.d[3], k3 = c.d[4];
+    const float r2 = __fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y));
+    // 1 + (k3 r2 r2 r2 + (k2 r2 r2 + k1 r2)), left-associated products
+    const float radial = __fadd_rn(
+        1.0f, __fadd_rn(__fmul_rn(__fmul_rn(__fmul_rn(k3, r2), r2), r2),
+                        __fadd_rn(__fmul_rn(__fmul_rn(k2, r2), r2), __fmul_rn(k1, r2))));
+    // x * radial + 2 p1 x y + p2 (r2 + 2 x x)
+    const float dxd = __fadd_rn(
+        __fadd_rn(__fmul_rn(x, radial), __fmul_rn(__fmul_rn(__fmul_rn(2.0f, p1), x), y)),
+        __fmul_rn(p2, __fadd_rn(r2, __fmul_rn(__fmul_rn(2.0f, x), x))));
+    // y * radial + p1 (r2 + 2 y y) + 2 p2 x y
+    const float dyd = __fadd_rn(
+        __fadd_rn(__fmul_rn(y, radial), __fmul_rn(p1, __fadd_rn(r2, __fmul_rn(__fmul_rn(2.0f, y), y)))),
+        __fmul_rn(__fmul_rn(__fmul_rn(2.0f, p2), x), y));
+    const float mx = __fadd_rn(__fadd_rn(__fmul_rn(c.k[0], dxd), __fmul_rn(c.k[1], dyd)), c.k[2]);
+    const float my = __fadd_rn(__fmul_rn(c.k[4], dyd), c.k[5]);
+    const float sx = __fsub_rn(__fmul_rn(__fadd_rn(mx, 0.5f), a.scale_x), 0.5f);
+    const float sy = __fsub_rn(__fmul_rn(__fadd_rn(my, 0.5f), a.scale_y), 0.5f);
+    const float fx0 = floorf(sx), fy0 = floorf(sy);
+    const int x0 = (int)fx0, y0 = (int)fy0;
+    const float cw = __fsub_rn(sx, fx0), rw = __fsub_rn(sy, fy0);
+    const uint8_t *frame = a.frame[side];
+    float s[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int xx = x0 + (t & 1), yy = y0 + (t >> 1);
+        const bool inside = xx >= 0 && xx < a.fw && yy >= 0 && yy < a.fh;
+        s[t] = inside ? (float)frame[(size_t)yy * a.fw + xx] : 0.0f;
+    }
+    const float top = __fadd_rn(__fmul_rn(__fsub_rn(1.0f, cw), s[0]), __fmul_rn(cw, s[1]));
+    const float bottom = __fadd_rn(__fmul_rn(__fsub_rn(1.0f, cw), s[2]), __fmul_rn(cw, s[3]));
+    const float value = __fadd_rn(__fmul_rn(__fsub_rn(1.0f, rw), top), __fmul_rn(rw, bottom));
+    const float r = fminf(fmaxf(roundf(value), 0.0f), 255.0f);
+    a.out[side][(size_t)v * a.width + u] = (uint8_t)r;
+}
+
+hipError_t launch_rectify(const uint8_t *lf, const uint8_t *rf, int fw, int fh, const Camera &left,
+                          const Camera &right, int width, int height, uint8_t *lo, uint8_t *ro,
+                          hipStream_t stream) {
+    RectifyArgs a;
+    a.camera[0] = left;
+    a.camera[1] = right;
+    a.frame[0] = lf;
+    a.frame[1] = rf;
+    a.out[0] = lo;
+    a.out[1] = ro;
+    a.fw = fw;
+    a.fh = fh;
+    a.width = width;
+    a.height = height;
+    a.scale_x = (float)((double)fw / (double)width);
+    a.scale_y = (float)((double)fh / (double)height);
+    hipLaunchKernelGGL(rectify_kernel, dim3((width + 255) / 256, height, 2), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// SpeckleFilter (filters.rs:222-268; filter.ex:48-70): keep c when c > 0 and at least
+// min_support of the other pixels within `radius` are > 0 and within max_diff of c;
+// outside the image reads -1.
+
+__global__ __launch_bounds__(256) void despeckle_kernel(const float *__restrict__ in,
+                                                        float *__restrict__ out, int W, int H,
+                                                        float min_support, float max_diff,
+                                                        int radius) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W) return;
+    const float c = in[(size_t)y * W + x];
+    unsigned support = 0;
+    const int y_lo = max(y - radius, 0), y_hi = min(y + radius, H - 1);
+    const int x_lo = max(x - radius, 0), x_hi = min(x + radius, W - 1);
+    for (int yy = y_lo; yy <= y_hi; yy++) {
+        const float *row = in + (size_t)yy * W;
+        for (int xx = x_lo; xx <= x_hi; xx++) {
+            if (yy == y && xx == x) continue;
+            const float n = row[xx];
+            support += (n > 0.0f) && (fabsf(__fsub_rn(n, c)) <= max_diff);
+        }
+    }
+    out[(size_t)y * W + x] = (c > 0.0f && (float)support >= min_support) ? c : -1.0f;
+}
+
+hipError_t launch_despeckle(const float *in, float *out, int W, int H, float min_support,
+                            float max_diff, int radius, hipStream_t stream) {
+    hipLaunchKernelGGL(despeckle_kernel, dim3((W + 255) / 256, H), dim3(256), 0, stream, in, out, W,
+                       H, min_support, max_diff, radius);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// Depth (depth.rs:12-44; depth.ex:26-32): m = f*B/d for d > 0 else 0; mm = round(m*1000),
+// 0 beyond 65535, at least 0.
+
+__global__ __launch_bounds__(256) void depth_kernel(const float *__restrict__ disparity, int pixels,
+                                                    float focal_baseline,
+                                                    uint16_t *__restrict__ mm,
+                                                    float *__restrict__ m) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pixels) return;
+    const float d = disparity[i];
+    const float metres = d > 0.0f ? __fdiv_rn(focal_baseline, d) : 0.0f;
+    const float rounded = roundf(__fmul_rn(metres, 1000.0f));
+    m[i] = metres;
+    mm[i] = rounded > 65535.0f ? (uint16_t)0 : (uint16_t)(rounded > 0.0f ? rounded : 0.0f);
+}
+
+hipError_t launch_depth(const float *disparity, int pixels, float f, float b, uint16_t *mm, float *m,
+                        hipStream_t stream) {
+    const float fb = f * b;  // host f32 multiply, as depth.rs:20
+    hipLaunchKernelGGL(depth_kernel, dim3((pixels + 255) / 256), dim3(256), 0, stream, disparity,
+                       pixels, fb, mm, m);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// PointCloud (depth.rs:58-89; depth.ex:73-90): every decimation-th pixel unprojected where
+// nearest < z < farthest, NaN otherwise, and the kept count.
+
+__global__ __launch_bounds__(256) void cloud_kernel(const float *__restrict__ depth_m, int W,
+                                                    int gw, int count, int dec, float f, float cx,
+                                                    float cy, float nearest, float farthest,
+                                                    float *__restrict__ points,
+                                                    int32_t *__restrict__ partials) {
+    __shared__ int32_t wave_kept[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int kept = 0;
+    if (i < count) {
+        const int gi = i / gw, gj = i - gi * gw;
+        const int y = gi * dec, x = gj * dec;
+        const float z = depth_m[(size_t)y * W + x];
+        float *p = points + (size_t)i * 3;
+        if (z > nearest && z < farthest) {
+            const float zf = __fdiv_rn(z, f);
+            p[0] = __fmul_rn(__fsub_rn((float)x, cx), zf);
+            p[1] = __fmul_rn(__fsub_rn((float)y, cy), zf);
+            p[2] = z;
+            kept = 1;
+        } else {
+            const float nan = __int_as_float(0x7FC00000);
+            p[0] = nan;
+            p[1] = nan;
+            p[2] = nan;
+        }
+    }
+    // Block count: ballot per wave, then four waves.
+    const unsigned long long ballot = __ballot(kept);
+    if ((threadIdx.x & 63) == 0) wave_kept[threadIdx.x >> 6] = __popcll(ballot);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = wave_kept[0] + wave_kept[1] + wave_kept[2] + wave_kept[3];
+}
+
+__global__ __launch_bounds__(256) void sum_partials_kernel(const int32_t *__restrict__ partials,
+                                                           int n, int32_t *__restrict__ total) {
+    __shared__ int32_t sums[256];
+    int s = 0;
+    for (int i = threadIdx.x; i < n; i += 256) s += partials[i];
+    sums[threadIdx.x] = s;
+    __syncthreads();
+    for (int half = 128; half > 0; half >>= 1) {
+        if ((int)threadIdx.x < half) sums[threadIdx.x] += sums[threadIdx.x + half];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = sums[0];
+}
+
+int cloud_partials(int W, int H, int dec) {
+    const int count = (W / dec) * (H / dec);
+    return (count + 255) / 256;
+}
+
+hipError_t launch_cloud(const float *depth_m, int W, int H, int dec, float f, float cx, float cy,
+                        float nearest, float farthest, float *points, int32_t *partials,
+                        int32_t *kept, hipStream_t stream) {
+    const int gw = W / dec, count = gw * (H / dec);
+    const int blocks = cloud_partials(W, H, dec);
+    if (blocks > 0)
+        hipLaunchKernelGGL(cloud_kernel, dim3(blocks), dim3(256), 0, stream, depth_m, W, gw, count,
+                           dec, f, cx, cy, nearest, farthest, points, partials);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, stream, partials, blocks, kept);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// TemporalFilter (filters.rs:296-318; filter.ex:122-129), with the carried previous
+// disparity and missed counts.
+
+__global__ __launch_bounds__(256) void temporal_kernel(float *__restrict__ disparity,
+                                                       float *__restrict__ previous,
+                                                       float *__restrict__ missed, int pixels,
+                                                       float alpha, float delta, float hold,
+                                                       float max_missed) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pixels) return;
+    const float c = disparity[i], p = previous[i];
+    const bool matched = c > 0.0f;
+    const bool agree = matched && p > 0.0f && fabsf(__fsub_rn(c, p)) <= delta;
+    const float smoothed =
+        agree ? __fadd_rn(__fmul_rn(alpha, c), __fmul_rn(__fsub_rn(1.0f, alpha), p)) : c;
+    const float pairs_missed = matched ? 0.0f : __fadd_rn(missed[i], 1.0f);
+    const bool held = !matched && p > 0.0f && pairs_missed <= hold;
+    const float filtered = held ? p : smoothed;
+    disparity[i] = filtered;
+    previous[i] = filtered;
+    missed[i] = fminf(pairs_missed, max_missed);
+}
+
+hipError_t launch_temporal(float *disparity, float *previous, float *missed, int pixels,
+                           float alpha, float delta, float hold, float max_missed,
+                           hipStream_t stream) {
+    hipLaunchKernelGGL(temporal_kernel, dim3((pixels + 255) / 256), dim3(256), 0, stream, disparity,
+                       previous, missed, pixels, alpha, delta, hold, max_missed);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// Quality (measures.rs:125-154; diagnostics.ex:264-271): per column, f64 sums down the rows
+// in row order of [matched, v, v^2, both, |c-p| * both]; each column's sum cast to f32; then
+// an f64 sum over columns, cast to f32. The carried previous disparity takes the current.
+
+__global__ __launch_bounds__(256) void quality_columns_kernel(const float *__restrict__ disparity,
+                                                              float *__restrict__ previous, int W,
+                                                              int H, float *__restrict__ columns) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= W) return;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+    for (int y = 0; y < H; y++) {
+        const size_t i = (size_t)y * W + x;
+        const float c = disparity[i], p = previous[i];
+        const float matched = c > 0.0f ? 1.0f : 0.0f;
+        const float both = __fmul_rn(p > 0.0f ? 1.0f : 0.0f, matched);
+        const float value = __fmul_rn(c, matched);
+        s0 += (double)matched;
+        s1 += (double)value;
+        s2 += (double)__fmul_rn(value, value);
+        s3 += (double)both;
+        s4 += (double)__fmul_rn(fabsf(__fsub_rn(c, p)), both);
+        previous[i] = c;
+    }
+    columns[0 * W + x] = (float)s0;
+    columns[1 * W + x] = (float)s1;
+    columns[2 * W + x] = (float)s2;
+    columns[3 * W + x] = (float)s3;
+    columns[4 * W + x] = (float)s4;
+}
+
+// One block per plane: an f64 tree over the columns' f32 sums.
+__global__ __launch_bounds__(256) void quality_total_kernel(const float *__restrict__ columns,
+                                                            int W, float *__restrict__ sums5) {
+    __shared__ double partial[256];
+    const float *plane = columns + (size_t)blockIdx.x * W;
+    double s = 0;
+    for (int x = threadIdx.x; x < W; x += 256) s += (double)plane[x];
+    partial[threadIdx.x] = s;
+    __syncthreads();
+    for (int half = 128; half > 0; half >>= 1) {
+        if ((int)threadIdx.x < half) partial[threadIdx.x] += partial[threadIdx.x + half];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums5[blockIdx.x] = (float)partial[0];
+}
+
+hipError_t launch_quality(const float *disparity, float *previous, int W, int H, float *columns,
+                          float *sums5, hipStream_t stream) {
+    hipLaunchKernelGGL(quality_columns_kernel, dim3((W + 255) / 256), dim3(256), 0, stream,
+                       disparity, previous, W, H, columns);
+    hipLaunchKernelGGL(quality_total_kernel, dim3(5), dim3(256), 0, stream, columns, W, sums5);
+    return hipGetLastError();
+}
+
+__global__ void fill_f32_kernel(float *dst, float value, size_t count) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = value;
+}
+
+hipError_t launch_fill_f32(float *dst, float value, size_t count, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream,
+                       dst, value, count);
+    return hipGetLastError();
+}
+
+}  // namespace nxs

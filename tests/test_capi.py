@@ -1,0 +1,55 @@
+"""The C ABI library loads and exports every entry point include/nxs_hip.h declares,
+and the checks it makes before touching a device (no compute runs here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from nx_stereo_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "nxs_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(nxs_[a-z_]+)\s*\(", text))
+    return sorted(n for n in names if n != "nxs_done_fn")
+
+
+def test_header_declares_the_entry_points_the_bindings_use():
+    assert set(declared_functions()) == set(_native.SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.load()
+    for symbol in declared_functions():
+        assert hasattr(lib, symbol), symbol
+
+
+def test_workspace_size_grows_with_the_volume():
+    small = _native.workspace_size(64, 48, 64, 8)
+    assert small >= 64 * 48 * 64 * 5
+    assert _native.workspace_size(64, 48, 128, 8) > small
+    assert _native.workspace_size(64, 48, 64, 3) < small
+
+
+@pytest.mark.parametrize("args,reason", [
+    ((8, 48, 64, 8, 8, 64, 1.0), "wider than 8"),
+    ((64, 6, 64, 8, 8, 64, 1.0), "higher than 6"),
+    ((4097, 48, 64, 8, 8, 64, 1.0), "at most 4096"),
+    ((64, 48, 96, 8, 8, 64, 1.0), "disparities must be"),
+    ((64, 48, 64, 4, 8, 64, 1.0), "paths must be 3 or 8"),
+    ((64, 48, 64, 8, 64, 64, 1.0), "p1 < p2"),
+    ((64, 48, 64, 8, 8, 80, 1.0), "p1 < p2"),
+    ((64, 48, 64, 8, 8, 64, 0.9), "uniqueness"),
+    ((64, 48, 64, 8, 8, 64, float("nan")), "uniqueness"),
+])
+def test_match_refuses_what_the_reference_refuses_before_any_device_work(args, reason):
+    """lib.rs:41-73 and 645-687: Parameters::check."""
+    w, h, d, paths, p1, p2, u = args
+    left = np.zeros((h, w), np.uint8)
+    with pytest.raises(_native.NativeError, match=reason):
+        _native.match_host(left, left, d, paths, p1, p2, u)

@@ -1,0 +1,249 @@
+"""HipSegment through the C ABI on the GPU, against the oracle: every stage alone
+and the full pipeline, bit for bit (the quality sums within 1e-5 relative, the
+reference's own bar, native_parity_test.exs:102-110), and the protocol's
+refusals, ordering and carries (segment.rs:574-778; main.rs:188-221;
+stage/stateful_pipeline_test.exs:59-120)."""
+import numpy as np
+import pytest
+
+from nx_stereo_amd import protocol, scenes, stages
+from nx_stereo_amd.segment import Context, HipSegment
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def context():
+    with Context(0) as ctx:
+        yield ctx
+
+
+def same_bits(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape and a.dtype == b.dtype, (a.shape, b.shape, a.dtype, b.dtype)
+    if a.dtype == np.float32:
+        a, b = a.view(np.uint32), b.view(np.uint32)
+    bad = np.argwhere(a != b)
+    assert bad.size == 0, f"{len(bad)} differ, first at {tuple(bad[0])}"
+
+
+def rig_geometry(width, height, frame_width=None, frame_height=None):
+    frame_width, frame_height = frame_width or width, frame_height or height
+    cal = scenes.rig(frame_width, frame_height)
+    scale = width / frame_width
+    geometry = {"width": width, "height": height, "frame_width": frame_width,
+                "frame_height": frame_height, "focal_length_px": cal["focal_length_px"] * scale,
+                "baseline_m": cal["baseline_m"],
+                "principal_point": (width / 2.0, height / 2.0)}
+    for side in ("left", "right"):
+        c = cal[side]
+        k = list(c["k"])
+        # The calibration at the working size: the rectified camera matrix scales with it.
+        p = [v * scale if i in (0, 2, 3, 5, 6) else v for i, v in enumerate(c["p"])]
+        geometry[f"{side}_camera"] = scenes.camera(k, c["d"], c["r"], p)
+    return geometry, cal
+
+
+def build(context, members, geometry):
+    ok, seg = HipSegment.build(members, geometry, context)
+    assert ok == "ok", seg
+    return seg
+
+
+def run_ok(seg, entries):
+    result = seg.run(entries)
+    assert result[0] == "ok", result
+    return result[1]
+
+
+def test_rectification_with_distortion_and_shrink(context):
+    geometry, cal = rig_geometry(182, 136, 364, 272)
+    frames = [scenes.noise(364, 272, 1), scenes.noise(364, 272, 2)]
+    seg = build(context, [("rectification", stages.Rectification(right_row_offset=1.5))], geometry)
+    out = run_ok(seg, {"left_frame": frames[0], "right_frame": frames[1]})
+    params = seg.members[0][1].params(geometry)
+    for side, frame in zip(("left", "right"), frames):
+        cam = params[side]
+        want = oracle.rectify(frame, cam["k"], cam["d"], cam["inverse_pr"], (136, 182))
+        same_bits(out[side], want)
+
+
+@pytest.mark.parametrize("disparities,paths", [(64, 3), (64, 8), (128, 8), (192, 8), (256, 8)])
+def test_disparity_stage(context, disparities, paths):
+    left, right, _ = scenes.planar_pair(320, 96, seed=disparities + paths, low=4,
+                                        high=min(disparities - 8, 120))
+    geometry, _ = rig_geometry(320, 96)
+    stage = stages.Disparity(num_disparities=disparities, paths=paths, p1=10, p2=70, uniqueness=1.1)
+    seg = build(context, [("disparity", stage)], geometry)
+    out = run_ok(seg, {"left": left, "right": right})
+    same_bits(out["disparity"], oracle.sgm(left, right, disparities, paths, 10, 70, 1.1))
+    assert seg.member_times()[0][0] == "disparity"
+
+
+def test_speckle_depth_cloud_quality_chain(context):
+    geometry, _ = rig_geometry(200, 120)
+    rng = np.random.default_rng(4)
+    disparity = np.where(rng.random((120, 200)) < 0.2, -1.0,
+                         rng.uniform(0.2, 60.0, (120, 200))).astype(np.float32)
+    disparity[30:60, 40:90] = 17.25
+    members = [("speckle_filter", stages.SpeckleFilter(min_support=10, max_diff=1.5)),
+               ("depth", stages.Depth()),
+               ("point_cloud", stages.PointCloud(decimation=3)),
+               ("quality", stages.Quality())]
+    seg = build(context, members, geometry)
+    quality = oracle.Quality((120, 200))
+    for pair in range(3):
+        entry = np.roll(disparity, pair * 7, axis=1)
+        out = run_ok(seg, {"disparity": entry})
+        assert list(out) == ["disparity", "depth_mm", "cloud_grid", "cloud_points", "quality_sums"]
+        d = oracle.despeckle(entry, 10, 1.5)
+        same_bits(out["disparity"], d)
+        mm, m = oracle.depth(d, geometry["focal_length_px"], geometry["baseline_m"])
+        same_bits(out["depth_mm"], mm)
+        cx, cy = geometry["principal_point"]
+        points, kept = oracle.cloud(m, 3, geometry["focal_length_px"], cx, cy)
+        same_bits(out["cloud_grid"], points)
+        assert int(out["cloud_points"]) == kept
+        want = quality.sums(d)
+        np.testing.assert_allclose(out["quality_sums"], want, rtol=1e-5)
+
+
+def test_temporal_filter_carries_across_pairs(context):
+    geometry, _ = rig_geometry(64, 32)
+    seg = build(context, [("temporal_filter", stages.TemporalFilter(alpha=0.5, delta=2.0, hold=1))],
+                geometry)
+    want = oracle.Temporal((32, 64))
+    rng = np.random.default_rng(5)
+    base = rng.uniform(1, 50, (32, 64)).astype(np.float32)
+    for pair in range(5):
+        entry = np.where(rng.random((32, 64)) < 0.3, -1.0, base + rng.uniform(-3, 3, (32, 64))).astype(np.float32)
+        out = run_ok(seg, {"disparity": entry})
+        same_bits(out["disparity"], want.filter(entry, 0.5, 2.0, 1))
+
+
+def test_stateful_pipeline_closed_forms(context):
+    """stage/stateful_pipeline_test.exs:59-99 on a uniform image: the temporal filter's
+    output over (10, 12, 12, -1, -1, -1, 50) and the quality jitter."""
+    geometry, _ = rig_geometry(16, 8)
+    members = [("temporal_filter", stages.TemporalFilter(alpha=0.5, delta=2.0, hold=2)),
+               ("quality", stages.Quality())]
+    seg = build(context, members, geometry)
+    filtered, jitter = [], []
+    for value in (10, 12, 12, -1, -1, -1, 50):
+        out = run_ok(seg, {"disparity": np.full((8, 16), value, np.float32)})
+        filtered.append(float(out["disparity"][0, 0]))
+        sums = out["quality_sums"]
+        jitter.append(float(sums[4] / sums[3]) if sums[3] else 0.0)
+    assert filtered == [10.0, 11.0, 11.5, 11.5, 11.5, -1.0, 50.0]
+    assert jitter[:3] == [0.0, 1.0, 0.5]
+    seg.reload(seg.members, geometry)
+    out = run_ok(seg, {"disparity": np.full((8, 16), 30.0, np.float32)})
+    assert float(out["disparity"][0, 0]) == 30.0, "a reload restarts the carries"
+    assert float(out["quality_sums"][3]) == 0.0
+
+
+def test_full_pipeline_728x544x128(context):
+    """The benchmark's workload, every output against the oracle's chain."""
+    geometry, cal = rig_geometry(728, 544)
+    left_frame, right_frame, _ = scenes.raw_frames(cal, 7)
+    members = [("rectification", stages.Rectification()),
+               ("disparity", stages.Disparity(num_disparities=128, paths=8)),
+               ("speckle_filter", stages.SpeckleFilter()), ("depth", stages.Depth()),
+               ("point_cloud", stages.PointCloud(decimation=4)), ("quality", stages.Quality())]
+    seg = build(context, members, geometry)
+    out = run_ok(seg, {"left_frame": left_frame, "right_frame": right_frame})
+    assert list(out) == ["left_frame", "right_frame", "left", "right", "disparity", "depth_mm",
+                         "cloud_grid", "cloud_points", "quality_sums"]
+    cams = seg.members[0][1].params(geometry)
+    rect = {}
+    for side, frame in (("left", left_frame), ("right", right_frame)):
+        c = cams[side]
+        rect[side] = oracle.rectify(frame, c["k"], c["d"], c["inverse_pr"], (544, 728))
+        same_bits(out[side], rect[side])
+    d = oracle.sgm(rect["left"], rect["right"], 128, 8)
+    same_bits(out["disparity"], oracle.despeckle(d, 12, 1.0))
+    assert (out["disparity"] > 0).mean() > 0.3, "the scene is mostly matched"
+
+
+def test_refusals_map_like_the_port(context):
+    geometry, _ = rig_geometry(32, 24)
+    g = {k: geometry[k] for k in ("width", "height", "frame_width", "frame_height")}
+    reply = context.request(protocol.configure(9001, g, [("decode", [])]))
+    assert reply.reason == "no HIP implementation of decode"
+    reply = context.request(protocol.run(9001, [], []))
+    assert reply.reason == "segment 9001 is not configured"
+    reply = context.request(protocol.configure(9001, g, [("temporal_filter", [("max_missed", 255.0),
+                                                                              ("hold", 2.0)])]))
+    assert reply.reason == "temporal_filter has no setting hold"
+    reply = context.request(protocol.configure(9001, g, [("speckle_filter", [])]))
+    assert reply.reason == "speckle_filter needs its radius setting"
+    reply = context.request(protocol.configure(9001, {**g, "frame_height": 12}, [("rectification", [])]))
+    assert reply.reason.startswith("rectification only shrinks: 32x12 frames")
+    assert context.request(protocol.configure(9001, g, [("depth", [])])) is True
+    reply = context.request(protocol.run(9001, [[]], [("disparity", np.zeros((2, 2), np.float32))]))
+    assert "disparity is a [2, 2] tensor of type 2, expected a [24, 32] f32 map" in reply.reason
+    reply = context.request(protocol.run(9001, [[]], []))
+    assert reply.reason == "no focal_length_px among the stage's inputs"
+    assert isinstance(protocol.decode(bytes([9])), protocol.Refused)
+    reply = context.request(bytes([2, 1, 0]))
+    assert "ends 2 bytes early" in reply.reason
+    reply = context.request(bytes([9, 1, 0, 0, 0]))
+    assert reply.reason == "unknown request kind 9"
+    assert context.request(protocol.release(9001)) is None
+    assert context.request(protocol.run(9001, [[]], [])).reason == "segment 9001 is not configured"
+
+
+def test_a_refused_reconfigure_leaves_the_segment_unconfigured(context):
+    g = {"width": 32, "height": 24, "frame_width": 32, "frame_height": 24}
+    assert context.request(protocol.configure(77, g, [("depth", [])])) is True
+    assert isinstance(context.request(protocol.configure(77, g, [("decode", [])])), protocol.Refused)
+    assert context.request(protocol.run(77, [[]], [])).reason == "segment 77 is not configured"
+
+
+def test_penalties_must_be_integer_bytes(context):
+    geometry, _ = rig_geometry(32, 24)
+    left = scenes.noise(32, 24, 1)
+    seg = build(context, [("disparity", stages.Disparity())], geometry)
+    seg.inputs = [[("p1", np.array([8.5], np.float32)), ("p2", np.array([64.0], np.float32)),
+                   ("uniqueness", np.array([1.0], np.float32))]]
+    result = seg.await_(seg.send_request({"left": left, "right": left}))
+    assert result == ("error", "p1 must be an integer from 0 to 255, got 8.5")
+
+
+def test_pairs_complete_in_the_order_sent(context):
+    """native_segment.ex:167-171: carried state depends on the order."""
+    geometry, _ = rig_geometry(48, 16)
+    seg = build(context, [("temporal_filter", stages.TemporalFilter(alpha=0.5, delta=100.0, hold=0))],
+                geometry)
+    values = [8.0, 16.0, 4.0, 12.0, 20.0]
+    requests = [seg.send_request({"disparity": np.full((16, 48), v, np.float32)}) for v in values]
+    got = [float(seg.await_(r)[1]["disparity"][0, 0]) for r in requests]
+    want, previous = [], None
+    for v in values:
+        previous = v if previous is None else 0.5 * v + 0.5 * previous
+        want.append(previous)
+    assert got == want
+
+
+def test_device_entries_run_in_place(context):
+    """nxs_run_device: entries already in HBM, outputs left in the segment's memory."""
+    import ctypes
+
+    torch = pytest.importorskip("torch")
+    geometry, _ = rig_geometry(96, 64)
+    left, right = scenes.scene(96, 64, 11)
+    seg = build(context, [("disparity", stages.Disparity())], geometry)
+    assert context.request(seg.configure_record()) is True
+    seg.configured_version = seg.version
+    dl, dr = (torch.from_numpy(a).cuda() for a in (left, right))
+    torch.cuda.synchronize()
+    outs = context.run_device(seg.run_record({}), [("left", 0, (64, 96), dl.data_ptr()),
+                                                   ("right", 0, (64, 96), dr.data_ptr())])
+    assert [(o[0], o[1], o[2]) for o in outs] == [("disparity", 2, (64, 96))]
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipDeviceSynchronize()
+    host = np.empty((64, 96), np.float32)
+    assert hip.hipMemcpy(host.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(outs[0][3]),
+                         ctypes.c_size_t(host.nbytes), 2) == 0  # hipMemcpyDeviceToHost
+    same_bits(host, oracle.sgm(left, right, 64, 8))
