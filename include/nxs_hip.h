@@ -70,6 +70,18 @@ void nxs_release_reply(nxs_ctx *ctx, int64_t ticket);
 /* How many kept replies are complete and not yet released (non-blocking). */
 int nxs_poll(nxs_ctx *ctx, int max);
 
+/* Serving: runs of different segments overlap on the device. Each run holds
+ * one of its segment's run slots (3 pairs in flight per segment) and, with a
+ * Disparity member, a matcher workspace (costs and path volumes, about
+ * 5*W*H*D bytes) from a pool whose HBM budget this sets: a run waits for its
+ * workspace when the budget is spent, and is admitted when a run in flight
+ * completes. The default is half the device memory free at nxs_open, or
+ * NXS_HBM_BUDGET_MB. No counterpart in the reference (the Port computes one
+ * request at a time). */
+int nxs_set_hbm_budget(nxs_ctx *ctx, size_t bytes);
+/* The workspace bytes allocated and the runs in flight, for monitoring. */
+int nxs_stats(nxs_ctx *ctx, size_t *workspace_bytes, int *runs_in_flight);
+
 /* ---- The matcher on device buffers (the hot path, without records) ----
  *
  * Stage.Disparity's computation (stage/disparity.ex:135-163, full search;

@@ -19,7 +19,7 @@ NXS_OK, NXS_TIMEOUT, NXS_EINVAL, NXS_EDEVICE, NXS_ENOMEM, NXS_EUNKNOWN = 0, 1, -
 # Every symbol include/nxs_hip.h declares.
 SYMBOLS = (
     "nxs_open", "nxs_close", "nxs_submit", "nxs_wait", "nxs_reply", "nxs_release_reply",
-    "nxs_poll", "nxs_match_workspace_size", "nxs_match_device", "nxs_match_host",
+    "nxs_poll", "nxs_set_hbm_budget", "nxs_stats", "nxs_match_workspace_size", "nxs_match_device", "nxs_match_host",
     "nxs_run_device", "nxs_profile_start", "nxs_profile_stop", "nxs_last_error",
 )
 
@@ -62,6 +62,10 @@ def load() -> ctypes.CDLL:
     lib.nxs_release_reply.restype = None
     lib.nxs_poll.argtypes = [P, i]
     lib.nxs_poll.restype = i
+    lib.nxs_set_hbm_budget.argtypes = [P, sz]
+    lib.nxs_set_hbm_budget.restype = i
+    lib.nxs_stats.argtypes = [P, ctypes.POINTER(sz), ctypes.POINTER(i)]
+    lib.nxs_stats.restype = i
     lib.nxs_match_workspace_size.argtypes = [i, i, i, i]
     lib.nxs_match_workspace_size.restype = sz
     lib.nxs_match_device.argtypes = [P, P, i, i, i, i, i, i, f, P, P, sz, P]

@@ -369,15 +369,35 @@ struct Slot {
     void *dev;  // nullptr when the tensor is of a shape or type no stage reads
 };
 
+// Per-run resources of a segment, one set per pair in flight: pinned staging both ways
+// and the events that time its members.
+struct RunSlot {
+    uint8_t *upload = nullptr, *download = nullptr;
+    size_t upload_bytes = 0, download_bytes = 0;
+    std::vector<hipEvent_t> events;  // before the first member, then after each
+    bool busy = false;
+};
+
+// A matcher workspace (costs and path volumes) from the context's pool.
+struct Workspace {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    bool busy = false;
+};
+
+constexpr int RUN_SLOTS = 3;  // pairs of one segment in flight at once
+
 struct Segment {
     uint32_t id = 0;
     Geometry g{};
     std::vector<Member> members;
     hipStream_t stream = nullptr;
-    std::vector<hipEvent_t> events;
     void *arena = nullptr;
-    uint8_t *host = nullptr;  // pinned staging for outputs
-    size_t host_bytes = 0;
+    std::mutex mu;  // held while work is enqueued on the segment
+    RunSlot slots[RUN_SLOTS];
+    int next_slot = 0;
+    size_t workspace_bytes = 0;  // of its largest matcher member, 0 without one
+    Workspace *device_workspace = nullptr;  // the pool's, held for nxs_run_device
 
     // device buffers carved from the arena
     uint8_t *frames[2]{}, *images[2]{};
@@ -385,15 +405,16 @@ struct Segment {
     uint16_t *depth_mm = nullptr;
     float *depth_m = nullptr, *cloud = nullptr, *quality_columns = nullptr, *quality_sums = nullptr;
     int32_t *cloud_partials = nullptr, *cloud_kept = nullptr;
-    void *match_workspace = nullptr;
-    size_t match_workspace_bytes = 0;
     std::vector<float *> carry_a, carry_b;  // per member: temporal previous/missed, quality previous
 
     ~Segment() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (auto e : events) (void)hipEventDestroy(e);
+        for (auto &slot : slots) {
+            for (auto e : slot.events) (void)hipEventDestroy(e);
+            if (slot.upload) (void)hipHostFree(slot.upload);
+            if (slot.download) (void)hipHostFree(slot.download);
+        }
         if (arena) (void)hipFree(arena);
-        if (host) (void)hipHostFree(host);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -403,12 +424,12 @@ static uint8_t *up256_ptr(uint8_t *p) {
     return reinterpret_cast<uint8_t *>((reinterpret_cast<uintptr_t>(p) + 255) & ~(uintptr_t)255);
 }
 
-static std::unique_ptr<Segment> configure(uint32_t id, const Geometry &g,
+static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
                                           const std::vector<std::pair<std::string, Options>> &ms) {
     if (g.width < 2 || g.height < 2)
         throw Error{"a segment works at 2x2 or more, got " + std::to_string(g.width) + "x" +
                     std::to_string(g.height)};
-    auto s = std::make_unique<Segment>();
+    auto s = std::make_shared<Segment>();
     s->id = id;
     s->g = g;
     for (auto &m : ms) s->members.push_back(make_member(g, m.first, m.second));
@@ -430,7 +451,7 @@ static std::unique_ptr<Segment> configure(uint32_t id, const Geometry &g,
     for (auto &m : s->members) carries += m.kind == Kind::TemporalFilter ? 2 : (m.kind == Kind::Quality);
     size_t bytes = up256(F) * 2 + up256(P) * 2 + up256(4 * P) * 2 + up256(2 * P) + up256(4 * P) +
                    up256(4 * cloud) + up256(4 * (partials + 1)) + up256(4 * (5 * (size_t)g.width + 5)) +
-                   up256(match) + up256(4 * P) * carries + 256;
+                   up256(4 * P) * carries + 256;
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&s->arena, bytes));
     uint8_t *p = static_cast<uint8_t *>(s->arena);
@@ -452,8 +473,7 @@ static std::unique_ptr<Segment> configure(uint32_t id, const Geometry &g,
     s->cloud_kept = s->cloud_partials + partials;
     s->quality_columns = reinterpret_cast<float *>(carve(4 * (5 * (size_t)g.width + 5)));
     s->quality_sums = s->quality_columns + 5 * (size_t)g.width;
-    s->match_workspace = carve(match);
-    s->match_workspace_bytes = match;
+    s->workspace_bytes = match;
     for (auto &m : s->members) {
         float *a = nullptr, *b = nullptr;
         if (m.kind == Kind::TemporalFilter) {
@@ -468,13 +488,17 @@ static std::unique_ptr<Segment> configure(uint32_t id, const Geometry &g,
         s->carry_a.push_back(a);
         s->carry_b.push_back(b);
     }
-    s->events.resize(s->members.size() + 1);
-    for (auto &e : s->events) HIP_TRY(hipEventCreate(&e));
-    // Pinned staging large enough for every output of every member.
-    // left and right (u8), disparity (f32), depth_mm (u16), the cloud and two small tensors,
-    // each span rounded up to 16 bytes.
-    s->host_bytes = 2 * P + 4 * P + 2 * P + 4 * cloud + 256;
-    HIP_TRY(hipHostMalloc((void **)&s->host, s->host_bytes));
+    // Pinned staging per pair in flight. Up: the frames, images and maps an entry may bring.
+    // Down: left and right (u8), disparity (f32), depth_mm (u16), the cloud and two small
+    // tensors, each span rounded up to 16 bytes.
+    for (auto &slot : s->slots) {
+        slot.upload_bytes = 2 * F + 2 * P + 8 * P + 256;
+        slot.download_bytes = 2 * P + 4 * P + 2 * P + 4 * cloud + 256;
+        HIP_TRY(hipHostMalloc((void **)&slot.upload, slot.upload_bytes));
+        HIP_TRY(hipHostMalloc((void **)&slot.download, slot.download_bytes));
+        slot.events.resize(s->members.size() + 1);
+        for (auto &e : slot.events) HIP_TRY(hipEventCreate(&e));
+    }
     HIP_TRY(hipStreamSynchronize(s->stream));
     return s;
 }
@@ -535,9 +559,12 @@ struct Enqueued {
 
 // Enqueues every member on `stream`: entries go to the segment's buffers (device entries
 // a stage only reads are read in place), then each stage in order (segment.rs:206-239).
+// Host entries go through the slot's pinned upload buffer (or, without a slot, a
+// synchronous copy); `workspace` holds workspace_bytes for the matcher, if any.
 static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
                         const std::vector<std::pair<std::string, HostTensor>> &entries,
-                        const std::vector<DeviceEntry> &device_entries, hipStream_t stream) {
+                        const std::vector<DeviceEntry> &device_entries, hipStream_t stream,
+                        RunSlot *run_slot, const std::vector<hipEvent_t> &events, void *workspace) {
     if (inputs.size() != s.members.size())
         throw Error{std::to_string(s.members.size()) + " stages, got the inputs of " +
                     std::to_string(inputs.size())};
@@ -560,11 +587,20 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
         if (name == "depth_m" && type == TYPE_F32 && shape == image) return s.depth_m;
         return nullptr;
     };
+    size_t staged = 0;
     for (auto &e : entries) {
         const HostTensor &t = e.second;
         Slot slot{t.type, t.shape, nullptr};
         if (void *target = target_of(e.first, t.type, t.shape)) {
-            HIP_TRY(hipMemcpyAsync(target, t.bytes, t.size, hipMemcpyHostToDevice, stream));
+            if (run_slot && staged + t.size <= run_slot->upload_bytes) {
+                std::memcpy(run_slot->upload + staged, t.bytes, t.size);
+                HIP_TRY(hipMemcpyAsync(target, run_slot->upload + staged, t.size,
+                                       hipMemcpyHostToDevice, stream));
+                staged += (t.size + 255) & ~(size_t)255;
+            } else {
+                HIP_TRY(hipStreamSynchronize(stream));
+                HIP_TRY(hipMemcpy(target, t.bytes, t.size, hipMemcpyHostToDevice));
+            }
             slot.dev = target;
             if (e.first == "disparity") free_disparity = 1;
         }
@@ -586,7 +622,7 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
     }
 
     std::vector<const char *> &filled = q.filled;
-    HIP_TRY(hipEventRecord(s.events[0], stream));
+    if (!events.empty()) HIP_TRY(hipEventRecord(events[0], stream));
     for (size_t i = 0; i < s.members.size(); i++) {
         const Member &m = s.members[i];
         const Inputs &in = inputs[i];
@@ -609,7 +645,8 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
                 auto *r = (const uint8_t *)shaped(tensors, "right", TYPE_U8, image, "u8 image");
                 float *out = s.disparity[free_disparity];
                 MatchBuffers b;
-                match_carve(b, s.match_workspace, W, H, m.disparities, m.paths);
+                if (!workspace) throw Error{"no matcher workspace"};
+                match_carve(b, workspace, W, H, m.disparities, m.paths);
                 MatchParams mp{(int)W, (int)H, m.disparities, m.paths, p1, p2, u};
                 HIP_TRY(launch_match(mp, l, r, b, out, nullptr, stream));
                 tensors["disparity"] = Slot{TYPE_F32, image, out};
@@ -659,7 +696,7 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
                 break;
             }
         }
-        HIP_TRY(hipEventRecord(s.events[i + 1], stream));
+        if (!events.empty()) HIP_TRY(hipEventRecord(events[i + 1], stream));
         for (const char *name : member_outputs(m.kind)) {
             bool seen = false;
             for (const char *f : filled) seen = seen || std::strcmp(f, name) == 0;
@@ -670,53 +707,66 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
     return q;
 }
 
-static std::vector<uint8_t> run(Segment &s, const std::vector<Inputs> &inputs,
-                                const std::vector<std::pair<std::string, HostTensor>> &entries) {
-    Enqueued q = enqueue(s, inputs, entries, {}, s.stream);
-    std::map<std::string, Slot> &tensors = q.tensors;
-    const std::vector<const char *> &filled = q.filled;
+// One filled tensor in a run slot's download staging.
+struct Output {
+    const char *name;
+    uint8_t type;
+    std::vector<uint32_t> shape;
+    size_t offset, bytes;
+};
 
-    // Downloads, then the reply: tag, member times, then each filled tensor.
+// Enqueues the downloads of every filled tensor into the slot's staging, in first-filled
+// order (segment.rs:220-234), and returns where each lands.
+static std::vector<Output> enqueue_downloads(Enqueued &q, RunSlot &slot, hipStream_t stream) {
+    std::vector<Output> outputs;
     size_t offset = 0;
-    std::vector<std::pair<size_t, size_t>> spans;
-    for (const char *name : filled) {
-        const Slot &slot = tensors[name];
+    for (const char *name : q.filled) {
+        const Slot &t = q.tensors[name];
         size_t elements = 1;
-        for (uint32_t d : slot.shape) elements *= d;
-        const size_t bytes = elements * element_bytes(slot.type);
-        if (offset + bytes > s.host_bytes) throw Error{"output staging too small"};
-        HIP_TRY(hipMemcpyAsync(s.host + offset, slot.dev, bytes, hipMemcpyDeviceToHost, s.stream));
-        spans.emplace_back(offset, bytes);
+        for (uint32_t d : t.shape) elements *= d;
+        const size_t bytes = elements * element_bytes(t.type);
+        if (offset + bytes > slot.download_bytes) throw Error{"output staging too small"};
+        HIP_TRY(hipMemcpyAsync(slot.download + offset, t.dev, bytes, hipMemcpyDeviceToHost, stream));
+        outputs.push_back(Output{name, t.type, t.shape, offset, bytes});
         offset += (bytes + 15) & ~(size_t)15;
     }
-    HIP_TRY(hipStreamSynchronize(s.stream));
+    return outputs;
+}
 
+// The reply of a completed run: tag 8, each member's GPU time in us, then the tensors
+// (protocol.rs:182-202).
+static std::vector<uint8_t> outputs_record(const RunSlot &slot, size_t members,
+                                           const std::vector<Output> &outputs) {
     std::vector<uint8_t> reply;
     reply.push_back(TAG_OUTPUTS);
-    reply.push_back((uint8_t)s.members.size());
-    for (size_t i = 0; i < s.members.size(); i++) {
+    reply.push_back((uint8_t)members);
+    for (size_t i = 0; i < members; i++) {
         float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, s.events[i], s.events[i + 1]));
+        HIP_TRY(hipEventElapsedTime(&ms, slot.events[i], slot.events[i + 1]));
         const uint32_t us = (uint32_t)std::min(4294967295.0, std::floor((double)ms * 1000.0));
         for (int k = 0; k < 4; k++) reply.push_back((uint8_t)(us >> (8 * k)));
     }
-    reply.push_back((uint8_t)filled.size());
-    for (size_t k = 0; k < filled.size(); k++) {
-        const Slot &slot = tensors[filled[k]];
-        const std::string name = filled[k];
-        reply.push_back((uint8_t)name.size());
-        reply.insert(reply.end(), name.begin(), name.end());
-        reply.push_back(slot.type);
-        reply.push_back((uint8_t)slot.shape.size());
-        for (uint32_t d : slot.shape)
+    reply.push_back((uint8_t)outputs.size());
+    for (const Output &o : outputs) {
+        const size_t length = std::strlen(o.name);
+        reply.push_back((uint8_t)length);
+        reply.insert(reply.end(), o.name, o.name + length);
+        reply.push_back(o.type);
+        reply.push_back((uint8_t)o.shape.size());
+        for (uint32_t d : o.shape)
             for (int b = 0; b < 4; b++) reply.push_back((uint8_t)(d >> (8 * b)));
-        reply.insert(reply.end(), s.host + spans[k].first, s.host + spans[k].first + spans[k].second);
+        reply.insert(reply.end(), slot.download + o.offset, slot.download + o.offset + o.bytes);
     }
     return reply;
 }
 
 // ------------------------------------------------------------------------------------------
-// The context: a queue of requests, one thread computing them in order.
+// The context. A submitting thread queues records; the context's worker thread parses
+// them and enqueues each run's work on its segment's stream without waiting; the
+// completion thread watches the runs in flight and posts each reply as its pair
+// finishes. Runs of one segment stay in order (one stream, FIFO); runs of different
+// segments overlap on the device, admitted as their segment's run slots and the
+// matcher workspace pool's HBM budget allow (continuous batching).
 
 struct Job {
     int64_t ticket;
@@ -730,77 +780,273 @@ struct Kept {
     std::vector<uint8_t> reply;
 };
 
+// A run in flight: what the completion thread needs to reply once its last event fires.
+struct Pending {
+    int64_t ticket;
+    std::shared_ptr<Segment> segment;
+    int slot;
+    Workspace *workspace;
+    std::vector<Output> outputs;
+    nxs_done_fn done;
+    void *user;
+};
+
 }  // namespace nxs
 
 struct nxs_ctx {
     int device = 0;
-    std::mutex mu;
-    std::condition_variable cv_jobs, cv_done;
+    std::mutex mu;  // everything below but the segments' device work
+    std::condition_variable cv_jobs, cv_done, cv_free, cv_pending;
     std::deque<nxs::Job> jobs;
     std::unordered_map<int64_t, nxs::Kept> kept;
-    std::map<uint32_t, std::unique_ptr<nxs::Segment>> segments;
+    std::map<uint32_t, std::shared_ptr<nxs::Segment>> segments;
+    std::deque<nxs::Pending> pending;
+    std::vector<std::unique_ptr<nxs::Workspace>> workspaces;
+    size_t budget = 0, allocated = 0;  // the matcher workspaces' HBM budget and use
     int64_t next_ticket = 1;
-    bool stopping = false;
-    std::thread worker;
-    std::mutex segments_mu;  // the segments, between the context thread and nxs_run_device
+    bool stopping = false, worker_done = false;
+    std::thread worker, completer;
 
-    std::vector<uint8_t> handle(const std::vector<uint8_t> &record) {
-        using namespace nxs;
-        Request q;
-        try {
-            q = parse(record.data(), record.size());
-        } catch (const Error &e) {
-            return refused_record(e.reason);
-        }
-        try {
-            if (q.kind == CONFIGURE) {
-                // main.rs:146-160: a refused (re)configuration leaves the segment unconfigured.
-                segments.erase(q.segment);
-                segments[q.segment] = configure(q.segment, q.geometry, q.members);
-                return {TAG_CONFIGURED};
-            }
-            if (q.kind == RELEASE) {
-                segments.erase(q.segment);
-                return {};
-            }
-            auto it = segments.find(q.segment);
-            if (it == segments.end())
-                return refused_record("segment " + std::to_string(q.segment) + " is not configured");
-            return run(*it->second, q.inputs, q.entries);
-        } catch (const Error &e) {
-            if (q.kind == CONFIGURE) segments.erase(q.segment);
-            return refused_record(e.reason);
-        } catch (const std::bad_alloc &) {
-            if (q.kind == CONFIGURE) segments.erase(q.segment);
-            return refused_record("out of host memory");
+    void post(int64_t ticket, nxs_done_fn done, void *user, std::vector<uint8_t> reply) {
+        if (done) {
+            done(user, ticket, reply.data(), reply.size());
+        } else {
+            std::lock_guard<std::mutex> lock(mu);
+            auto &k = kept[ticket];
+            k.reply = std::move(reply);
+            k.complete = true;
+            cv_done.notify_all();
         }
     }
 
-    void loop() {
+    // A free workspace of `bytes`, allocated within the budget, or waited for.
+    nxs::Workspace *acquire_workspace(size_t bytes, std::unique_lock<std::mutex> &lock) {
+        using namespace nxs;
+        for (;;) {
+            for (auto &w : workspaces)
+                if (!w->busy && w->bytes == bytes) {
+                    w->busy = true;
+                    return w.get();
+                }
+            bool any_busy = false;
+            for (auto &w : workspaces) any_busy = any_busy || w->busy;
+            // Within the budget, or alone on the device: allocate.
+            if (allocated + bytes <= budget || !any_busy) {
+                void *ptr = nullptr;
+                if (hipMalloc(&ptr, bytes) == hipSuccess) {
+                    workspaces.push_back(std::make_unique<Workspace>(Workspace{ptr, bytes, true}));
+                    allocated += bytes;
+                    return workspaces.back().get();
+                }
+                (void)hipGetLastError();
+                if (!any_busy) {
+                    // Free idle workspaces of other sizes and try once more.
+                    bool freed = false;
+                    for (auto it = workspaces.begin(); it != workspaces.end();) {
+                        if (!(*it)->busy) {
+                            (void)hipFree((*it)->ptr);
+                            allocated -= (*it)->bytes;
+                            it = workspaces.erase(it);
+                            freed = true;
+                        } else {
+                            ++it;
+                        }
+                    }
+                    if (!freed) throw Error{"out of device memory for the matcher"};
+                    continue;
+                }
+            } else {
+                // Over the budget: give back an idle workspace of another size first.
+                for (auto it = workspaces.begin(); it != workspaces.end(); ++it)
+                    if (!(*it)->busy) {
+                        (void)hipFree((*it)->ptr);
+                        allocated -= (*it)->bytes;
+                        workspaces.erase(it);
+                        break;
+                    }
+                if (allocated + bytes <= budget) continue;
+            }
+            cv_free.wait(lock);  // admission: a run completes and frees memory
+        }
+    }
+
+    // Enqueues a run; the reply comes from the completion thread.
+    void start_run(nxs::Request &q, int64_t ticket, nxs_done_fn done, void *user) {
+        using namespace nxs;
+        std::shared_ptr<Segment> seg;
+        int slot_index = -1;
+        Workspace *ws = nullptr;
+        {
+            std::unique_lock<std::mutex> lock(mu);
+            auto it = segments.find(q.segment);
+            if (it == segments.end()) {
+                lock.unlock();
+                post(ticket, done, user,
+                     refused_record("segment " + std::to_string(q.segment) + " is not configured"));
+                return;
+            }
+            seg = it->second;
+            // A free run slot of this segment (its pairs in flight are bounded).
+            for (;;) {
+                for (int k = 0; k < RUN_SLOTS && slot_index < 0; k++) {
+                    const int s = (seg->next_slot + k) % RUN_SLOTS;
+                    if (!seg->slots[s].busy) slot_index = s;
+                }
+                if (slot_index >= 0) break;
+                cv_free.wait(lock);
+            }
+            seg->slots[slot_index].busy = true;
+            seg->next_slot = (slot_index + 1) % RUN_SLOTS;
+            try {
+                if (seg->workspace_bytes) ws = acquire_workspace(seg->workspace_bytes, lock);
+            } catch (const Error &e) {
+                seg->slots[slot_index].busy = false;
+                lock.unlock();
+                post(ticket, done, user, refused_record(e.reason));
+                return;
+            }
+        }
+        RunSlot &slot = seg->slots[slot_index];
+        try {
+            std::lock_guard<std::mutex> guard(seg->mu);
+            Enqueued e = enqueue(*seg, q.inputs, q.entries, {}, seg->stream, &slot, slot.events,
+                                 ws ? ws->ptr : nullptr);
+            std::vector<Output> outputs = enqueue_downloads(e, slot, seg->stream);
+            HIP_TRY(hipEventRecord(slot.events.back(), seg->stream));
+            std::lock_guard<std::mutex> lock(mu);
+            pending.push_back(Pending{ticket, seg, slot_index, ws, std::move(outputs), done, user});
+            cv_pending.notify_all();
+        } catch (const Error &e) {
+            // Work already enqueued may still use the slot and workspace: let it finish.
+            (void)hipStreamSynchronize(seg->stream);
+            {
+                std::lock_guard<std::mutex> lock(mu);
+                slot.busy = false;
+                if (ws) ws->busy = false;
+                cv_free.notify_all();
+            }
+            post(ticket, done, user, refused_record(e.reason));
+        }
+    }
+
+    // Forgets a segment. Runs in flight keep it alive until they are answered; the
+    // workspace its device runs held goes back to the pool once its stream is idle.
+    void drop_segment(uint32_t id) {
+        std::shared_ptr<nxs::Segment> old;
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            auto it = segments.find(id);
+            if (it == segments.end()) return;
+            old = it->second;
+            segments.erase(it);
+        }
+        if (old->device_workspace) {
+            (void)hipStreamSynchronize(old->stream);
+            std::lock_guard<std::mutex> lock(mu);
+            old->device_workspace->busy = false;
+            old->device_workspace = nullptr;
+            cv_free.notify_all();
+        }
+    }
+
+    void handle(nxs::Job &job) {
+        using namespace nxs;
+        Request q;
+        try {
+            q = parse(job.record.data(), job.record.size());
+        } catch (const Error &e) {
+            post(job.ticket, job.done, job.user, refused_record(e.reason));
+            return;
+        }
+        if (q.kind == RUN) {
+            start_run(q, job.ticket, job.done, job.user);
+            return;
+        }
+        std::vector<uint8_t> reply;
+        try {
+            // main.rs:146-160: a refused (re)configuration leaves the segment unconfigured.
+            drop_segment(q.segment);
+            if (q.kind == CONFIGURE) {
+                std::shared_ptr<Segment> s = configure(q.segment, q.geometry, q.members);
+                std::lock_guard<std::mutex> lock(mu);
+                segments[q.segment] = std::move(s);
+                reply = {TAG_CONFIGURED};
+            }
+        } catch (const Error &e) {
+            reply = refused_record(e.reason);
+        } catch (const std::bad_alloc &) {
+            reply = refused_record("out of host memory");
+        }
+        post(job.ticket, job.done, job.user, std::move(reply));
+    }
+
+    void work() {
         (void)hipSetDevice(device);
         for (;;) {
             nxs::Job job;
             {
                 std::unique_lock<std::mutex> lock(mu);
                 cv_jobs.wait(lock, [&] { return stopping || !jobs.empty(); });
-                if (jobs.empty()) return;
+                if (jobs.empty()) {
+                    worker_done = true;
+                    cv_pending.notify_all();
+                    return;
+                }
                 job = std::move(jobs.front());
                 jobs.pop_front();
             }
-            std::vector<uint8_t> reply;
+            handle(job);
+        }
+    }
+
+    // Replies to runs as their last events complete, in any order across segments.
+    void complete() {
+        using namespace nxs;
+        (void)hipSetDevice(device);
+        for (;;) {
+            std::vector<int64_t> tickets;
             {
-                std::lock_guard<std::mutex> lock(segments_mu);
-                reply = handle(job.record);
+                std::unique_lock<std::mutex> lock(mu);
+                cv_pending.wait(lock, [&] { return !pending.empty() || worker_done; });
+                if (pending.empty()) return;
             }
-            if (job.done) {
-                job.done(job.user, job.ticket, reply.data(), reply.size());
-            } else {
-                std::lock_guard<std::mutex> lock(mu);
-                auto &k = kept[job.ticket];
-                k.reply = std::move(reply);
-                k.complete = true;
-                cv_done.notify_all();
+            bool progressed = false;
+            for (;;) {
+                Pending p;
+                bool found = false;
+                hipError_t status = hipSuccess;
+                {
+                    std::lock_guard<std::mutex> lock(mu);
+                    for (auto it = pending.begin(); it != pending.end(); ++it) {
+                        status = hipEventQuery(it->segment->slots[it->slot].events.back());
+                        if (status != hipErrorNotReady) {
+                            p = std::move(*it);
+                            pending.erase(it);
+                            found = true;
+                            break;
+                        }
+                    }
+                }
+                if (!found) break;
+                progressed = true;
+                std::vector<uint8_t> reply;
+                RunSlot &slot = p.segment->slots[p.slot];
+                try {
+                    if (status != hipSuccess)
+                        throw Error{std::string("HIP error: ") + hipGetErrorString(status)};
+                    reply = outputs_record(slot, p.segment->members.size(), p.outputs);
+                } catch (const Error &e) {
+                    reply = refused_record(e.reason);
+                }
+                {
+                    std::lock_guard<std::mutex> lock(mu);
+                    slot.busy = false;
+                    if (p.workspace) p.workspace->busy = false;
+                    cv_free.notify_all();
+                }
+                post(p.ticket, p.done, p.user, std::move(reply));
             }
+            if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
     }
 };
@@ -833,8 +1079,31 @@ int nxs_open(int device, nxs_ctx **out, char *err, size_t errlen) {
     }
     auto *ctx = new nxs_ctx();
     ctx->device = device;
-    ctx->worker = std::thread([ctx] { ctx->loop(); });
+    // The matcher workspaces may take half the device memory free now, unless
+    // NXS_HBM_BUDGET_MB says otherwise (nxs_set_hbm_budget changes it later).
+    size_t free_bytes = 0, total_bytes = 0;
+    if (hipMemGetInfo(&free_bytes, &total_bytes) != hipSuccess) free_bytes = (size_t)8 << 30;
+    ctx->budget = free_bytes / 2;
+    if (const char *mb = std::getenv("NXS_HBM_BUDGET_MB")) ctx->budget = (size_t)std::atoll(mb) << 20;
+    ctx->worker = std::thread([ctx] { ctx->work(); });
+    ctx->completer = std::thread([ctx] { ctx->complete(); });
     *out = ctx;
+    return NXS_OK;
+}
+
+int nxs_set_hbm_budget(nxs_ctx *ctx, size_t bytes) {
+    if (!ctx) return nxs::fail(NXS_EINVAL, "no context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->budget = bytes;
+    ctx->cv_free.notify_all();
+    return NXS_OK;
+}
+
+int nxs_stats(nxs_ctx *ctx, size_t *workspace_bytes, int *runs_in_flight) {
+    if (!ctx) return nxs::fail(NXS_EINVAL, "no context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (workspace_bytes) *workspace_bytes = ctx->allocated;
+    if (runs_in_flight) *runs_in_flight = (int)ctx->pending.size();
     return NXS_OK;
 }
 
@@ -847,12 +1116,15 @@ void nxs_close(nxs_ctx *ctx) {
         pending.swap(ctx->jobs);
     }
     ctx->cv_jobs.notify_all();
-    if (ctx->worker.joinable()) ctx->worker.join();
-    const std::vector<uint8_t> closed = nxs::refused_record("the context is closed");
+    // Requests not started fail; runs already on the device finish and are answered.
     for (auto &job : pending)
-        if (job.done) job.done(job.user, job.ticket, closed.data(), closed.size());
+        ctx->post(job.ticket, job.done, job.user, nxs::refused_record("the context is closed"));
+    if (ctx->worker.joinable()) ctx->worker.join();
+    if (ctx->completer.joinable()) ctx->completer.join();
     (void)hipSetDevice(ctx->device);
     ctx->segments.clear();
+    for (auto &w : ctx->workspaces) (void)hipFree(w->ptr);
+    ctx->workspaces.clear();
     delete ctx;
 }
 
@@ -984,14 +1256,24 @@ int nxs_run_device(nxs_ctx *ctx, const uint8_t *record, size_t len, const nxs_te
     using namespace nxs;
     if (!ctx || !record || (n_entries > 0 && !entries) || !n_outputs)
         return fail(NXS_EINVAL, "no context, record or output count");
-    std::lock_guard<std::mutex> lock(ctx->segments_mu);
     try {
         Request q = parse(record, len);
         if (q.kind != RUN) return fail(NXS_EINVAL, "nxs_run_device takes a run record");
-        auto it = ctx->segments.find(q.segment);
-        if (it == ctx->segments.end())
-            return fail(NXS_EINVAL, "segment " + std::to_string(q.segment) + " is not configured");
-        Segment &seg = *it->second;
+        std::shared_ptr<Segment> held;
+        void *workspace = nullptr;
+        {
+            std::unique_lock<std::mutex> lock(ctx->mu);
+            auto it = ctx->segments.find(q.segment);
+            if (it == ctx->segments.end())
+                return fail(NXS_EINVAL, "segment " + std::to_string(q.segment) + " is not configured");
+            held = it->second;
+            // Device runs of a segment keep one workspace of the pool for themselves.
+            if (held->workspace_bytes && !held->device_workspace)
+                held->device_workspace = ctx->acquire_workspace(held->workspace_bytes, lock);
+            if (held->device_workspace) workspace = held->device_workspace->ptr;
+        }
+        Segment &seg = *held;
+        std::lock_guard<std::mutex> guard(seg.mu);
         std::vector<DeviceEntry> device;
         for (int e = 0; e < n_entries; e++) {
             const nxs_tensor &t = entries[e];
@@ -999,7 +1281,7 @@ int nxs_run_device(nxs_ctx *ctx, const uint8_t *record, size_t len, const nxs_te
             device.push_back(DeviceEntry{t.name, t.type, std::vector<uint32_t>(t.shape, t.shape + t.rank), t.data});
         }
         hipStream_t s = stream ? (hipStream_t)stream : seg.stream;
-        Enqueued out = enqueue(seg, q.inputs, q.entries, device, s);
+        Enqueued out = enqueue(seg, q.inputs, q.entries, device, s, nullptr, {}, workspace);
         *n_outputs = (int)out.filled.size();
         for (int k = 0; k < (int)out.filled.size() && k < max_outputs; k++) {
             const Slot &slot = out.tensors[out.filled[k]];
