@@ -113,9 +113,7 @@ def main() -> int:
               f"with no exchange (--gpus {args.gpus} is not run)", file=sys.stderr)
         return 2
 
-    import torch
-
-    from nx_stereo_amd import protocol, segment
+    from nx_stereo_amd import hip, segment
 
     geometry, members, left_frame, right_frame = workload()
     context = segment.Context(0)
@@ -126,34 +124,32 @@ def main() -> int:
     seg.configured_version = seg.version
     record = seg.run_record({})
 
-    device = torch.device("cuda", 0)
-    frames = [torch.from_numpy(np.ascontiguousarray(f)).to(device) for f in (left_frame, right_frame)]
-    stream = torch.cuda.Stream(device)
-    entries = [("left_frame", 0, (HEIGHT, WIDTH), frames[0].data_ptr()),
-               ("right_frame", 0, (HEIGHT, WIDTH), frames[1].data_ptr())]
-    torch.cuda.synchronize()
+    frames = [hip.Buffer.of(f) for f in (left_frame, right_frame)]
+    stream = hip.Stream()
+    entries = [("left_frame", 0, (HEIGHT, WIDTH), frames[0].ptr),
+               ("right_frame", 0, (HEIGHT, WIDTH), frames[1].ptr)]
+    hip.synchronize()
 
     def step():
-        return context.run_device(record, entries, stream=stream.cuda_stream)
+        return context.run_device(record, entries, stream=stream.handle)
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    hip.synchronize()
 
-    events = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    events = [hip.Event() for _ in range(args.steps + 1)]
     segment.profile_start(capacity=8 * args.steps + 16)
-    torch.cuda.synchronize()
+    hip.synchronize()
     wall = time.perf_counter()
-    with torch.cuda.stream(stream):
-        events[0].record(stream)
-        for k in range(args.steps):
-            step()
-            events[k + 1].record(stream)
-    torch.cuda.synchronize()
+    events[0].record(stream)
+    for k in range(args.steps):
+        step()
+        events[k + 1].record(stream)
+    hip.synchronize()
     wall = time.perf_counter() - wall
     kernels = segment.profile_stop()
-    gpu_ms = events[0].elapsed_time(events[-1])
-    per_pair = np.array([events[k].elapsed_time(events[k + 1]) for k in range(args.steps)])
+    gpu_ms = events[0].elapsed_ms(events[-1])
+    per_pair = np.array([events[k].elapsed_ms(events[k + 1]) for k in range(args.steps)])
 
     ms_per_step = gpu_ms / args.steps
     pairs_per_s = 1e3 / ms_per_step

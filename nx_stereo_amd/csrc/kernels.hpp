@@ -61,9 +61,10 @@ hipError_t launch_temporal(float *disparity, float *previous, float *missed, int
                            float alpha, float delta, float hold, float max_missed,
                            hipStream_t stream);
 
-// `columns` holds 5 * width floats of scratch.
+// `partials` holds quality_scratch_bytes(width, height) of scratch.
+size_t quality_scratch_bytes(int width, int height);
 hipError_t launch_quality(const float *disparity, float *previous, int width, int height,
-                          float *columns, float *sums5, hipStream_t stream);
+                          double *partials, float *sums5, hipStream_t stream);
 
 hipError_t launch_fill_f32(float *dst, float value, size_t count, hipStream_t stream);
 

@@ -403,7 +403,8 @@ struct Segment {
     uint8_t *frames[2]{}, *images[2]{};
     float *disparity[2]{};
     uint16_t *depth_mm = nullptr;
-    float *depth_m = nullptr, *cloud = nullptr, *quality_columns = nullptr, *quality_sums = nullptr;
+    float *depth_m = nullptr, *cloud = nullptr, *quality_sums = nullptr;
+    double *quality_partials = nullptr;
     int32_t *cloud_partials = nullptr, *cloud_kept = nullptr;
     std::vector<float *> carry_a, carry_b;  // per member: temporal previous/missed, quality previous
 
@@ -450,7 +451,8 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
     int carries = 0;
     for (auto &m : s->members) carries += m.kind == Kind::TemporalFilter ? 2 : (m.kind == Kind::Quality);
     size_t bytes = up256(F) * 2 + up256(P) * 2 + up256(4 * P) * 2 + up256(2 * P) + up256(4 * P) +
-                   up256(4 * cloud) + up256(4 * (partials + 1)) + up256(4 * (5 * (size_t)g.width + 5)) +
+                   up256(4 * cloud) + up256(4 * (partials + 1)) +
+                   up256(quality_scratch_bytes(g.width, g.height)) + up256(32) +
                    up256(4 * P) * carries + 256;
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&s->arena, bytes));
@@ -471,8 +473,8 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
     s->cloud = reinterpret_cast<float *>(carve(4 * cloud));
     s->cloud_partials = reinterpret_cast<int32_t *>(carve(4 * (partials + 1)));
     s->cloud_kept = s->cloud_partials + partials;
-    s->quality_columns = reinterpret_cast<float *>(carve(4 * (5 * (size_t)g.width + 5)));
-    s->quality_sums = s->quality_columns + 5 * (size_t)g.width;
+    s->quality_partials = reinterpret_cast<double *>(carve(quality_scratch_bytes(g.width, g.height)));
+    s->quality_sums = reinterpret_cast<float *>(carve(4 * 8));
     s->workspace_bytes = match;
     for (auto &m : s->members) {
         float *a = nullptr, *b = nullptr;
@@ -654,8 +656,9 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
                 break;
             }
             case Kind::SpeckleFilter: {
-                const float support = scalar(in, "min_support"), diff = scalar(in, "max_diff");
+                // segment.rs:437-447 reads the map before the scalars.
                 auto *d = (const float *)shaped(tensors, "disparity", TYPE_F32, image, "f32 map");
+                const float support = scalar(in, "min_support"), diff = scalar(in, "max_diff");
                 float *out = s.disparity[0] == d ? s.disparity[1] : s.disparity[0];
                 HIP_TRY(launch_despeckle(d, out, W, H, support, diff, m.radius, stream));
                 tensors["disparity"] = Slot{TYPE_F32, image, out};
@@ -671,8 +674,9 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
                 break;
             }
             case Kind::Depth: {
-                const float f = scalar(in, "focal_length_px"), b = scalar(in, "baseline_m");
+                // segment.rs:475-482 reads the map before the scalars.
                 auto *d = (const float *)shaped(tensors, "disparity", TYPE_F32, image, "f32 map");
+                const float f = scalar(in, "focal_length_px"), b = scalar(in, "baseline_m");
                 HIP_TRY(launch_depth(d, (int)P, f, b, s.depth_mm, s.depth_m, stream));
                 tensors["depth_mm"] = Slot{TYPE_U16, image, s.depth_mm};
                 tensors["depth_m"] = Slot{TYPE_F32, image, s.depth_m};
@@ -690,7 +694,7 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
             }
             case Kind::Quality: {
                 auto *d = (const float *)shaped(tensors, "disparity", TYPE_F32, image, "f32 map");
-                HIP_TRY(launch_quality(d, s.carry_a[i], W, H, s.quality_columns, s.quality_sums,
+                HIP_TRY(launch_quality(d, s.carry_a[i], W, H, s.quality_partials, s.quality_sums,
                                        stream));
                 tensors["quality_sums"] = Slot{TYPE_F32, {5}, s.quality_sums};
                 break;

@@ -58,19 +58,33 @@ __global__ __launch_bounds__(256) void census_kernel(const uint8_t *__restrict__
 
 // ------------------------------------------------------------------------------------------
 // Costs (reference.rs:41-63): C[y,x,d] = popcount(FL[y,x] ^ FR[y,x-d]), FR = 0 left of
-// the image. One thread per pixel and 16 disparities, one 16-byte store.
+// the image. A block takes a run of `tile` pixels of one row: the right features they
+// compare with, [x0-D+1, x0+tile), are staged in LDS once; each thread then takes one
+// pixel and 16 disparities and makes one 16-byte store, so the block's stores are one
+// contiguous run of tile*D bytes.
 
-__global__ __launch_bounds__(256) void cost_kernel(const uint64_t *__restrict__ cl,
-                                                   const uint64_t *__restrict__ cr, int W,
-                                                   int pixels, int D,
-                                                   uint8_t *__restrict__ costs) {
+constexpr int COST_BLOCK = 256;
+constexpr int MAX_D = 256;
+
+__global__ __launch_bounds__(COST_BLOCK) void cost_kernel(const uint64_t *__restrict__ cl,
+                                                          const uint64_t *__restrict__ cr, int W,
+                                                          int D, uint8_t *__restrict__ costs) {
+    __shared__ uint64_t right[COST_BLOCK + MAX_D];
     const int chunks = D >> 4;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long pix = t / chunks;
-    if (pix >= pixels) return;
-    const int chunk = (int)(t - pix * chunks);
-    const int x = (int)(pix % W);
-    const uint64_t l = cl[pix];
+    const int tile = COST_BLOCK / chunks;
+    const int y = blockIdx.y;
+    const int x0 = blockIdx.x * tile;
+    const int base = x0 - D + 1;  // the LDS window's first column
+    const uint64_t *row = cr + (size_t)y * W;
+    for (int i = threadIdx.x; i < tile + D - 1; i += COST_BLOCK) {
+        const int x = base + i;
+        right[i] = (x >= 0 && x < W) ? row[x] : 0;
+    }
+    __syncthreads();
+    const int p = threadIdx.x / chunks, chunk = threadIdx.x - p * chunks;
+    const int x = x0 + p;
+    if (p >= tile || x >= W) return;
+    const uint64_t l = cl[(size_t)y * W + x];
     uint32_t words[4];
 #pragma unroll
     for (int w = 0; w < 4; w++) {
@@ -78,12 +92,11 @@ __global__ __launch_bounds__(256) void cost_kernel(const uint64_t *__restrict__ 
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int d = chunk * 16 + w * 4 + j;
-            const uint64_t r = x >= d ? cr[pix - d] : 0;
-            word |= (uint32_t)__builtin_popcountll(l ^ r) << (8 * j);
+            word |= (uint32_t)__builtin_popcountll(l ^ right[x - d - base]) << (8 * j);
         }
         words[w] = word;
     }
-    *reinterpret_cast<uint4 *>(costs + pix * D + chunk * 16) =
+    *reinterpret_cast<uint4 *>(costs + ((size_t)y * W + x) * D + chunk * 16) =
         make_uint4(words[0], words[1], words[2], words[3]);
 }
 
@@ -282,52 +295,89 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
     auto offset = [&](int t) { return (long)min(max(t, 0), last) * stride; };
     const uint32_t p1x2 = (uint32_t)a.p1 * 0x10001u;
     const uint32_t p2 = (uint32_t)a.p2;
+    const int L = g.length;
 
-    PathState<K> s;
-    s.reset();
-    Bytes<K> ring[PREFETCH];
+    if (o == 1 && !a.backward_columns) {
+        // Columns with 3 paths: down only, its own path costs.
+        PathState<K> s;
+        s.reset();
+        Bytes<K> ring[PREFETCH];
 #pragma unroll
-    for (int p = 0; p < PREFETCH; p++) ring[p].load(costs + offset(p));
-    for (int t0 = 0; t0 < longest; t0 += PREFETCH) {
+        for (int p = 0; p < PREFETCH; p++) ring[p].load(costs + offset(p));
+        for (int t0 = 0; t0 < longest; t0 += PREFETCH) {
 #pragma unroll
-        for (int p = 0; p < PREFETCH; p++) {
-            const int t = t0 + p;
-            if (t >= longest) break;  // wave-uniform
-            const Bytes<K> c = ring[p];
-            ring[p].load(costs + offset(t + PREFETCH));
-            s.step(c, p1x2, p2, lane16);
-            if (t < g.length) s.bytes().store(paired + offset(t));
+            for (int p = 0; p < PREFETCH; p++) {
+                const int t = t0 + p;
+                if (t >= longest) break;  // wave-uniform
+                const Bytes<K> c = ring[p];
+                ring[p].load(costs + offset(t + PREFETCH));
+                s.step(c, p1x2, p2, lane16);
+                if (t < L) s.bytes().store(paired + offset(t));
+            }
         }
+        return;
     }
-    if (o == 1 && !a.backward_columns) return;
 
-    // Back along the line: the other direction, summed with the forward one in bytes
-    // (each <= p2 + 48 <= 127, so no byte carries into the next).
-    s.reset();
-    Bytes<K> forward[PREFETCH];
-    const int top = longest - 1;
+    // Both directions at once, from the two ends of the line (two independent chains,
+    // half the sequential steps). At step i the forward direction is at t = i and the
+    // backward one at u = L-1-i. Before they meet, each stores its own path costs; past
+    // the middle, each adds the other's (stored when that one passed) and stores the byte
+    // sum (each <= p2 + 48 <= 127, so no byte carries into the next). An odd line's middle
+    // pixel sums both directions in registers.
+    PathState<K> fwd, bwd;
+    fwd.reset();
+    bwd.reset();
+    Bytes<K> ring_f[PREFETCH], ring_b[PREFETCH];    // costs, PREFETCH steps ahead
+    Bytes<K> other_f[PREFETCH], other_b[PREFETCH];  // the other direction's, past the middle
+    bool late[PREFETCH];  // the other's costs were not yet stored when prefetched
 #pragma unroll
     for (int p = 0; p < PREFETCH; p++) {
-        ring[p].load(costs + offset(top - p));
-        forward[p].load(paired + offset(top - p));
+        ring_f[p].load(costs + offset(p));
+        ring_b[p].load(costs + offset(L - 1 - p));
+        late[p] = true;
     }
-    for (int t0 = top; t0 >= 0; t0 -= PREFETCH) {
+    for (int i0 = 0; i0 < longest; i0 += PREFETCH) {
 #pragma unroll
         for (int p = 0; p < PREFETCH; p++) {
-            const int t = t0 - p;
-            if (t < 0) break;  // wave-uniform
-            const Bytes<K> c = ring[p];
-            Bytes<K> f = forward[p];
-            ring[p].load(costs + offset(t - PREFETCH));
-            forward[p].load(paired + offset(t - PREFETCH));
-            s.step(c, p1x2, p2, lane16);
-            if (t < g.length) {
-                const Bytes<K> mine = s.bytes();
+            const int i = i0 + p;
+            if (i >= longest) break;  // wave-uniform
+            const Bytes<K> cf = ring_f[p], cb = ring_b[p];
+            Bytes<K> of = other_f[p], ob = other_b[p];
+            const bool past = 2 * i > L - 1;  // this step is past the middle
+            if (past && late[p]) {
+                of.load(paired + offset(i));
+                ob.load(paired + offset(L - 1 - i));
+            }
+            // Prefetch step i + PREFETCH. Past the middle its values were stored by the other
+            // direction at step L-1-(i+PREFETCH), which must already be behind (< i).
+            const int j = i + PREFETCH;
+            ring_f[p].load(costs + offset(j));
+            ring_b[p].load(costs + offset(L - 1 - j));
+            late[p] = 2 * i + PREFETCH <= L - 1;
+            if (2 * j > L - 1 && !late[p]) {
+                other_f[p].load(paired + offset(j));
+                other_b[p].load(paired + offset(L - 1 - j));
+            }
+            fwd.step(cf, p1x2, p2, lane16);
+            bwd.step(cb, p1x2, p2, lane16);
+            if (i < L) {
+                Bytes<K> mf = fwd.bytes(), mb = bwd.bytes();
+                if (2 * i < L - 1) {
+                    mf.store(paired + offset(i));
+                    mb.store(paired + offset(L - 1 - i));
+                } else if (2 * i == L - 1) {
 #pragma unroll
-                for (int k = 0; k < K / 4; k++) f.w[k] += mine.w[k];
-                f.store(paired + offset(t));
-            } else {
-                s.reset();
+                    for (int k = 0; k < K / 4; k++) mf.w[k] += mb.w[k];
+                    mf.store(paired + offset(i));
+                } else {
+#pragma unroll
+                    for (int k = 0; k < K / 4; k++) {
+                        mf.w[k] += of.w[k];
+                        mb.w[k] += ob.w[k];
+                    }
+                    mf.store(paired + offset(i));
+                    mb.store(paired + offset(L - 1 - i));
+                }
             }
         }
     }
@@ -578,10 +628,10 @@ hipError_t launch_match(const MatchParams &p, const uint8_t *left, const uint8_t
     hipLaunchKernelGGL(census_kernel, dim3((W + 255) / 256, H, 2), dim3(256), 0, stream, left,
                        right, W, H, b.census_left, b.census_right);
     profile_end(slot, stream);
-    const long cost_threads = (long)W * H * (D / 16);
+    const int tile = COST_BLOCK / (D / 16);
     slot = profile_begin(PK_COST, stream);
-    hipLaunchKernelGGL(cost_kernel, dim3((unsigned)((cost_threads + 255) / 256)), dim3(256), 0,
-                       stream, b.census_left, b.census_right, W, W * H, D, b.costs);
+    hipLaunchKernelGGL(cost_kernel, dim3((W + tile - 1) / tile, H), dim3(COST_BLOCK), 0, stream,
+                       b.census_left, b.census_right, W, D, b.costs);
     profile_end(slot, stream);
     hipError_t e;
     switch (D) {

@@ -61,6 +61,14 @@ def test_wide_disparity_ranges(disparities):
     check_all(left, right, disparities, 8)
 
 
+@pytest.mark.parametrize("size", [(9, 7), (10, 8), (33, 17), (17, 40)])
+def test_small_and_odd_sizes(size):
+    """Lines of every length from 1 up, odd and even, meet in the middle correctly."""
+    w, h = size
+    for paths in (3, 8):
+        check_all(scenes.noise(w, h, 3), scenes.noise(w, h, 4), 64, paths)
+
+
 def test_planar_pair_364x272x64():
     left, right, _ = scenes.planar_pair(364, 272, seed=1)
     check_all(left, right, 64, 8)
