@@ -109,6 +109,34 @@ int nxs_match_host(const uint8_t *left, const uint8_t *right, int width, int hei
                    uint16_t *sums, float *selected, uint32_t *keys, float *checked,
                    float *disparity);
 
+/* ---- The batched form ----
+ *
+ * Many pairs of uneven sizes and disparity ranges through the matcher
+ * (as nxs_match_device) in one graph: pairs are grouped into size classes
+ * (width, height, disparities, paths); each kernel is one launch per class
+ * over a descriptor table in device memory; the whole batch is captured
+ * once into a hipGraph that nxs_batch_run replays. With `autotune`, each
+ * class's aggregation schedule (prefetch depth 2/4/8; orientations in one
+ * launch or one each) is timed on the batch itself and the fastest kept,
+ * also for later single-pair launches of that size. The batch owns one
+ * matcher workspace per pair; the pairs' images and outputs stay the
+ * caller's, read and written at every replay. No counterpart in the
+ * reference, whose matcher takes one pair at a time. */
+typedef struct nxs_pair {
+    const uint8_t *left, *right; /* device, u8 {height, width} */
+    float *disparity;            /* device, f32 {height, width} */
+    int width, height, disparities, paths, p1, p2;
+    float uniqueness;
+} nxs_pair;
+
+typedef struct nxs_batch nxs_batch;
+
+int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch **out);
+int nxs_batch_run(nxs_batch *batch, void *stream);
+/* The classes and their schedules, one line each; valid while the batch lives. */
+const char *nxs_batch_describe(nxs_batch *batch);
+void nxs_batch_destroy(nxs_batch *batch);
+
 /* ---- A configured segment on device-resident entries ----
  *
  * Segment::run (native/stereo_match/src/segment.rs:206-239) with the entry

@@ -31,10 +31,37 @@ struct MatchParams {
     float uniqueness;
 };
 
+// One pair of a matching launch: its images, output, workspace and live settings. A launch
+// covers one pair (passed by value) or a table of pairs of one size class in device memory.
+struct PairDesc {
+    const uint8_t *left, *right;
+    float *disparity;
+    uint16_t *sums;  // optional: the aggregated path costs, W*H*D u16
+    MatchBuffers b;
+    int p1, p2;
+    float uniqueness;
+};
+
+// How the aggregation is scheduled: the steps of costs the path kernel loads ahead, and
+// whether the four line orientations go in one launch or in one launch each.
+struct Schedule {
+    int prefetch = 4;  // 2, 4 or 8
+    bool split = false;
+};
+
+// The schedule launches at this size use: the autotuner's pick, or the default.
+Schedule schedule_for(int width, int height, int disparities, int paths);
+void remember_schedule(int width, int height, int disparities, int paths, const Schedule &s);
+
 // Disparity stage: left/right u8 {H,W} -> disparity f32 {H,W}, -1 where unmatched.
 // `sums` (optional, W*H*D u16) receives the aggregated path costs.
 hipError_t launch_match(const MatchParams &p, const uint8_t *left, const uint8_t *right,
                         MatchBuffers &b, float *disparity, uint16_t *sums, hipStream_t stream);
+
+// The same for `count` pairs of one size class (width, height, disparities and paths of
+// `p`) whose descriptors are `table` in device memory, each kernel one launch for all.
+hipError_t launch_match_batch(const MatchParams &p, const PairDesc *table, int count,
+                              const Schedule &schedule, hipStream_t stream);
 
 // Camera model of the rectification stage: k[9], plumb_bob d[5], inverse_pr[9], all f32.
 struct Camera {

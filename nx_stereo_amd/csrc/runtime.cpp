@@ -43,8 +43,11 @@ struct Profiler {
     std::vector<int> kernel;
 };
 static Profiler g_profiler;
+// Launches captured into a graph are not bracketed by profiling events.
+static thread_local bool t_capturing = false;
 
 int profile_begin(ProfiledKernel k, hipStream_t stream) {
+    if (t_capturing) return -1;
     std::lock_guard<std::mutex> lock(g_profiler.mu);
     if (!g_profiler.on || g_profiler.used >= (int)g_profiler.begin.size()) return -1;
     const int slot = g_profiler.used++;
@@ -55,6 +58,23 @@ int profile_begin(ProfiledKernel k, hipStream_t stream) {
 
 void profile_end(int slot, hipStream_t stream) {
     if (slot >= 0) (void)hipEventRecord(g_profiler.end[slot], stream);
+}
+
+// ------------------------------------------------------------------------------------------
+// Aggregation schedules per size class (kernels.hpp): the autotuner's picks.
+
+static std::mutex g_schedules_mu;
+static std::map<std::tuple<int, int, int, int>, Schedule> g_schedules;
+
+Schedule schedule_for(int width, int height, int disparities, int paths) {
+    std::lock_guard<std::mutex> lock(g_schedules_mu);
+    auto it = g_schedules.find(std::make_tuple(width, height, disparities, paths));
+    return it == g_schedules.end() ? Schedule{} : it->second;
+}
+
+void remember_schedule(int width, int height, int disparities, int paths, const Schedule &s) {
+    std::lock_guard<std::mutex> lock(g_schedules_mu);
+    g_schedules[std::make_tuple(width, height, disparities, paths)] = s;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1341,6 +1361,187 @@ int nxs_profile_stop(double *total_ms, int *launches, int kernels) {
     g_profiler.used = 0;
     return NXS_OK;
 }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// The batched form: many pairs, grouped into size classes, each kernel one launch per
+// class over a descriptor table in device memory, the whole batch captured once into a
+// hipGraph and replayed. The autotuner times every aggregation schedule per class on the
+// batch itself and keeps the fastest.
+
+namespace nxs {
+
+struct BatchClass {
+    MatchParams p;
+    int first = 0, count = 0;  // its descriptors: tables[first .. first+count)
+    Schedule schedule;
+    float tuned_ms = 0;
+};
+
+}  // namespace nxs
+
+struct nxs_batch {
+    std::vector<nxs::BatchClass> classes;
+    void *workspace = nullptr;
+    nxs::PairDesc *tables = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::string description;
+
+    ~nxs_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (tables) (void)hipFree(tables);
+        if (workspace) (void)hipFree(workspace);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void enqueue_all(hipStream_t s) {
+        using namespace nxs;
+        for (auto &c : classes)
+            HIP_TRY(nxs::launch_match_batch(c.p, tables + c.first, c.count, c.schedule, s));
+    }
+
+    // Times each candidate schedule on this class's pairs: one warm run, then three.
+    void tune(nxs::BatchClass &c) {
+        using namespace nxs;
+        hipEvent_t start, stop;
+        HIP_TRY(hipEventCreate(&start));
+        HIP_TRY(hipEventCreate(&stop));
+        float best = 1e30f;
+        for (int prefetch : {2, 4, 8})
+            for (bool split : {false, true}) {
+                const Schedule s{prefetch, split};
+                HIP_TRY(launch_match_batch(c.p, tables + c.first, c.count, s, stream));
+                HIP_TRY(hipEventRecord(start, stream));
+                for (int r = 0; r < 3; r++)
+                    HIP_TRY(launch_match_batch(c.p, tables + c.first, c.count, s, stream));
+                HIP_TRY(hipEventRecord(stop, stream));
+                HIP_TRY(hipEventSynchronize(stop));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, start, stop));
+                if (ms < best) {
+                    best = ms;
+                    c.schedule = s;
+                }
+            }
+        c.tuned_ms = best / 3;
+        (void)hipEventDestroy(start);
+        (void)hipEventDestroy(stop);
+        remember_schedule(c.p.width, c.p.height, c.p.disparities, c.p.paths, c.schedule);
+    }
+};
+
+extern "C" {
+
+int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch **out) {
+    using namespace nxs;
+    if (!pairs || count < 1 || !out) return fail(NXS_EINVAL, "no pairs or output pointer");
+    auto batch = std::make_unique<nxs_batch>();
+    try {
+        // Size classes in order of first appearance; each pair's workspace in one allocation.
+        std::vector<std::vector<int>> members;
+        for (int i = 0; i < count; i++) {
+            const nxs_pair &q = pairs[i];
+            if (!q.left || !q.right || !q.disparity) throw Error{"a pair has a null pointer"};
+            check_matching(q.width, q.height, q.disparities, q.paths, q.p1, q.p2, q.uniqueness);
+            if (q.p1 < 0) throw Error{"p1 must be at least 0"};
+            int c = 0;
+            for (; c < (int)batch->classes.size(); c++) {
+                const MatchParams &p = batch->classes[c].p;
+                if (p.width == q.width && p.height == q.height && p.disparities == q.disparities &&
+                    p.paths == q.paths)
+                    break;
+            }
+            if (c == (int)batch->classes.size()) {
+                BatchClass cls;
+                cls.p = MatchParams{q.width, q.height, q.disparities, q.paths, q.p1, q.p2, q.uniqueness};
+                cls.schedule = schedule_for(q.width, q.height, q.disparities, q.paths);
+                batch->classes.push_back(cls);
+                members.emplace_back();
+            }
+            members[c].push_back(i);
+        }
+        size_t total = 0;
+        for (size_t c = 0; c < batch->classes.size(); c++) {
+            const MatchParams &p = batch->classes[c].p;
+            total += up256(match_bytes(p.width, p.height, p.disparities, p.paths)) * members[c].size();
+        }
+        HIP_TRY(hipMalloc(&batch->workspace, total));
+        HIP_TRY(hipMalloc((void **)&batch->tables, sizeof(PairDesc) * count));
+        HIP_TRY(hipStreamCreateWithFlags(&batch->stream, hipStreamNonBlocking));
+        std::vector<PairDesc> host(count);
+        uint8_t *ws = static_cast<uint8_t *>(batch->workspace);
+        int next = 0;
+        for (size_t c = 0; c < batch->classes.size(); c++) {
+            BatchClass &cls = batch->classes[c];
+            cls.first = next;
+            cls.count = (int)members[c].size();
+            const size_t bytes = up256(match_bytes(cls.p.width, cls.p.height, cls.p.disparities, cls.p.paths));
+            for (int i : members[c]) {
+                const nxs_pair &q = pairs[i];
+                PairDesc &d = host[next++];
+                d.left = q.left;
+                d.right = q.right;
+                d.disparity = q.disparity;
+                d.sums = nullptr;
+                match_carve(d.b, ws, q.width, q.height, q.disparities, q.paths);
+                ws += bytes;
+                d.p1 = q.p1;
+                d.p2 = q.p2;
+                d.uniqueness = q.uniqueness;
+            }
+        }
+        HIP_TRY(hipMemcpy(batch->tables, host.data(), sizeof(PairDesc) * count, hipMemcpyHostToDevice));
+        if (autotune)
+            for (auto &c : batch->classes) batch->tune(c);
+        // One graph for the whole batch.
+        t_capturing = true;
+        hipError_t e = hipStreamBeginCapture(batch->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            try {
+                batch->enqueue_all(batch->stream);
+            } catch (const Error &) {
+                hipGraph_t broken = nullptr;
+                (void)hipStreamEndCapture(batch->stream, &broken);
+                if (broken) (void)hipGraphDestroy(broken);
+                t_capturing = false;
+                throw;
+            }
+            e = hipStreamEndCapture(batch->stream, &batch->graph);
+        }
+        t_capturing = false;
+        HIP_TRY(e);
+        HIP_TRY(hipGraphInstantiate(&batch->exec, batch->graph, nullptr, nullptr, 0));
+        std::ostringstream text;
+        for (auto &c : batch->classes)
+            text << c.count << " x " << c.p.width << "x" << c.p.height << "x" << c.p.disparities
+                 << " paths " << c.p.paths << ": prefetch " << c.schedule.prefetch << ", "
+                 << (c.schedule.split ? "one launch per orientation" : "orientations in one launch")
+                 << (autotune ? " (tuned, " + std::to_string(c.tuned_ms) + " ms)" : "") << "\n";
+        batch->description = text.str();
+    } catch (const Error &e) {
+        return fail(NXS_EDEVICE, e.reason);
+    }
+    *out = batch.release();
+    return NXS_OK;
+}
+
+int nxs_batch_run(nxs_batch *batch, void *stream) {
+    if (!batch) return nxs::fail(NXS_EINVAL, "no batch");
+    hipError_t e = hipGraphLaunch(batch->exec, (hipStream_t)stream);
+    if (e != hipSuccess) return nxs::fail(NXS_EDEVICE, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    return NXS_OK;
+}
+
+const char *nxs_batch_describe(nxs_batch *batch) {
+    return batch ? batch->description.c_str() : "";
+}
+
+void nxs_batch_destroy(nxs_batch *batch) { delete batch; }
 
 const char *nxs_last_error(void) { return nxs::g_last_error.c_str(); }
 

@@ -13,10 +13,15 @@
 // up-right). Each line is independent and owned by one 16-lane DPP row: lane
 // i holds disparities [i*K, i*K+K) as K/2 packed u16 pairs, neighbours d-1
 // and d+1 across lanes come from row_shr/row_shl DPP moves, and the minimum
-// over D is a 4-step row_ror DPP reduction. The forward direction writes its
-// byte path costs into the orientation's volume; the backward direction reads
-// them back and writes the byte sum of both (each <= p2+48 <= 127, so the
-// sum fits a byte). Selection sums the orientations.
+// over D is a 4-step row_ror DPP reduction. Both directions of a line run at
+// once from its two ends; each stores its byte path costs until they meet,
+// then adds the other's and stores the byte sum of both (each <= p2+48 <= 127,
+// so the sum fits a byte) into the orientation's volume. Selection sums the
+// orientations.
+//
+// Every kernel takes one pair by value, or a table of pairs of one size class
+// in device memory with the grid's last dimension indexing it (the batched
+// form: one launch per kernel for the whole class).
 #include "device.hpp"
 #include "kernels.hpp"
 #include "profile.hpp"
@@ -28,14 +33,21 @@ namespace nxs {
 // dy in (-3..3 without 0) x dx in (-4..4 without 0), is set iff the clamped
 // neighbour + 1 < the centre.
 
-__global__ __launch_bounds__(256) void census_kernel(const uint8_t *__restrict__ left,
-                                                     const uint8_t *__restrict__ right, int W,
-                                                     int H, uint64_t *__restrict__ cl,
-                                                     uint64_t *__restrict__ cr) {
+// The pair a block works on: entry `index` of the batch table, or the one passed by value.
+__device__ __forceinline__ PairDesc pair_of(const PairDesc &single, const PairDesc *table,
+                                            int index) {
+    return table ? table[index] : single;
+}
+
+// Grid z: the pair times two, plus the image (0 left, 1 right).
+__global__ __launch_bounds__(256) void census_kernel(PairDesc single, const PairDesc *table, int W,
+                                                     int H) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
-    const uint8_t *img = blockIdx.z ? right : left;
-    uint64_t *out = blockIdx.z ? cr : cl;
+    const int side = blockIdx.z & 1;
+    const PairDesc pd = pair_of(single, table, blockIdx.z >> 1);
+    const uint8_t *img = side ? pd.right : pd.left;
+    uint64_t *out = side ? pd.b.census_right : pd.b.census_left;
     if (x >= W) return;
     const int centre = img[(size_t)y * W + x];
     uint64_t f = 0;
@@ -66,10 +78,14 @@ __global__ __launch_bounds__(256) void census_kernel(const uint8_t *__restrict__
 constexpr int COST_BLOCK = 256;
 constexpr int MAX_D = 256;
 
-__global__ __launch_bounds__(COST_BLOCK) void cost_kernel(const uint64_t *__restrict__ cl,
-                                                          const uint64_t *__restrict__ cr, int W,
-                                                          int D, uint8_t *__restrict__ costs) {
+// Grid z: the pair.
+__global__ __launch_bounds__(COST_BLOCK) void cost_kernel(PairDesc single, const PairDesc *table,
+                                                          int W, int D) {
     __shared__ uint64_t right[COST_BLOCK + MAX_D];
+    const PairDesc pd = pair_of(single, table, blockIdx.z);
+    const uint64_t *__restrict__ cl = pd.b.census_left;
+    const uint64_t *__restrict__ cr = pd.b.census_right;
+    uint8_t *__restrict__ costs = pd.b.costs;
     const int chunks = D >> 4;
     const int tile = COST_BLOCK / chunks;
     const int y = blockIdx.y;
@@ -251,31 +267,36 @@ __device__ __forceinline__ LineGeometry line_geometry(int o, int i, int W, int H
 }
 
 struct PathArgs {
-    const uint8_t *costs;
-    uint8_t *paired[4];
-    int W, H, p1, p2;
+    PairDesc single;
+    const PairDesc *table;
+    int W, H;
     int backward_columns;  // 0 with 3 paths: columns go down only
-    int block_start[4];    // first block of each orientation
+    // The launch covers orientations first..last; block_start[o] is the first block of o.
+    int first, last;
+    int block_start[4];
 };
 
 // Lines per block: one wave, four 16-lane DPP rows, one line each.
 constexpr int PATH_BLOCK = 64;
 constexpr int LINES_PER_BLOCK = 4;
-// Steps of costs (and, going back, forward path costs) loaded ahead of use.
-constexpr int PREFETCH = 4;
 
-template <int K>
+// Grid y: the pair. PREFETCH: steps of costs (and, past the middle, of the other
+// direction's path costs) loaded ahead of use.
+template <int K, int PREFETCH>
 __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
     constexpr int D = 16 * K;
     const int lane16 = threadIdx.x & 15;
     const int row = threadIdx.x >> 4;
     const int b = blockIdx.x;
-    int o = 3;
-    if (b < a.block_start[1]) o = 0;
-    else if (b < a.block_start[2]) o = 1;
-    else if (b < a.block_start[3]) o = 2;
+    int o = a.last;
+    for (int k = a.first; k < a.last; k++)
+        if (b < a.block_start[k + 1]) {
+            o = k;
+            break;
+        }
     const int first_line = (b - a.block_start[o]) * LINES_PER_BLOCK;
     const int W = a.W, H = a.H;
+    const PairDesc pd = pair_of(a.single, a.table, blockIdx.y);
 
     const LineGeometry g = line_geometry(o, first_line + row, W, H);
     int longest = 0;
@@ -290,11 +311,11 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
     // A line past the last one (length 0) reads the first pixel and writes nothing.
     const size_t first = g.length > 0 ? ((size_t)g.y0 * W + g.x0) * D : 0;
     const int last = max(g.length - 1, 0);
-    const uint8_t *costs = a.costs + first + (size_t)lane16 * K;
-    uint8_t *paired = a.paired[o] + first + (size_t)lane16 * K;
+    const uint8_t *costs = pd.b.costs + first + (size_t)lane16 * K;
+    uint8_t *paired = pd.b.paired[o] + first + (size_t)lane16 * K;
     auto offset = [&](int t) { return (long)min(max(t, 0), last) * stride; };
-    const uint32_t p1x2 = (uint32_t)a.p1 * 0x10001u;
-    const uint32_t p2 = (uint32_t)a.p2;
+    const uint32_t p1x2 = (uint32_t)pd.p1 * 0x10001u;
+    const uint32_t p2 = (uint32_t)pd.p2;
     const int L = g.length;
 
     if (o == 1 && !a.backward_columns) {
@@ -386,16 +407,20 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
 // ------------------------------------------------------------------------------------------
 // Selection (reference.rs:148-186; nx_stereo.ex:608-641), 16 lanes per pixel.
 
+// Grid y: the pair.
 template <int K>
-__global__ __launch_bounds__(256) void select_kernel(const uint8_t *__restrict__ p0,
-                                                     const uint8_t *__restrict__ p1,
-                                                     const uint8_t *__restrict__ p2,
-                                                     const uint8_t *__restrict__ p3,
-                                                     int orientations, int W, int pixels,
-                                                     float uniqueness, float *__restrict__ selected,
-                                                     uint32_t *__restrict__ keys,
-                                                     uint16_t *__restrict__ sums_out) {
+__global__ __launch_bounds__(256) void select_kernel(PairDesc single, const PairDesc *table,
+                                                     int orientations, int W, int pixels) {
     constexpr int D = 16 * K;
+    const PairDesc pd = pair_of(single, table, blockIdx.y);
+    const uint8_t *__restrict__ p0 = pd.b.paired[0];
+    const uint8_t *__restrict__ p1 = pd.b.paired[1];
+    const uint8_t *__restrict__ p2 = pd.b.paired[2];
+    const uint8_t *__restrict__ p3 = pd.b.paired[3];
+    const float uniqueness = pd.uniqueness;
+    float *__restrict__ selected = pd.b.selected;
+    uint32_t *__restrict__ keys = pd.b.keys;
+    uint16_t *__restrict__ sums_out = pd.sums;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane16 = threadIdx.x & 15;
     const long pix_raw = t >> 4;
@@ -479,10 +504,14 @@ __global__ __launch_bounds__(256) void select_kernel(const uint8_t *__restrict__
 
 constexpr int MAX_WIDTH = 4096;
 
-__global__ __launch_bounds__(256) void consistency_kernel(const float *__restrict__ selected,
-                                                          const uint32_t *__restrict__ keys, int W,
-                                                          float *__restrict__ checked) {
+// Grid y: the pair.
+__global__ __launch_bounds__(256) void consistency_kernel(PairDesc single, const PairDesc *table,
+                                                          int W) {
     __shared__ uint32_t right[MAX_WIDTH];
+    const PairDesc pd = pair_of(single, table, blockIdx.y);
+    const float *__restrict__ selected = pd.b.selected;
+    const uint32_t *__restrict__ keys = pd.b.keys;
+    float *__restrict__ checked = pd.b.checked;
     const int y = blockIdx.x;
     const float *sel = selected + (size_t)y * W;
     const uint32_t *row = keys + (size_t)y * W;
@@ -524,8 +553,12 @@ __global__ __launch_bounds__(256) void consistency_kernel(const float *__restric
 // on i32 bit patterns, which order -1.0 and non-negative values as f32 does; then the
 // Disparity stage's d > 0 ? d : -1 (stage/disparity.ex:162).
 
-__global__ __launch_bounds__(256) void median_kernel(const float *__restrict__ in, int W, int H,
-                                                     float *__restrict__ out) {
+// Grid z: the pair.
+__global__ __launch_bounds__(256) void median_kernel(PairDesc single, const PairDesc *table, int W,
+                                                     int H) {
+    const PairDesc pd = pair_of(single, table, blockIdx.z);
+    const float *__restrict__ in = pd.b.checked;
+    float *__restrict__ out = pd.disparity;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= W) return;
@@ -589,68 +622,109 @@ void match_carve(MatchBuffers &b, void *base, int W, int H, int D, int paths) {
     b.checked = reinterpret_cast<float *>(p);
 }
 
+// ------------------------------------------------------------------------------------------
+// Launchers. One launch per kernel covers `count` pairs of one size class: grid y (or z)
+// indexes the pair, whose descriptor comes by value (count 1) or from `table`.
+
+template <int K, int PREFETCH>
+static void launch_paths(const PathArgs &a, int blocks, int count, hipStream_t stream) {
+    hipLaunchKernelGGL((path_kernel<K, PREFETCH>), dim3(blocks, count), dim3(PATH_BLOCK), 0, stream,
+                       a);
+}
+
 template <int K>
-static hipError_t launch_paths_and_select(const MatchParams &p, MatchBuffers &b, uint16_t *sums,
+static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &single,
+                                          const PairDesc *table, int count, const Schedule &sch,
                                           hipStream_t stream) {
     const int W = p.width, H = p.height;
     const int n = orientations_of(p.paths);
+    const int lines[4] = {H, W, W + H - 1, W + H - 1};
     PathArgs a;
-    a.costs = b.costs;
-    for (int o = 0; o < 4; o++) a.paired[o] = b.paired[o] ? b.paired[o] : b.paired[0];
+    a.single = single;
+    a.table = table;
     a.W = W;
     a.H = H;
-    a.p1 = p.p1;
-    a.p2 = p.p2;
     a.backward_columns = p.paths == 8;
-    const int lines[4] = {H, W, W + H - 1, W + H - 1};
-    int blocks = 0;
-    for (int o = 0; o < 4; o++) {
-        a.block_start[o] = blocks;
-        if (o < n) blocks += (lines[o] + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
-    }
+    auto launch = [&](int first, int last) {
+        a.first = first;
+        a.last = last;
+        int blocks = 0;
+        for (int o = 0; o < 4; o++) {
+            a.block_start[o] = blocks;
+            if (o >= first && o <= last) blocks += (lines[o] + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
+        }
+        switch (sch.prefetch) {
+            case 2: launch_paths<K, 2>(a, blocks, count, stream); break;
+            case 8: launch_paths<K, 8>(a, blocks, count, stream); break;
+            default: launch_paths<K, 4>(a, blocks, count, stream); break;
+        }
+    };
     int slot = profile_begin(PK_PATH, stream);
-    hipLaunchKernelGGL(path_kernel<K>, dim3(blocks), dim3(PATH_BLOCK), 0, stream, a);
+    if (sch.split) {
+        for (int o = 0; o < n; o++) launch(o, o);
+    } else {
+        launch(0, n - 1);
+    }
     profile_end(slot, stream);
     const long threads = (long)W * H * 16;
     slot = profile_begin(PK_SELECT, stream);
-    hipLaunchKernelGGL(select_kernel<K>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
-                       stream, b.paired[0], b.paired[1], b.paired[2], b.paired[3], n, W, W * H,
-                       p.uniqueness, b.selected, b.keys, sums);
+    hipLaunchKernelGGL(select_kernel<K>, dim3((unsigned)((threads + 255) / 256), count), dim3(256), 0,
+                       stream, single, table, n, W, W * H);
+    profile_end(slot, stream);
+    return hipGetLastError();
+}
+
+static hipError_t launch_all(const MatchParams &p, const PairDesc &single, const PairDesc *table,
+                             int count, const Schedule &sch, hipStream_t stream) {
+    const int W = p.width, H = p.height, D = p.disparities;
+    if (W > MAX_WIDTH || W < 1 || H < 1 || count < 1) return hipErrorInvalidValue;
+    int slot = profile_begin(PK_CENSUS, stream);
+    hipLaunchKernelGGL(census_kernel, dim3((W + 255) / 256, H, 2 * count), dim3(256), 0, stream,
+                       single, table, W, H);
+    profile_end(slot, stream);
+    const int tile = COST_BLOCK / (D / 16);
+    slot = profile_begin(PK_COST, stream);
+    hipLaunchKernelGGL(cost_kernel, dim3((W + tile - 1) / tile, H, count), dim3(COST_BLOCK), 0,
+                       stream, single, table, W, D);
+    profile_end(slot, stream);
+    hipError_t e;
+    switch (D) {
+        case 64: e = launch_paths_and_select<4>(p, single, table, count, sch, stream); break;
+        case 128: e = launch_paths_and_select<8>(p, single, table, count, sch, stream); break;
+        case 192: e = launch_paths_and_select<12>(p, single, table, count, sch, stream); break;
+        case 256: e = launch_paths_and_select<16>(p, single, table, count, sch, stream); break;
+        default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    slot = profile_begin(PK_CONSISTENCY, stream);
+    hipLaunchKernelGGL(consistency_kernel, dim3(H, count), dim3(256), 0, stream, single, table, W);
+    profile_end(slot, stream);
+    slot = profile_begin(PK_MEDIAN, stream);
+    hipLaunchKernelGGL(median_kernel, dim3((W + 255) / 256, H, count), dim3(256), 0, stream, single,
+                       table, W, H);
     profile_end(slot, stream);
     return hipGetLastError();
 }
 
 hipError_t launch_match(const MatchParams &p, const uint8_t *left, const uint8_t *right,
                         MatchBuffers &b, float *disparity, uint16_t *sums, hipStream_t stream) {
-    const int W = p.width, H = p.height, D = p.disparities;
-    if (W > MAX_WIDTH || W < 1 || H < 1) return hipErrorInvalidValue;
-    int slot = profile_begin(PK_CENSUS, stream);
-    hipLaunchKernelGGL(census_kernel, dim3((W + 255) / 256, H, 2), dim3(256), 0, stream, left,
-                       right, W, H, b.census_left, b.census_right);
-    profile_end(slot, stream);
-    const int tile = COST_BLOCK / (D / 16);
-    slot = profile_begin(PK_COST, stream);
-    hipLaunchKernelGGL(cost_kernel, dim3((W + tile - 1) / tile, H), dim3(COST_BLOCK), 0, stream,
-                       b.census_left, b.census_right, W, D, b.costs);
-    profile_end(slot, stream);
-    hipError_t e;
-    switch (D) {
-        case 64: e = launch_paths_and_select<4>(p, b, sums, stream); break;
-        case 128: e = launch_paths_and_select<8>(p, b, sums, stream); break;
-        case 192: e = launch_paths_and_select<12>(p, b, sums, stream); break;
-        case 256: e = launch_paths_and_select<16>(p, b, sums, stream); break;
-        default: return hipErrorInvalidValue;
-    }
-    if (e != hipSuccess) return e;
-    slot = profile_begin(PK_CONSISTENCY, stream);
-    hipLaunchKernelGGL(consistency_kernel, dim3(H), dim3(256), 0, stream, b.selected, b.keys, W,
-                       b.checked);
-    profile_end(slot, stream);
-    slot = profile_begin(PK_MEDIAN, stream);
-    hipLaunchKernelGGL(median_kernel, dim3((W + 255) / 256, H), dim3(256), 0, stream, b.checked, W,
-                       H, disparity);
-    profile_end(slot, stream);
-    return hipGetLastError();
+    PairDesc single;
+    single.left = left;
+    single.right = right;
+    single.disparity = disparity;
+    single.sums = sums;
+    single.b = b;
+    single.p1 = p.p1;
+    single.p2 = p.p2;
+    single.uniqueness = p.uniqueness;
+    return launch_all(p, single, nullptr, 1, schedule_for(p.width, p.height, p.disparities, p.paths),
+                      stream);
+}
+
+hipError_t launch_match_batch(const MatchParams &p, const PairDesc *table, int count,
+                              const Schedule &schedule, hipStream_t stream) {
+    PairDesc unused{};
+    return launch_all(p, unused, table, count, schedule, stream);
 }
 
 }  // namespace nxs

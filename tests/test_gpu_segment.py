@@ -184,6 +184,8 @@ def test_refusals_map_like_the_port(context):
     reply = context.request(protocol.run(9001, [[]], [("disparity", np.zeros((2, 2), np.float32))]))
     assert "disparity is a [2, 2] tensor of type 2, expected a [24, 32] f32 map" in reply.reason
     reply = context.request(protocol.run(9001, [[]], []))
+    assert reply.reason == "no disparity among the segment's tensors"  # the map is read first
+    reply = context.request(protocol.run(9001, [[]], [("disparity", np.zeros((24, 32), np.float32))]))
     assert reply.reason == "no focal_length_px among the stage's inputs"
     assert isinstance(protocol.decode(bytes([9])), protocol.Refused)
     reply = context.request(bytes([2, 1, 0]))
