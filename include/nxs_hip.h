@@ -131,6 +131,13 @@ typedef struct nxs_pair {
 
 typedef struct nxs_batch nxs_batch;
 
+/* The aggregation schedule matcher launches of one size use: the autotuner's
+ * pick, or what nxs_schedule_set says (prefetch 2, 4 or 8 steps; split 1 for
+ * one launch per line orientation). Results are the same under every
+ * schedule; only the time differs. */
+int nxs_schedule_set(int width, int height, int disparities, int paths, int prefetch, int split);
+int nxs_schedule_get(int width, int height, int disparities, int paths, int *prefetch, int *split);
+
 int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch **out);
 int nxs_batch_run(nxs_batch *batch, void *stream);
 /* The classes and their schedules, one line each; valid while the batch lives. */

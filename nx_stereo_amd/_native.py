@@ -20,7 +20,9 @@ NXS_OK, NXS_TIMEOUT, NXS_EINVAL, NXS_EDEVICE, NXS_ENOMEM, NXS_EUNKNOWN = 0, 1, -
 SYMBOLS = (
     "nxs_open", "nxs_close", "nxs_submit", "nxs_wait", "nxs_reply", "nxs_release_reply",
     "nxs_poll", "nxs_set_hbm_budget", "nxs_stats", "nxs_match_workspace_size", "nxs_match_device", "nxs_match_host",
-    "nxs_run_device", "nxs_profile_start", "nxs_profile_stop", "nxs_last_error",
+    "nxs_run_device", "nxs_profile_start", "nxs_profile_stop", "nxs_schedule_set",
+    "nxs_schedule_get", "nxs_batch_create",
+    "nxs_batch_run", "nxs_batch_describe", "nxs_batch_destroy", "nxs_last_error",
 )
 
 DONE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int64,
@@ -34,6 +36,15 @@ class NxsTensor(ctypes.Structure):
 
     _fields_ = [("name", ctypes.c_char_p), ("type", ctypes.c_uint8), ("rank", ctypes.c_uint8),
                 ("shape", ctypes.c_uint32 * 4), ("data", ctypes.c_void_p)]
+
+
+class NxsPair(ctypes.Structure):
+    """struct nxs_pair: one pair of a batch, its images and output in device memory."""
+
+    _fields_ = [("left", ctypes.c_void_p), ("right", ctypes.c_void_p),
+                ("disparity", ctypes.c_void_p), ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("disparities", ctypes.c_int), ("paths", ctypes.c_int), ("p1", ctypes.c_int),
+                ("p2", ctypes.c_int), ("uniqueness", ctypes.c_float)]
 
 
 class NativeError(RuntimeError):
@@ -79,6 +90,18 @@ def load() -> ctypes.CDLL:
     lib.nxs_profile_start.restype = i
     lib.nxs_profile_stop.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i), i]
     lib.nxs_profile_stop.restype = i
+    lib.nxs_schedule_set.argtypes = [i, i, i, i, i, i]
+    lib.nxs_schedule_set.restype = i
+    lib.nxs_schedule_get.argtypes = [i, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i)]
+    lib.nxs_schedule_get.restype = i
+    lib.nxs_batch_create.argtypes = [ctypes.POINTER(NxsPair), i, i, ctypes.POINTER(P)]
+    lib.nxs_batch_create.restype = i
+    lib.nxs_batch_run.argtypes = [P, P]
+    lib.nxs_batch_run.restype = i
+    lib.nxs_batch_describe.argtypes = [P]
+    lib.nxs_batch_describe.restype = ctypes.c_char_p
+    lib.nxs_batch_destroy.argtypes = [P]
+    lib.nxs_batch_destroy.restype = None
     lib.nxs_last_error.argtypes = []
     lib.nxs_last_error.restype = ctypes.c_char_p
     _lib = lib
@@ -136,3 +159,47 @@ def match_device(left_ptr: int, right_ptr: int, width: int, height: int, dispari
         paths, p1, p2, float(np.float32(uniqueness)), ctypes.c_void_p(out_ptr),
         ctypes.c_void_p(workspace_ptr), workspace_bytes, ctypes.c_void_p(stream or None))
     _check(rc, "nxs_match_device")
+
+
+def set_schedule(width, height, disparities, paths, prefetch, split) -> None:
+    _check(load().nxs_schedule_set(width, height, disparities, paths, prefetch, int(split)),
+           "nxs_schedule_set")
+
+
+def get_schedule(width, height, disparities, paths) -> tuple[int, bool]:
+    prefetch, split = ctypes.c_int(), ctypes.c_int()
+    load().nxs_schedule_get(width, height, disparities, paths, ctypes.byref(prefetch),
+                            ctypes.byref(split))
+    return prefetch.value, bool(split.value)
+
+
+class Batch:
+    """nxs_batch: pairs (left, right, out device addresses; width, height, disparities,
+    paths, p1, p2, uniqueness) captured into one graph; run() replays it on a stream."""
+
+    def __init__(self, pairs, autotune: bool = True):
+        lib = load()
+        arr = (NxsPair * len(pairs))()
+        for k, (left, right, out, w, h, d, paths, p1, p2, u) in enumerate(pairs):
+            arr[k] = NxsPair(left, right, out, w, h, d, paths, p1, p2, float(np.float32(u)))
+        handle = ctypes.c_void_p()
+        _check(lib.nxs_batch_create(arr, len(pairs), int(autotune), ctypes.byref(handle)),
+               "nxs_batch_create")
+        self.lib, self.handle = lib, handle
+
+    def run(self, stream: int = 0) -> None:
+        _check(self.lib.nxs_batch_run(self.handle, ctypes.c_void_p(stream or None)), "nxs_batch_run")
+
+    def describe(self) -> str:
+        return self.lib.nxs_batch_describe(self.handle).decode()
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.nxs_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1405,6 +1405,8 @@ struct nxs_batch {
             HIP_TRY(nxs::launch_match_batch(c.p, tables + c.first, c.count, c.schedule, s));
     }
 
+    std::string trials;  // every candidate's time, for the description
+
     // Times each candidate schedule on this class's pairs: one warm run, then three.
     void tune(nxs::BatchClass &c) {
         using namespace nxs;
@@ -1423,6 +1425,8 @@ struct nxs_batch {
                 HIP_TRY(hipEventSynchronize(stop));
                 float ms = 0;
                 HIP_TRY(hipEventElapsedTime(&ms, start, stop));
+                trials += "  prefetch " + std::to_string(prefetch) + (split ? " split" : " fused") +
+                          ": " + std::to_string(ms / 3) + " ms\n";
                 if (ms < best) {
                     best = ms;
                     c.schedule = s;
@@ -1497,7 +1501,12 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
         }
         HIP_TRY(hipMemcpy(batch->tables, host.data(), sizeof(PairDesc) * count, hipMemcpyHostToDevice));
         if (autotune)
-            for (auto &c : batch->classes) batch->tune(c);
+            for (auto &c : batch->classes) {
+                batch->trials += std::to_string(c.count) + " x " + std::to_string(c.p.width) + "x" +
+                                 std::to_string(c.p.height) + "x" + std::to_string(c.p.disparities) +
+                                 ":\n";
+                batch->tune(c);
+            }
         // One graph for the whole batch.
         t_capturing = true;
         hipError_t e = hipStreamBeginCapture(batch->stream, hipStreamCaptureModeThreadLocal);
@@ -1522,11 +1531,25 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
                  << " paths " << c.p.paths << ": prefetch " << c.schedule.prefetch << ", "
                  << (c.schedule.split ? "one launch per orientation" : "orientations in one launch")
                  << (autotune ? " (tuned, " + std::to_string(c.tuned_ms) + " ms)" : "") << "\n";
-        batch->description = text.str();
+        batch->description = text.str() + batch->trials;
     } catch (const Error &e) {
         return fail(NXS_EDEVICE, e.reason);
     }
     *out = batch.release();
+    return NXS_OK;
+}
+
+int nxs_schedule_set(int width, int height, int disparities, int paths, int prefetch, int split) {
+    if (prefetch != 2 && prefetch != 4 && prefetch != 8)
+        return nxs::fail(NXS_EINVAL, "prefetch must be 2, 4 or 8");
+    nxs::remember_schedule(width, height, disparities, paths, nxs::Schedule{prefetch, split != 0});
+    return NXS_OK;
+}
+
+int nxs_schedule_get(int width, int height, int disparities, int paths, int *prefetch, int *split) {
+    const nxs::Schedule s = nxs::schedule_for(width, height, disparities, paths);
+    if (prefetch) *prefetch = s.prefetch;
+    if (split) *split = s.split ? 1 : 0;
     return NXS_OK;
 }
 

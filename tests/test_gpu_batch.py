@@ -1,0 +1,70 @@
+"""The batched form (nxs_batch): pairs of several size classes in one captured graph,
+every output bit for bit the oracle's, on every replay, with new images between replays;
+and the autotuner's candidate schedules all give the same results."""
+import numpy as np
+import pytest
+
+from nx_stereo_amd import _native, hip, scenes
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+# (width, height, disparities, paths, p1, p2, uniqueness)
+CLASSES = [(97, 31, 64, 8, 8, 64, 1.0526316), (160, 48, 128, 8, 10, 70, 1.1),
+           (120, 40, 192, 8, 8, 64, 1.0526316), (64, 48, 64, 3, 4, 40, 1.2)]
+
+
+def make_pairs(seed):
+    pairs = []
+    for k, (w, h, d, paths, p1, p2, u) in enumerate(CLASSES):
+        for j in range(2 if k != 1 else 3):
+            left, right, _ = scenes.planar_pair(w, h, seed=seed * 100 + k * 10 + j, low=4,
+                                                high=min(d - 8, 60))
+            pairs.append((left, right, (w, h, d, paths, p1, p2, u)))
+    return pairs
+
+
+def test_batch_matches_the_oracle_on_every_replay():
+    pairs = make_pairs(1)
+    bufs = []
+    for left, right, (w, h, d, paths, p1, p2, u) in pairs:
+        bufs.append((hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h)))
+    batch = _native.Batch([(l.ptr, r.ptr, o.ptr, *cfg) for (l, r, o), (_, _, cfg) in zip(bufs, pairs)],
+                          autotune=True)
+    print(batch.describe(), flush=True)
+    stream = hip.Stream()
+    for replay in range(3):
+        if replay:
+            pairs = [(*make_pairs(replay + 1)[i][:2], pairs[i][2]) for i in range(len(pairs))]
+            for (left, right, _), (l, r, _o) in zip(pairs, bufs):
+                l.upload(left)
+                r.upload(right)
+        batch.run(stream.handle)
+        stream.synchronize()
+        for (left, right, (w, h, d, paths, p1, p2, u)), (_l, _r, o) in zip(pairs, bufs):
+            got = o.download((h, w), np.float32)
+            want = oracle.sgm(left, right, d, paths, p1, p2, u)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (replay, w, h, d)
+    batch.close()
+
+
+@pytest.mark.parametrize("disparities,paths", [(64, 8), (128, 8), (128, 3), (256, 8)])
+def test_every_schedule_gives_the_same_disparity(disparities, paths):
+    """Tuning picks by time only: each candidate schedule is checked bit for bit here."""
+    w, h = 97, 31
+    left, right = scenes.scene(w, h, 7)
+    want = oracle.sgm(left, right, disparities, paths)
+    l, r, o = hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h)
+    ws = hip.Buffer(_native.workspace_size(w, h, disparities, paths))
+    kept = _native.get_schedule(w, h, disparities, paths)
+    try:
+        for prefetch in (2, 4, 8):
+            for split in (False, True):
+                _native.set_schedule(w, h, disparities, paths, prefetch, split)
+                _native.match_device(l.ptr, r.ptr, w, h, disparities, paths, 8, 64, 1.0526316,
+                                     o.ptr, ws.ptr, ws.nbytes)
+                hip.synchronize()
+                got = o.download((h, w), np.float32)
+                assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (prefetch, split)
+    finally:
+        _native.set_schedule(w, h, disparities, paths, *kept)
