@@ -29,6 +29,28 @@ __device__ __forceinline__ uint32_t dpp(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
 }
 
+// The same move, where a lane whose source falls outside its row keeps `edge`.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_or(uint32_t edge, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+// Raw buffer access (MUBUF) with 32-bit byte offsets: an offset at or past `bytes` loads
+// zeros and drops stores, which masks an access without a branch.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr uint32_t MASKED = 0x80000000u;  // past every buffer made here (all under 2^31 bytes)
+
+// `base` must be the same in every lane of the wave: it is read from the first active lane,
+// so the descriptor lives in scalar registers.
+__device__ __forceinline__ Rsrc buffer_of(const void *base, uint32_t bytes) {
+    const uint64_t v = reinterpret_cast<uint64_t>(base);
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(hi << 32 | lo), (short)0,
+                                             (int)__builtin_amdgcn_readfirstlane((int)bytes),
+                                             0x00020000);
+}
+
 // Minimum over the 16 lanes of a DPP row, left in every lane of the row.
 __device__ __forceinline__ uint32_t row_min_u32(uint32_t v) {
     v = min(v, dpp<dpp_row_ror(8)>(v));

@@ -307,6 +307,11 @@ static void check_matching(int W, int H, int D, int paths, int p1, int p2, float
     if (D != 64 && D != 128 && D != 192 && D != 256)
         throw Error{"disparities must be 64, 128, 192 or 256, got " + std::to_string(D)};
     if (paths != 3 && paths != 8) throw Error{"paths must be 3 or 8, got " + std::to_string(paths)};
+    // The aggregation addresses each volume with 32-bit buffer offsets, and keeps 2^31 and
+    // above as the offset of a masked access.
+    if ((uint64_t)W * H * D >= (1ull << 31))
+        throw Error{"the cost volume (width x height x disparities) must stay under 2^31 bytes, got " +
+                    std::to_string(W) + "x" + std::to_string(H) + "x" + std::to_string(D)};
     if (!(p1 < p2 && p2 + 48 <= 127))
         throw Error{"expected 0 <= p1 < p2 <= 79, got " + std::to_string(p1) + " and " +
                     std::to_string(p2)};

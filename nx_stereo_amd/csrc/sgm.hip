@@ -70,50 +70,61 @@ __global__ __launch_bounds__(256) void census_kernel(PairDesc single, const Pair
 
 // ------------------------------------------------------------------------------------------
 // Costs (reference.rs:41-63): C[y,x,d] = popcount(FL[y,x] ^ FR[y,x-d]), FR = 0 left of
-// the image. A block takes a run of `tile` pixels of one row: the right features they
-// compare with, [x0-D+1, x0+tile), are staged in LDS once; each thread then takes one
-// pixel and 16 disparities and makes one 16-byte store, so the block's stores are one
-// contiguous run of tile*D bytes.
+// the image. A block takes a run of `tile` pixels of one row. The right features they
+// compare with, [x0-D+1, x0+tile), are staged in LDS once. Thread t takes pixel t % tile
+// and the 16 disparities of chunk t / tile, so the lanes of a wave read consecutive LDS
+// words (no bank conflicts); the bytes go through an LDS tile and leave as one contiguous
+// run of tile*D bytes, 16 bytes per thread.
 
 constexpr int COST_BLOCK = 256;
 constexpr int MAX_D = 256;
+constexpr int COST_PITCH_PAD = 16;  // LDS bytes between pixels of the staged tile, spreading banks
 
 // Grid z: the pair.
 __global__ __launch_bounds__(COST_BLOCK) void cost_kernel(PairDesc single, const PairDesc *table,
                                                           int W, int D) {
     __shared__ uint64_t right[COST_BLOCK + MAX_D];
+    // tile*(D + pad) bytes: at most 64 pixels (D = 64) or 4096 bytes of costs plus pads.
+    __shared__ uint4 staged[(COST_BLOCK * 16 + 64 * COST_PITCH_PAD) / 16];
     const PairDesc pd = pair_of(single, table, blockIdx.z);
     const uint64_t *__restrict__ cl = pd.b.census_left;
     const uint64_t *__restrict__ cr = pd.b.census_right;
     uint8_t *__restrict__ costs = pd.b.costs;
     const int chunks = D >> 4;
     const int tile = COST_BLOCK / chunks;
+    const int pitch = D + COST_PITCH_PAD;  // LDS bytes per pixel
     const int y = blockIdx.y;
     const int x0 = blockIdx.x * tile;
-    const int base = x0 - D + 1;  // the LDS window's first column
+    const int pixels = min(tile, W - x0);
     const uint64_t *row = cr + (size_t)y * W;
     for (int i = threadIdx.x; i < tile + D - 1; i += COST_BLOCK) {
-        const int x = base + i;
+        const int x = x0 - D + 1 + i;
         right[i] = (x >= 0 && x < W) ? row[x] : 0;
     }
     __syncthreads();
-    const int p = threadIdx.x / chunks, chunk = threadIdx.x - p * chunks;
-    const int x = x0 + p;
-    if (p >= tile || x >= W) return;
-    const uint64_t l = cl[(size_t)y * W + x];
-    uint32_t words[4];
+    const int p = threadIdx.x % tile, chunk = threadIdx.x / tile;
+    if (chunk < chunks && p < pixels) {
+        const uint64_t l = cl[(size_t)y * W + x0 + p];
+        // FR[x - d] for x = x0 + p sits at right[p + D - 1 - d].
+        const uint64_t *r = right + p + D - 1 - chunk * 16;
+        uint32_t words[4];
 #pragma unroll
-    for (int w = 0; w < 4; w++) {
-        uint32_t word = 0;
+        for (int w = 0; w < 4; w++) {
+            uint32_t word = 0;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int d = chunk * 16 + w * 4 + j;
-            word |= (uint32_t)__builtin_popcountll(l ^ right[x - d - base]) << (8 * j);
+            for (int j = 0; j < 4; j++)
+                word |= (uint32_t)__builtin_popcountll(l ^ r[-(w * 4 + j)]) << (8 * j);
+            words[w] = word;
         }
-        words[w] = word;
+        staged[(p * pitch + chunk * 16) / 16] = make_uint4(words[0], words[1], words[2], words[3]);
     }
-    *reinterpret_cast<uint4 *>(costs + ((size_t)y * W + x) * D + chunk * 16) =
-        make_uint4(words[0], words[1], words[2], words[3]);
+    __syncthreads();
+    // The run of pixels*D bytes at (y*W + x0)*D, 16 bytes per thread.
+    uint4 *out = reinterpret_cast<uint4 *>(costs + ((size_t)y * W + x0) * D);
+    for (int q = threadIdx.x; q < pixels * chunks; q += COST_BLOCK) {
+        const int pixel = q / chunks, c = q - pixel * chunks;
+        out[q] = staged[(pixel * pitch + c * 16) / 16];
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -168,6 +179,48 @@ struct Bytes<16> {
     }
 };
 
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+typedef uint32_t v3u __attribute__((ext_vector_type(3)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+
+// K bytes of one lane at a byte offset of a buffer, as whole dwords (MUBUF loads/stores).
+template <int K>
+__device__ __forceinline__ Bytes<K> vload(Rsrc r, uint32_t at) {
+    Bytes<K> b;
+    if constexpr (K == 4) {
+        b.w[0] = __builtin_amdgcn_raw_buffer_load_b32(r, at, 0, 0);
+    } else if constexpr (K == 8) {
+        const v2u v = __builtin_amdgcn_raw_buffer_load_b64(r, at, 0, 0);
+        b.w[0] = v[0];
+        b.w[1] = v[1];
+    } else if constexpr (K == 12) {
+        const v3u v = __builtin_amdgcn_raw_buffer_load_b96(r, at, 0, 0);
+        b.w[0] = v[0];
+        b.w[1] = v[1];
+        b.w[2] = v[2];
+    } else {
+        const v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, at, 0, 0);
+        b.w[0] = v[0];
+        b.w[1] = v[1];
+        b.w[2] = v[2];
+        b.w[3] = v[3];
+    }
+    return b;
+}
+
+template <int K>
+__device__ __forceinline__ void vstore(Rsrc r, uint32_t at, const Bytes<K> &b) {
+    if constexpr (K == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(b.w[0], r, at, 0, 0);
+    } else if constexpr (K == 8) {
+        __builtin_amdgcn_raw_buffer_store_b64((v2u){b.w[0], b.w[1]}, r, at, 0, 0);
+    } else if constexpr (K == 12) {
+        __builtin_amdgcn_raw_buffer_store_b96((v3u){b.w[0], b.w[1], b.w[2]}, r, at, 0, 0);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b128((v4u){b.w[0], b.w[1], b.w[2], b.w[3]}, r, at, 0, 0);
+    }
+}
+
 // Bytes b0 b1 b2 b3 of a dword as the u16 pairs (b0, b1) and (b2, b3).
 // v_perm_b32 (S0, S1, sel): result byte i is byte sel[i] of {S0:S1}, 0-3 from S1, 4-7 from
 // S0, and 0x0C reads zero.
@@ -195,13 +248,11 @@ struct PathState {
     // min(prev[d-1], prev[d+1]) + p1) - pm + C. Out-of-range neighbours are a
     // sentinel that only ever loses to the pm + p2 term, as libSGM's p2 - p1
     // padding does.
-    __device__ __forceinline__ void step(const Bytes<K> &cost, uint32_t p1x2, uint32_t p2,
-                                         int lane16) {
+    __device__ __forceinline__ void step(const Bytes<K> &cost, uint32_t p1x2, uint32_t p2) {
+        // Lanes 0 and 15 have no neighbour across the line's ends: they take the sentinel.
         constexpr uint32_t SENT = 0x3FFF3FFFu;
-        uint32_t left_in = dpp<DPP_ROW_SHR1>(v[R - 1]);
-        uint32_t right_in = dpp<DPP_ROW_SHL1>(v[0]);
-        left_in = lane16 == 0 ? SENT : left_in;
-        right_in = lane16 == 15 ? SENT : right_in;
+        const uint32_t left_in = dpp_or<DPP_ROW_SHR1>(SENT, v[R - 1]);
+        const uint32_t right_in = dpp_or<DPP_ROW_SHL1>(SENT, v[0]);
         uint32_t s[R + 1];
         s[0] = (v[0] << 16) | (left_in >> 16);
 #pragma unroll
@@ -307,33 +358,51 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
 
     const int dy = o == 0 ? 0 : 1;
     const int dx = o == 0 ? 1 : (o == 1 ? 0 : (o == 2 ? 1 : -1));
-    const long stride = (long)(dy * W + dx) * D;  // bytes from one step's pixel to the next
-    // A line past the last one (length 0) reads the first pixel and writes nothing.
-    const size_t first = g.length > 0 ? ((size_t)g.y0 * W + g.x0) * D : 0;
-    const int last = max(g.length - 1, 0);
-    const uint8_t *costs = pd.b.costs + first + (size_t)lane16 * K;
-    uint8_t *paired = pd.b.paired[o] + first + (size_t)lane16 * K;
-    auto offset = [&](int t) { return (long)min(max(t, 0), last) * stride; };
+    // Byte offsets in the volumes (all under 2^31): one step's pixel to the next, and this
+    // lane's bytes of the line's first and last pixels. A line past the last one (length 0)
+    // computes on whatever it loads and stores nothing.
+    const uint32_t stride = (uint32_t)((dy * W + dx) * D);
+    const int L = g.length;
+    const uint32_t first = ((uint32_t)g.y0 * W + (uint32_t)g.x0) * D + lane16 * K;
+    const uint32_t last = first + (uint32_t)max(L - 1, 0) * stride;
+    const uint32_t volume = (uint32_t)W * H * D;
+    const Rsrc costs = buffer_of(pd.b.costs, volume);
+    const Rsrc paired = buffer_of(pd.b.paired[o], volume);
     const uint32_t p1x2 = (uint32_t)pd.p1 * 0x10001u;
     const uint32_t p2 = (uint32_t)pd.p2;
-    const int L = g.length;
+    // Every step issues the same loads and stores, and the loops run whole groups of PREFETCH
+    // steps (up to PREFETCH-1 past the longest line, storing nothing): an access that must not
+    // happen takes the MASKED offset, and one past a line's end reads bytes it never uses. So
+    // the compiler counts outstanding accesses exactly and waits only for the step it needs.
 
     if (o == 1 && !a.backward_columns) {
         // Columns with 3 paths: down only, its own path costs.
         PathState<K> s;
         s.reset();
         Bytes<K> ring[PREFETCH];
+        // The prologue issues what a step issues (a masked store, then the load), so the
+        // loop is entered with the same accesses outstanding as it goes round with.
+        Bytes<K> none;
 #pragma unroll
-        for (int p = 0; p < PREFETCH; p++) ring[p].load(costs + offset(p));
+        for (int k = 0; k < K / 4; k++) none.w[k] = 0;
+#pragma unroll
+        for (int p = 0; p < PREFETCH; p++) {
+            __builtin_amdgcn_sched_barrier(0);
+            vstore<K>(paired, MASKED, none);
+            ring[p] = vload<K>(costs, first + p * stride);
+        }
+        uint32_t at = first, ahead = first + PREFETCH * stride;
         for (int t0 = 0; t0 < longest; t0 += PREFETCH) {
 #pragma unroll
             for (int p = 0; p < PREFETCH; p++) {
                 const int t = t0 + p;
-                if (t >= longest) break;  // wave-uniform
-                const Bytes<K> c = ring[p];
-                ring[p].load(costs + offset(t + PREFETCH));
-                s.step(c, p1x2, p2, lane16);
-                if (t < L) s.bytes().store(paired + offset(t));
+                // Keep the steps in program order, so each waits only for its own loads.
+                __builtin_amdgcn_sched_barrier(0);
+                s.step(ring[p], p1x2, p2);
+                vstore<K>(paired, t < L ? at : MASKED, s.bytes());
+                ring[p] = vload<K>(costs, ahead);
+                at += stride;
+                ahead += stride;
             }
         }
         return;
@@ -345,61 +414,78 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
     // the middle, each adds the other's (stored when that one passed) and stores the byte
     // sum (each <= p2 + 48 <= 127, so no byte carries into the next). An odd line's middle
     // pixel sums both directions in registers.
+    //
+    // The other's bytes for step j are loaded with its costs, PREFETCH steps ahead, when
+    // the other direction stored them by then: 2j - (L-1) >= PREFETCH. Just past the middle
+    // (0 < 2j - (L-1) < PREFETCH) they are younger than that, and come instead from the
+    // history ring of each direction's last PREFETCH own bytes.
     PathState<K> fwd, bwd;
     fwd.reset();
     bwd.reset();
     Bytes<K> ring_f[PREFETCH], ring_b[PREFETCH];    // costs, PREFETCH steps ahead
-    Bytes<K> other_f[PREFETCH], other_b[PREFETCH];  // the other direction's, past the middle
-    bool late[PREFETCH];  // the other's costs were not yet stored when prefetched
+    Bytes<K> other_f[PREFETCH], other_b[PREFETCH];  // the other direction's bytes, ditto
+    Bytes<K> hist_f[PREFETCH], hist_b[PREFETCH];    // own bytes of the last PREFETCH steps
 #pragma unroll
     for (int p = 0; p < PREFETCH; p++) {
-        ring_f[p].load(costs + offset(p));
-        ring_b[p].load(costs + offset(L - 1 - p));
-        late[p] = true;
+        // As a step issues: two (masked) stores, then the four loads; the masked loads of
+        // the other's bytes read zeros.
+#pragma unroll
+        for (int k = 0; k < K / 4; k++) hist_f[p].w[k] = hist_b[p].w[k] = 0;
+        __builtin_amdgcn_sched_barrier(0);
+        vstore<K>(paired, MASKED, hist_f[p]);
+        vstore<K>(paired, MASKED, hist_b[p]);
+        ring_f[p] = vload<K>(costs, first + p * stride);
+        ring_b[p] = vload<K>(costs, last - p * stride);
+        other_f[p] = vload<K>(paired, MASKED);
+        other_b[p] = vload<K>(paired, MASKED);
     }
+    uint32_t at_f = first, at_b = last;  // this step's pixels
+    uint32_t ahead_f = first + PREFETCH * stride, ahead_b = last - PREFETCH * stride;
     for (int i0 = 0; i0 < longest; i0 += PREFETCH) {
 #pragma unroll
         for (int p = 0; p < PREFETCH; p++) {
             const int i = i0 + p;
-            if (i >= longest) break;  // wave-uniform
-            const Bytes<K> cf = ring_f[p], cb = ring_b[p];
+            // Keep the steps in program order, so each waits only for its own loads.
+            __builtin_amdgcn_sched_barrier(0);
+            fwd.step(ring_f[p], p1x2, p2);
+            bwd.step(ring_b[p], p1x2, p2);
+            const Bytes<K> mf = fwd.bytes(), mb = bwd.bytes();
+            const int past = 2 * i - (L - 1);  // < 0 before the middle, 0 at an odd line's middle
             Bytes<K> of = other_f[p], ob = other_b[p];
-            const bool past = 2 * i > L - 1;  // this step is past the middle
-            if (past && late[p]) {
-                of.load(paired + offset(i));
-                ob.load(paired + offset(L - 1 - i));
-            }
-            // Prefetch step i + PREFETCH. Past the middle its values were stored by the other
-            // direction at step L-1-(i+PREFETCH), which must already be behind (< i).
-            const int j = i + PREFETCH;
-            ring_f[p].load(costs + offset(j));
-            ring_b[p].load(costs + offset(L - 1 - j));
-            late[p] = 2 * i + PREFETCH <= L - 1;
-            if (2 * j > L - 1 && !late[p]) {
-                other_f[p].load(paired + offset(j));
-                other_b[p].load(paired + offset(L - 1 - j));
-            }
-            fwd.step(cf, p1x2, p2, lane16);
-            bwd.step(cb, p1x2, p2, lane16);
-            if (i < L) {
-                Bytes<K> mf = fwd.bytes(), mb = bwd.bytes();
-                if (2 * i < L - 1) {
-                    mf.store(paired + offset(i));
-                    mb.store(paired + offset(L - 1 - i));
-                } else if (2 * i == L - 1) {
+            if (past > 0 && past < PREFETCH) {
+                // The other passed here at step i - past, which the history ring still holds.
+                const int s = (i - past) & (PREFETCH - 1);
 #pragma unroll
-                    for (int k = 0; k < K / 4; k++) mf.w[k] += mb.w[k];
-                    mf.store(paired + offset(i));
-                } else {
+                for (int q = 0; q < PREFETCH; q++) {
+                    if (q == p) continue;  // step i itself: never the other's
 #pragma unroll
                     for (int k = 0; k < K / 4; k++) {
-                        mf.w[k] += of.w[k];
-                        mb.w[k] += ob.w[k];
+                        of.w[k] = s == q ? hist_b[q].w[k] : of.w[k];
+                        ob.w[k] = s == q ? hist_f[q].w[k] : ob.w[k];
                     }
-                    mf.store(paired + offset(i));
-                    mb.store(paired + offset(L - 1 - i));
                 }
             }
+            hist_f[p] = mf;
+            hist_b[p] = mb;
+            Bytes<K> sf, sb;
+#pragma unroll
+            for (int k = 0; k < K / 4; k++) {
+                sf.w[k] = mf.w[k] + (past < 0 ? 0u : (past == 0 ? mb.w[k] : of.w[k]));
+                sb.w[k] = mb.w[k] + (past <= 0 ? 0u : ob.w[k]);
+            }
+            vstore<K>(paired, i < L ? at_f : MASKED, sf);
+            vstore<K>(paired, i < L && past != 0 ? at_b : MASKED, sb);
+            // Step i + PREFETCH: its costs, and the other's bytes when already stored (the
+            // stores just made come first in program order, so the loads see them).
+            ring_f[p] = vload<K>(costs, ahead_f);
+            ring_b[p] = vload<K>(costs, ahead_b);
+            const bool stored = 2 * (i + PREFETCH) - (L - 1) >= PREFETCH;
+            other_f[p] = vload<K>(paired, stored ? ahead_f : MASKED);
+            other_b[p] = vload<K>(paired, stored ? ahead_b : MASKED);
+            at_f += stride;
+            at_b -= stride;
+            ahead_f += stride;
+            ahead_b -= stride;
         }
     }
 }
