@@ -101,12 +101,172 @@ def cpu_baseline(geometry, members, left_frame, right_frame, budget_s: float = 2
     }
 
 
+def match_pair(width, height, disparities, seed):
+    """A rectified planar pair for a matcher-only leg (disparities 4 .. D-8)."""
+    from nx_stereo_amd import scenes
+
+    left, right, _ = scenes.planar_pair(width, height, seed=seed, low=4, high=disparities - 8)
+    return left, right
+
+
+def large_pair_leg(runs: int = 10) -> dict:
+    """One 1920x1080 pair at 256 disparities, 8 paths, through the matcher alone: the HBM
+    measurement (its u8 volume, 531 MB, and the aggregation's traffic exceed the caches)."""
+    from nx_stereo_amd import _native, hip, segment
+
+    w, h, d = 1920, 1080, 256
+    left, right = match_pair(w, h, d, SEED + 3)
+    l, r, o = hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h)
+    ws = hip.Buffer(_native.workspace_size(w, h, d, 8))
+    stream = hip.Stream()
+
+    def match():
+        _native.match_device(l.ptr, r.ptr, w, h, d, 8, 8, 64, 1.0526315789473684, o.ptr, ws.ptr,
+                             ws.nbytes, stream.handle)
+
+    for _ in range(2):
+        match()
+    stream.synchronize()
+    segment.profile_start(8 * runs + 8)
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(runs):
+        match()
+    e1.record(stream)
+    e1.synchronize()
+    kernels = segment.profile_stop()
+    ms = e0.elapsed_ms(e1) / runs
+    us = {k: v[0] / max(v[1], 1) * 1e3 for k, v in kernels.items()}
+    pixels, n = w * h, w * h * d
+    aggregation_us = us["path"] + us["select"]
+    agg = (6 * n + 8 * pixels) / (aggregation_us * 1e-6) / 1e9
+    cost = (n + 16 * pixels) / (us["cost"] * 1e-6) / 1e9
+    for b in (l, r, o, ws):
+        b.free()
+    return {
+        "workload": "one rectified planar pair 1920x1080, 256 disparities (4..248 px), 8 paths, "
+                    "Stage.Disparity's matcher alone",
+        "ms_per_pair": round(ms, 4), "pairs_per_s": round(1e3 / ms, 2),
+        "cost_evals_per_s": round(1e3 / ms * n, 1),
+        "kernel_us": {k: round(v, 2) for k, v in us.items()},
+        "roofline": {
+            "aggregation": {"achieved": round(agg, 1), "frac": round(agg / HBM_PEAK_GBS, 4),
+                            "algorithmic_bytes": 6 * n + 8 * pixels},
+            "cost": {"achieved": round(cost, 1), "frac": round(cost / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes": n + 16 * pixels},
+            "peak": HBM_PEAK_GBS, "unit": "GB/s"},
+    }
+
+
+BATCH_CLASSES = [(480, 270, 64, 24), (728, 544, 128, 48), (1280, 720, 192, 24)]
+
+
+def batched_leg(replays: int = 5) -> dict:
+    """96 pairs (24 x 480x270/64, 48 x 728x544/128, 24 x 1280x720/192) in one captured graph,
+    each class's aggregation schedule autotuned on the batch, against the same 96 pairs as
+    plain per-pair launches on one stream."""
+    from nx_stereo_amd import _native, hip
+
+    pairs, buffers = [], []
+    for k, (w, h, d, count) in enumerate(BATCH_CLASSES):
+        for j in range(count):
+            left, right = match_pair(w, h, d, SEED + 1000 * (k + 1) + j)
+            l, r, o = hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h)
+            buffers += [l, r, o]
+            pairs.append((l.ptr, r.ptr, o.ptr, w, h, d, 8, 8, 64, 1.0526315789473684))
+    batch = _native.Batch(pairs, autotune=True)
+    stream = hip.Stream()
+    batch.run(stream.handle)
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(replays):
+        batch.run(stream.handle)
+    e1.record(stream)
+    e1.synchronize()
+    batched_ms = e0.elapsed_ms(e1) / replays
+    describe = batch.describe()
+    batch.close()
+
+    # Plain: one nxs_match_device per pair, one workspace per size reused pair after pair.
+    spaces = {}
+    for (_, _, _, w, h, d, paths, *_rest) in pairs:
+        if (w, h, d) not in spaces:
+            spaces[(w, h, d)] = hip.Buffer(_native.workspace_size(w, h, d, paths))
+
+    def plain():
+        for (lp, rp, op, w, h, d, paths, p1, p2, u) in pairs:
+            ws = spaces[(w, h, d)]
+            _native.match_device(lp, rp, w, h, d, paths, p1, p2, u, op, ws.ptr, ws.nbytes,
+                                 stream.handle)
+
+    plain()
+    stream.synchronize()
+    e0.record(stream)
+    for _ in range(replays):
+        plain()
+    e1.record(stream)
+    e1.synchronize()
+    plain_ms = e0.elapsed_ms(e1) / replays
+    for b in list(spaces.values()) + buffers:
+        b.free()
+    n = sum(w * h * d * c for w, h, d, c in BATCH_CLASSES)
+    return {
+        "workload": "96 rectified planar pairs: 24 x 480x270/64, 48 x 728x544/128, "
+                    "24 x 1280x720/192, 8 paths, Stage.Disparity's matcher",
+        "pairs": len(pairs), "cost_evals_per_batch": n,
+        "batched_ms": round(batched_ms, 3), "batched_pairs_per_s": round(len(pairs) / batched_ms * 1e3, 2),
+        "plain_ms": round(plain_ms, 3), "plain_pairs_per_s": round(len(pairs) / plain_ms * 1e3, 2),
+        "speedup": round(plain_ms / batched_ms, 3),
+        "schedules": describe,
+    }
+
+
+SERVED_STREAMS = [(728, 544, 128, 30.0), (480, 270, 64, 60.0), (1280, 720, 192, 15.0)]
+
+
+def served_leg(duration_s: float = 1.5) -> dict:
+    """24 camera streams (8 x 728x544/128 at 30 Hz, 8 x 480x270/64 at 60 Hz, 8 x 1280x720/192
+    at 15 Hz) with Poisson arrivals, each Disparity -> TemporalFilter -> Quality with its
+    carries on the device, served continuously; then the same arrivals one pair at a time."""
+    from nx_stereo_amd import segment, serving
+
+    specs = [serving.StreamSpec(w, h, d, rate, SEED + 100 * k + j)
+             for k, (w, h, d, rate) in enumerate(SERVED_STREAMS) for j in range(8)]
+    arrivals = [serving.poisson_arrivals(spec.rate_hz, duration_s, spec.seed) for spec in specs]
+    warm = 3
+    frames = [serving.stream_frames(spec, len(a) + warm) for spec, a in zip(specs, arrivals)]
+    result = {"workload": "24 streams with Poisson arrivals: 8 x 728x544/128 at 30 Hz, "
+                          "8 x 480x270/64 at 60 Hz, 8 x 1280x720/192 at 15 Hz; planes drifting "
+                          "0.5 px per frame; Disparity -> TemporalFilter -> Quality per stream, "
+                          "host pairs in, disparity and quality sums back",
+              "duration_s": duration_s,
+              "offered_pairs_per_s": round(sum(len(a) for a in arrivals) / duration_s, 2)}
+    with segment.Context(0) as ctx:
+        segs = serving.configure_streams(ctx, specs)
+        records = serving.run_records(segs, frames)
+        warm_records = [r[:warm] for r in records]
+        timed = [r[warm:] for r in records]
+        serving.serve(ctx, segs, warm_records, [[0.0] * warm for _ in specs], all_at_once=True)
+        result["continuous"] = serving.serve(ctx, segs, timed, arrivals).summary()
+        result["one_at_a_time"] = serving.serve_one_at_a_time(ctx, segs, timed, arrivals).summary()
+        # Capacity: every pair of the trace submitted at once.
+        result["saturated"] = {
+            "continuous_pairs_per_s": serving.serve(ctx, segs, timed, arrivals,
+                                                    all_at_once=True).summary()["pairs_per_s"],
+            "one_at_a_time_pairs_per_s": serving.serve_one_at_a_time(
+                ctx, segs, timed, arrivals, all_at_once=True).summary()["pairs_per_s"]}
+    return result
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=200)
     parser.add_argument("--warmup", type=int, default=20)
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    parser.add_argument("--legs", default="large,batched,served",
+                        help="comma-separated extra legs: large, batched, served (or none)")
     args = parser.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one, and pairs shard across GPUs "
@@ -200,9 +360,18 @@ def main() -> int:
             "kernel_us": {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in kernels.items()},
         },
     }
+    context.close()
+    for frame in frames:
+        frame.free()
+    legs = {leg for leg in args.legs.split(",") if leg and leg != "none"}
+    if "large" in legs:
+        result["large_pair"] = large_pair_leg()
+    if "batched" in legs:
+        result["batched"] = batched_leg()
+    if "served" in legs:
+        result["served"] = served_leg()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(geometry, members, left_frame, right_frame)
-    context.close()
     print(json.dumps(result), flush=True)
     return 0
 

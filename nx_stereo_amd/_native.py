@@ -90,9 +90,10 @@ def load() -> ctypes.CDLL:
     lib.nxs_profile_start.restype = i
     lib.nxs_profile_stop.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i), i]
     lib.nxs_profile_stop.restype = i
-    lib.nxs_schedule_set.argtypes = [i, i, i, i, i, i]
+    lib.nxs_schedule_set.argtypes = [i, i, i, i, i, i, i]
     lib.nxs_schedule_set.restype = i
-    lib.nxs_schedule_get.argtypes = [i, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i)]
+    lib.nxs_schedule_get.argtypes = [i, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i),
+                                     ctypes.POINTER(i)]
     lib.nxs_schedule_get.restype = i
     lib.nxs_batch_create.argtypes = [ctypes.POINTER(NxsPair), i, i, ctypes.POINTER(P)]
     lib.nxs_batch_create.restype = i
@@ -161,16 +162,16 @@ def match_device(left_ptr: int, right_ptr: int, width: int, height: int, dispari
     _check(rc, "nxs_match_device")
 
 
-def set_schedule(width, height, disparities, paths, prefetch, split) -> None:
-    _check(load().nxs_schedule_set(width, height, disparities, paths, prefetch, int(split)),
+def set_schedule(width, height, disparities, paths, prefetch, split, lanes=16) -> None:
+    _check(load().nxs_schedule_set(width, height, disparities, paths, prefetch, int(split), lanes),
            "nxs_schedule_set")
 
 
-def get_schedule(width, height, disparities, paths) -> tuple[int, bool]:
-    prefetch, split = ctypes.c_int(), ctypes.c_int()
+def get_schedule(width, height, disparities, paths) -> tuple[int, bool, int]:
+    prefetch, split, lanes = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     load().nxs_schedule_get(width, height, disparities, paths, ctypes.byref(prefetch),
-                            ctypes.byref(split))
-    return prefetch.value, bool(split.value)
+                            ctypes.byref(split), ctypes.byref(lanes))
+    return prefetch.value, bool(split.value), lanes.value
 
 
 class Batch:

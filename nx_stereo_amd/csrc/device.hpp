@@ -60,6 +60,23 @@ __device__ __forceinline__ uint32_t row_min_u32(uint32_t v) {
     return v;
 }
 
+// Wave-wide DPP shifts by one lane (GFX9): lane i reads lane i-1 (shr) or i+1 (shl).
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
+
+// v_permlane16_swap with the same value in both operands leaves rows (0, 0, 2, 2) in one
+// result and rows (1, 1, 3, 3) in the other; v_permlane32_swap likewise the wave's halves.
+// On row-uniform values, the minimum of the two results is the minimum over rows 2r and
+// 2r+1 (over the halves), in every lane.
+__device__ __forceinline__ uint32_t pair_rows_min(uint32_t v) {
+    const auto s = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return min((uint32_t)s[0], (uint32_t)s[1]);
+}
+__device__ __forceinline__ uint32_t halves_min(uint32_t v) {
+    const auto s = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return min((uint32_t)s[0], (uint32_t)s[1]);
+}
+
 // Lane-wise u16 minimum of two halves over the 16 lanes of a DPP row.
 __device__ __forceinline__ uint32_t row_min_pk(uint32_t v) {
     v = pk_min(v, dpp<dpp_row_ror(8)>(v));

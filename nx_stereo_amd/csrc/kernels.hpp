@@ -42,12 +42,17 @@ struct PairDesc {
     float uniqueness;
 };
 
-// How the aggregation is scheduled: the steps of costs the path kernel loads ahead, and
-// whether the four line orientations go in one launch or in one launch each.
+// How the aggregation is scheduled: the steps of costs the path kernel loads ahead,
+// whether the four line orientations go in one launch or in one launch each, and how many
+// lanes share a line (16, 32 or 64; D / lanes disparities each, an even count up to 16).
 struct Schedule {
     int prefetch = 4;  // 2, 4 or 8
     bool split = false;
+    int lanes = 16;
 };
+
+// Whether `lanes` lanes per line fit D disparities (D / lanes is 2, 4, 8, 12 or 16).
+bool lanes_fit(int disparities, int lanes);
 
 // The schedule launches at this size use: the autotuner's pick, or the default.
 Schedule schedule_for(int width, int height, int disparities, int paths);

@@ -1419,24 +1419,29 @@ struct nxs_batch {
         HIP_TRY(hipEventCreate(&start));
         HIP_TRY(hipEventCreate(&stop));
         float best = 1e30f;
-        for (int prefetch : {2, 4, 8})
-            for (bool split : {false, true}) {
-                const Schedule s{prefetch, split};
+        // Every prefetch depth and lane count in one launch, and one launch per orientation.
+        std::vector<Schedule> candidates;
+        for (int lanes : {16, 32, 64})
+            if (lanes_fit(c.p.disparities, lanes))
+                for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, lanes});
+        candidates.push_back(Schedule{4, true, 16});
+        for (const Schedule &s : candidates) {
+            HIP_TRY(launch_match_batch(c.p, tables + c.first, c.count, s, stream));
+            HIP_TRY(hipEventRecord(start, stream));
+            for (int r = 0; r < 3; r++)
                 HIP_TRY(launch_match_batch(c.p, tables + c.first, c.count, s, stream));
-                HIP_TRY(hipEventRecord(start, stream));
-                for (int r = 0; r < 3; r++)
-                    HIP_TRY(launch_match_batch(c.p, tables + c.first, c.count, s, stream));
-                HIP_TRY(hipEventRecord(stop, stream));
-                HIP_TRY(hipEventSynchronize(stop));
-                float ms = 0;
-                HIP_TRY(hipEventElapsedTime(&ms, start, stop));
-                trials += "  prefetch " + std::to_string(prefetch) + (split ? " split" : " fused") +
-                          ": " + std::to_string(ms / 3) + " ms\n";
-                if (ms < best) {
-                    best = ms;
-                    c.schedule = s;
-                }
+            HIP_TRY(hipEventRecord(stop, stream));
+            HIP_TRY(hipEventSynchronize(stop));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, start, stop));
+            trials += "  prefetch " + std::to_string(s.prefetch) + ", " +
+                      std::to_string(s.lanes) + " lanes" + (s.split ? ", split" : "") + ": " +
+                      std::to_string(ms / 3) + " ms\n";
+            if (ms < best) {
+                best = ms;
+                c.schedule = s;
             }
+        }
         c.tuned_ms = best / 3;
         (void)hipEventDestroy(start);
         (void)hipEventDestroy(stop);
@@ -1534,6 +1539,7 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
         for (auto &c : batch->classes)
             text << c.count << " x " << c.p.width << "x" << c.p.height << "x" << c.p.disparities
                  << " paths " << c.p.paths << ": prefetch " << c.schedule.prefetch << ", "
+                 << c.schedule.lanes << " lanes per line, "
                  << (c.schedule.split ? "one launch per orientation" : "orientations in one launch")
                  << (autotune ? " (tuned, " + std::to_string(c.tuned_ms) + " ms)" : "") << "\n";
         batch->description = text.str() + batch->trials;
@@ -1544,17 +1550,24 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
     return NXS_OK;
 }
 
-int nxs_schedule_set(int width, int height, int disparities, int paths, int prefetch, int split) {
+int nxs_schedule_set(int width, int height, int disparities, int paths, int prefetch, int split,
+                     int lanes) {
     if (prefetch != 2 && prefetch != 4 && prefetch != 8)
         return nxs::fail(NXS_EINVAL, "prefetch must be 2, 4 or 8");
-    nxs::remember_schedule(width, height, disparities, paths, nxs::Schedule{prefetch, split != 0});
+    if (!nxs::lanes_fit(disparities, lanes))
+        return nxs::fail(NXS_EINVAL, std::to_string(lanes) + " lanes per line do not fit " +
+                                         std::to_string(disparities) + " disparities");
+    nxs::remember_schedule(width, height, disparities, paths,
+                           nxs::Schedule{prefetch, split != 0, lanes});
     return NXS_OK;
 }
 
-int nxs_schedule_get(int width, int height, int disparities, int paths, int *prefetch, int *split) {
+int nxs_schedule_get(int width, int height, int disparities, int paths, int *prefetch, int *split,
+                     int *lanes) {
     const nxs::Schedule s = nxs::schedule_for(width, height, disparities, paths);
     if (prefetch) *prefetch = s.prefetch;
     if (split) *split = s.split ? 1 : 0;
+    if (lanes) *lanes = s.lanes;
     return NXS_OK;
 }
 

@@ -131,7 +131,11 @@ __global__ __launch_bounds__(COST_BLOCK) void cost_kernel(PairDesc single, const
 // Path recurrence (reference.rs:65-121; paths.rs:19-42), on a 16-lane row.
 
 template <int K>
-struct Bytes;  // K bytes of one lane, loaded and stored as whole dwords
+struct Bytes;  // K bytes of one lane, loaded and stored as whole dwords (K = 2: one short)
+template <>
+struct Bytes<2> {
+    uint32_t w[1];  // the two bytes in the low half
+};
 template <>
 struct Bytes<4> {
     uint32_t w[1];
@@ -187,7 +191,9 @@ typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 template <int K>
 __device__ __forceinline__ Bytes<K> vload(Rsrc r, uint32_t at) {
     Bytes<K> b;
-    if constexpr (K == 4) {
+    if constexpr (K == 2) {
+        b.w[0] = __builtin_amdgcn_raw_buffer_load_b16(r, at, 0, 0);
+    } else if constexpr (K == 4) {
         b.w[0] = __builtin_amdgcn_raw_buffer_load_b32(r, at, 0, 0);
     } else if constexpr (K == 8) {
         const v2u v = __builtin_amdgcn_raw_buffer_load_b64(r, at, 0, 0);
@@ -210,7 +216,9 @@ __device__ __forceinline__ Bytes<K> vload(Rsrc r, uint32_t at) {
 
 template <int K>
 __device__ __forceinline__ void vstore(Rsrc r, uint32_t at, const Bytes<K> &b) {
-    if constexpr (K == 4) {
+    if constexpr (K == 2) {
+        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)b.w[0], r, at, 0, 0);
+    } else if constexpr (K == 4) {
         __builtin_amdgcn_raw_buffer_store_b32(b.w[0], r, at, 0, 0);
     } else if constexpr (K == 8) {
         __builtin_amdgcn_raw_buffer_store_b64((v2u){b.w[0], b.w[1]}, r, at, 0, 0);
@@ -231,11 +239,56 @@ __device__ __forceinline__ uint32_t pack_pairs(uint32_t a, uint32_t b) {
     return __builtin_amdgcn_perm(b, a, 0x06040200u);
 }
 
-template <int K>
+// A line's D disparities spread over LANES consecutive lanes of a wave (16: a DPP row; 32:
+// two rows; 64: the wave). Moves from the neighbouring lane of the same line, where the
+// line's first (last) lane takes `edge`, and the minimum over the line's lanes.
+template <int LANES>
+struct LineLanes;
+
+template <>
+struct LineLanes<16> {
+    static __device__ __forceinline__ uint32_t from_left(uint32_t edge, uint32_t v, int) {
+        return dpp_or<DPP_ROW_SHR1>(edge, v);
+    }
+    static __device__ __forceinline__ uint32_t from_right(uint32_t edge, uint32_t v, int) {
+        return dpp_or<DPP_ROW_SHL1>(edge, v);
+    }
+    static __device__ __forceinline__ uint32_t min(uint32_t v) { return row_min_u32(v); }
+};
+
+// Two rows: wave-wide shifts (lane 32 would read lane 31, which is another line's), and the
+// row minima of rows 2r and 2r+1 combined through v_permlane16_swap.
+template <>
+struct LineLanes<32> {
+    static __device__ __forceinline__ uint32_t from_left(uint32_t edge, uint32_t v, int lane) {
+        const uint32_t t = dpp_or<DPP_WAVE_SHR1>(edge, v);
+        return lane == 0 ? edge : t;
+    }
+    static __device__ __forceinline__ uint32_t from_right(uint32_t edge, uint32_t v, int lane) {
+        const uint32_t t = dpp_or<DPP_WAVE_SHL1>(edge, v);
+        return lane == 31 ? edge : t;
+    }
+    static __device__ __forceinline__ uint32_t min(uint32_t v) { return pair_rows_min(row_min_u32(v)); }
+};
+
+template <>
+struct LineLanes<64> {
+    static __device__ __forceinline__ uint32_t from_left(uint32_t edge, uint32_t v, int) {
+        return dpp_or<DPP_WAVE_SHR1>(edge, v);
+    }
+    static __device__ __forceinline__ uint32_t from_right(uint32_t edge, uint32_t v, int) {
+        return dpp_or<DPP_WAVE_SHL1>(edge, v);
+    }
+    static __device__ __forceinline__ uint32_t min(uint32_t v) {
+        return halves_min(pair_rows_min(row_min_u32(v)));
+    }
+};
+
+template <int K, int LANES>
 struct PathState {
     static constexpr int R = K / 2;
     uint32_t v[R];  // disparities (2k, 2k+1) of this lane's K in the low and high halves
-    uint32_t pm;    // the minimum over the line's D values, same in all 16 lanes
+    uint32_t pm;    // the minimum over the line's D values, same in all its lanes
 
     __device__ void reset() {
 #pragma unroll
@@ -248,11 +301,12 @@ struct PathState {
     // min(prev[d-1], prev[d+1]) + p1) - pm + C. Out-of-range neighbours are a
     // sentinel that only ever loses to the pm + p2 term, as libSGM's p2 - p1
     // padding does.
-    __device__ __forceinline__ void step(const Bytes<K> &cost, uint32_t p1x2, uint32_t p2) {
-        // Lanes 0 and 15 have no neighbour across the line's ends: they take the sentinel.
+    __device__ __forceinline__ void step(const Bytes<K> &cost, uint32_t p1x2, uint32_t p2,
+                                         int lane) {
+        // The line's first and last lanes have no neighbour past its ends: the sentinel.
         constexpr uint32_t SENT = 0x3FFF3FFFu;
-        const uint32_t left_in = dpp_or<DPP_ROW_SHR1>(SENT, v[R - 1]);
-        const uint32_t right_in = dpp_or<DPP_ROW_SHL1>(SENT, v[0]);
+        const uint32_t left_in = LineLanes<LANES>::from_left(SENT, v[R - 1], lane);
+        const uint32_t right_in = LineLanes<LANES>::from_right(SENT, v[0], lane);
         uint32_t s[R + 1];
         s[0] = (v[0] << 16) | (left_in >> 16);
 #pragma unroll
@@ -261,10 +315,14 @@ struct PathState {
         const uint32_t PM = pm * 0x10001u;
         const uint32_t JUMP = (pm + p2) * 0x10001u;
         uint32_t c[R];
+        if constexpr (K == 2) {
+            c[0] = lo_pair(cost.w[0]);
+        } else {
 #pragma unroll
-        for (int k = 0; k < R / 2; k++) {
-            c[2 * k] = lo_pair(cost.w[k]);
-            c[2 * k + 1] = hi_pair(cost.w[k]);
+            for (int k = 0; k < R / 2; k++) {
+                c[2 * k] = lo_pair(cost.w[k]);
+                c[2 * k + 1] = hi_pair(cost.w[k]);
+            }
         }
         uint32_t mn = 0xFFFFFFFFu;
 #pragma unroll
@@ -278,13 +336,17 @@ struct PathState {
             mn = pk_min(mn, t);
         }
         uint32_t m = min(mn & 0xFFFFu, mn >> 16);
-        pm = row_min_u32(m);
+        pm = LineLanes<LANES>::min(m);
     }
 
     __device__ __forceinline__ Bytes<K> bytes() const {
         Bytes<K> b;
+        if constexpr (K == 2) {
+            b.w[0] = __builtin_amdgcn_perm(0u, v[0], 0x0C0C0200u);
+        } else {
 #pragma unroll
-        for (int k = 0; k < R / 2; k++) b.w[k] = pack_pairs(v[2 * k], v[2 * k + 1]);
+            for (int k = 0; k < R / 2; k++) b.w[k] = pack_pairs(v[2 * k], v[2 * k + 1]);
+        }
         return b;
     }
 };
@@ -327,17 +389,18 @@ struct PathArgs {
     int block_start[4];
 };
 
-// Lines per block: one wave, four 16-lane DPP rows, one line each.
+// One wave per block, 64 / LANES lines per wave.
 constexpr int PATH_BLOCK = 64;
-constexpr int LINES_PER_BLOCK = 4;
 
 // Grid y: the pair. PREFETCH: steps of costs (and, past the middle, of the other
-// direction's path costs) loaded ahead of use.
-template <int K, int PREFETCH>
+// direction's path costs) loaded ahead of use. LANES: lanes per line, K = D / LANES
+// disparities per lane.
+template <int K, int PREFETCH, int LANES>
 __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
-    constexpr int D = 16 * K;
-    const int lane16 = threadIdx.x & 15;
-    const int row = threadIdx.x >> 4;
+    constexpr int D = LANES * K;
+    constexpr int LINES_PER_BLOCK = PATH_BLOCK / LANES;
+    const int lane = threadIdx.x % LANES;  // within the line
+    const int row = threadIdx.x / LANES;   // the block's line
     const int b = blockIdx.x;
     int o = a.last;
     for (int k = a.first; k < a.last; k++)
@@ -363,7 +426,7 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
     // computes on whatever it loads and stores nothing.
     const uint32_t stride = (uint32_t)((dy * W + dx) * D);
     const int L = g.length;
-    const uint32_t first = ((uint32_t)g.y0 * W + (uint32_t)g.x0) * D + lane16 * K;
+    const uint32_t first = ((uint32_t)g.y0 * W + (uint32_t)g.x0) * D + lane * K;
     const uint32_t last = first + (uint32_t)max(L - 1, 0) * stride;
     const uint32_t volume = (uint32_t)W * H * D;
     const Rsrc costs = buffer_of(pd.b.costs, volume);
@@ -377,14 +440,14 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
 
     if (o == 1 && !a.backward_columns) {
         // Columns with 3 paths: down only, its own path costs.
-        PathState<K> s;
+        PathState<K, LANES> s;
         s.reset();
         Bytes<K> ring[PREFETCH];
         // The prologue issues what a step issues (a masked store, then the load), so the
         // loop is entered with the same accesses outstanding as it goes round with.
         Bytes<K> none;
 #pragma unroll
-        for (int k = 0; k < K / 4; k++) none.w[k] = 0;
+        for (int k = 0; k < (K + 3) / 4; k++) none.w[k] = 0;
 #pragma unroll
         for (int p = 0; p < PREFETCH; p++) {
             __builtin_amdgcn_sched_barrier(0);
@@ -398,7 +461,7 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
                 const int t = t0 + p;
                 // Keep the steps in program order, so each waits only for its own loads.
                 __builtin_amdgcn_sched_barrier(0);
-                s.step(ring[p], p1x2, p2);
+                s.step(ring[p], p1x2, p2, lane);
                 vstore<K>(paired, t < L ? at : MASKED, s.bytes());
                 ring[p] = vload<K>(costs, ahead);
                 at += stride;
@@ -419,7 +482,7 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
     // the other direction stored them by then: 2j - (L-1) >= PREFETCH. Just past the middle
     // (0 < 2j - (L-1) < PREFETCH) they are younger than that, and come instead from the
     // history ring of each direction's last PREFETCH own bytes.
-    PathState<K> fwd, bwd;
+    PathState<K, LANES> fwd, bwd;
     fwd.reset();
     bwd.reset();
     Bytes<K> ring_f[PREFETCH], ring_b[PREFETCH];    // costs, PREFETCH steps ahead
@@ -430,7 +493,7 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
         // As a step issues: two (masked) stores, then the four loads; the masked loads of
         // the other's bytes read zeros.
 #pragma unroll
-        for (int k = 0; k < K / 4; k++) hist_f[p].w[k] = hist_b[p].w[k] = 0;
+        for (int k = 0; k < (K + 3) / 4; k++) hist_f[p].w[k] = hist_b[p].w[k] = 0;
         __builtin_amdgcn_sched_barrier(0);
         vstore<K>(paired, MASKED, hist_f[p]);
         vstore<K>(paired, MASKED, hist_b[p]);
@@ -447,8 +510,8 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
             const int i = i0 + p;
             // Keep the steps in program order, so each waits only for its own loads.
             __builtin_amdgcn_sched_barrier(0);
-            fwd.step(ring_f[p], p1x2, p2);
-            bwd.step(ring_b[p], p1x2, p2);
+            fwd.step(ring_f[p], p1x2, p2, lane);
+            bwd.step(ring_b[p], p1x2, p2, lane);
             const Bytes<K> mf = fwd.bytes(), mb = bwd.bytes();
             const int past = 2 * i - (L - 1);  // < 0 before the middle, 0 at an odd line's middle
             Bytes<K> of = other_f[p], ob = other_b[p];
@@ -459,7 +522,7 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
                 for (int q = 0; q < PREFETCH; q++) {
                     if (q == p) continue;  // step i itself: never the other's
 #pragma unroll
-                    for (int k = 0; k < K / 4; k++) {
+                    for (int k = 0; k < (K + 3) / 4; k++) {
                         of.w[k] = s == q ? hist_b[q].w[k] : of.w[k];
                         ob.w[k] = s == q ? hist_f[q].w[k] : ob.w[k];
                     }
@@ -469,7 +532,7 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
             hist_b[p] = mb;
             Bytes<K> sf, sb;
 #pragma unroll
-            for (int k = 0; k < K / 4; k++) {
+            for (int k = 0; k < (K + 3) / 4; k++) {
                 sf.w[k] = mf.w[k] + (past < 0 ? 0u : (past == 0 ? mb.w[k] : of.w[k]));
                 sb.w[k] = mb.w[k] + (past <= 0 ? 0u : ob.w[k]);
             }
@@ -712,10 +775,39 @@ void match_carve(MatchBuffers &b, void *base, int W, int H, int D, int paths) {
 // Launchers. One launch per kernel covers `count` pairs of one size class: grid y (or z)
 // indexes the pair, whose descriptor comes by value (count 1) or from `table`.
 
-template <int K, int PREFETCH>
+template <int K, int PREFETCH, int LANES>
 static void launch_paths(const PathArgs &a, int blocks, int count, hipStream_t stream) {
-    hipLaunchKernelGGL((path_kernel<K, PREFETCH>), dim3(blocks, count), dim3(PATH_BLOCK), 0, stream,
-                       a);
+    hipLaunchKernelGGL((path_kernel<K, PREFETCH, LANES>), dim3(blocks, count), dim3(PATH_BLOCK), 0,
+                       stream, a);
+}
+
+template <int K, int LANES>
+static void launch_paths_prefetch(const PathArgs &a, int blocks, int count, int prefetch,
+                                  hipStream_t stream) {
+    switch (prefetch) {
+        case 2: launch_paths<K, 2, LANES>(a, blocks, count, stream); break;
+        case 8: launch_paths<K, 8, LANES>(a, blocks, count, stream); break;
+        default: launch_paths<K, 4, LANES>(a, blocks, count, stream); break;
+    }
+}
+
+bool lanes_fit(int disparities, int lanes) {
+    if (lanes != 16 && lanes != 32 && lanes != 64) return false;
+    const int k = disparities / lanes;
+    return disparities % lanes == 0 && (k == 2 || k == 4 || k == 8 || k == 12 || k == 16);
+}
+
+// The path launch of D disparities with `lanes` lanes per line (16 where that does not fit).
+template <int D>
+static void launch_paths_lanes(const PathArgs &a, int blocks, int count, const Schedule &sch,
+                               hipStream_t stream) {
+    if constexpr (D == 128 || D == 256) {
+        if (sch.lanes == 64) return launch_paths_prefetch<D / 64, 64>(a, blocks, count, sch.prefetch, stream);
+    }
+    if constexpr (D == 64 || D == 128 || D == 256) {
+        if (sch.lanes == 32) return launch_paths_prefetch<D / 32, 32>(a, blocks, count, sch.prefetch, stream);
+    }
+    launch_paths_prefetch<D / 16, 16>(a, blocks, count, sch.prefetch, stream);
 }
 
 template <int K>
@@ -731,19 +823,16 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
     a.W = W;
     a.H = H;
     a.backward_columns = p.paths == 8;
+    const int per_block = PATH_BLOCK / (lanes_fit(16 * K, sch.lanes) ? sch.lanes : 16);
     auto launch = [&](int first, int last) {
         a.first = first;
         a.last = last;
         int blocks = 0;
         for (int o = 0; o < 4; o++) {
             a.block_start[o] = blocks;
-            if (o >= first && o <= last) blocks += (lines[o] + LINES_PER_BLOCK - 1) / LINES_PER_BLOCK;
+            if (o >= first && o <= last) blocks += (lines[o] + per_block - 1) / per_block;
         }
-        switch (sch.prefetch) {
-            case 2: launch_paths<K, 2>(a, blocks, count, stream); break;
-            case 8: launch_paths<K, 8>(a, blocks, count, stream); break;
-            default: launch_paths<K, 4>(a, blocks, count, stream); break;
-        }
+        launch_paths_lanes<16 * K>(a, blocks, count, sch, stream);
     };
     int slot = profile_begin(PK_PATH, stream);
     if (sch.split) {

@@ -48,23 +48,40 @@ def test_batch_matches_the_oracle_on_every_replay():
     batch.close()
 
 
-@pytest.mark.parametrize("disparities,paths", [(64, 8), (128, 8), (128, 3), (256, 8)])
+LANES = {64: (16, 32), 128: (16, 32, 64), 192: (16,), 256: (16, 32, 64)}
+
+
+@pytest.mark.parametrize("disparities,paths", [(64, 8), (128, 8), (128, 3), (192, 8), (256, 8)])
 def test_every_schedule_gives_the_same_disparity(disparities, paths):
-    """Tuning picks by time only: each candidate schedule is checked bit for bit here."""
+    """Tuning picks by time only: each candidate schedule is checked bit for bit here, on
+    the path sums themselves as well as the output."""
     w, h = 97, 31
     left, right = scenes.scene(w, h, 7)
     want = oracle.sgm(left, right, disparities, paths)
+    want_sums = oracle.aggregate(oracle.costs(oracle.census(left), oracle.census(right), disparities),
+                                 paths, 8, 64)
     l, r, o = hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h)
     ws = hip.Buffer(_native.workspace_size(w, h, disparities, paths))
     kept = _native.get_schedule(w, h, disparities, paths)
     try:
-        for prefetch in (2, 4, 8):
-            for split in (False, True):
-                _native.set_schedule(w, h, disparities, paths, prefetch, split)
-                _native.match_device(l.ptr, r.ptr, w, h, disparities, paths, 8, 64, 1.0526316,
-                                     o.ptr, ws.ptr, ws.nbytes)
-                hip.synchronize()
-                got = o.download((h, w), np.float32)
-                assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (prefetch, split)
+        for lanes in LANES[disparities]:
+            for prefetch in (2, 4, 8):
+                for split in (False, True):
+                    _native.set_schedule(w, h, disparities, paths, prefetch, split, lanes)
+                    _native.match_device(l.ptr, r.ptr, w, h, disparities, paths, 8, 64, 1.0526316,
+                                         o.ptr, ws.ptr, ws.nbytes)
+                    hip.synchronize()
+                    got = o.download((h, w), np.float32)
+                    case = (lanes, prefetch, split)
+                    assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), case
+                    sums = _native.match_host(left, right, disparities, paths, intermediates=True)["sums"]
+                    assert np.array_equal(sums, want_sums), case
     finally:
         _native.set_schedule(w, h, disparities, paths, *kept)
+
+
+def test_lanes_that_do_not_fit_are_refused():
+    with pytest.raises(_native.NativeError, match="do not fit"):
+        _native.set_schedule(97, 31, 192, 8, 4, False, 32)
+    with pytest.raises(_native.NativeError, match="do not fit"):
+        _native.set_schedule(97, 31, 64, 8, 4, False, 64)

@@ -1,0 +1,98 @@
+"""The served form: streams of uneven sizes through one context at once.
+
+Every stream's replies are the same, bit for bit, when its pairs are served in the
+mix (all streams' pairs submitted together, overlapping on the device, admitted by
+run slots and the HBM budget) as when the stream is served alone one pair at a time,
+and both are the oracle's Disparity -> TemporalFilter -> Quality chain.
+"""
+import numpy as np
+import pytest
+
+from nx_stereo_amd import _native, protocol, segment, serving
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+# Two streams share a size class; the others differ in size and disparity range.
+SPECS = [serving.StreamSpec(97, 31, 64, 60.0, 11), serving.StreamSpec(160, 48, 128, 30.0, 12),
+         serving.StreamSpec(120, 40, 192, 15.0, 13), serving.StreamSpec(97, 31, 64, 60.0, 14),
+         serving.StreamSpec(64, 48, 256, 30.0, 15)]
+FRAMES = 5
+
+
+def oracle_stream(spec, frames):
+    temporal = oracle.Temporal((spec.height, spec.width))
+    quality = oracle.Quality((spec.height, spec.width))
+    out = []
+    for left, right in frames:
+        d = oracle.sgm(left, right, spec.disparities, 8)
+        d = temporal.filter(d, 0.4, 2.0, 2)
+        out.append((d, quality.sums(d)))
+    return out
+
+
+def decoded(reply):
+    r = protocol.decode(reply)
+    assert isinstance(r, protocol.Outputs), r
+    return r.tensors["disparity"], r.tensors["quality_sums"]
+
+
+def same(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def frames():
+    return [serving.stream_frames(spec, FRAMES) for spec in SPECS]
+
+
+def alone(spec, fs):
+    """The stream by itself in a fresh context, one pair at a time."""
+    with segment.Context(0) as ctx:
+        (seg,) = serving.configure_streams(ctx, [spec])
+        out = []
+        for left, right in fs:
+            ticket = ctx.submit(seg.run_record({"left": left, "right": right}))
+            assert ctx.wait(ticket, 60000)
+            out.append(decoded(ctx.reply(ticket)))
+        return out
+
+
+@pytest.mark.parametrize("budget", [None, 1])
+def test_each_stream_is_the_same_alone_and_in_the_mix(frames, budget):
+    """budget=1 byte: every matcher run waits for the one before to free the device memory."""
+    with segment.Context(0) as ctx:
+        if budget is not None:
+            assert ctx.lib.nxs_set_hbm_budget(ctx.handle, budget) == _native.NXS_OK
+        segs = serving.configure_streams(ctx, SPECS)
+        records = serving.run_records(segs, frames)
+        arrivals = [np.zeros(FRAMES) for _ in SPECS]  # every pair at once: the most overlap
+        served = serving.serve(ctx, segs, records, arrivals, keep_replies=True, all_at_once=True)
+    mixed = [[None] * FRAMES for _ in SPECS]
+    for s, k, reply in zip(served.stream, served.frame, served.replies):
+        mixed[s][k] = decoded(reply)
+    for s, spec in enumerate(SPECS):
+        want = oracle_stream(spec, frames[s])
+        solo = alone(spec, frames[s])
+        for k in range(FRAMES):
+            assert same(mixed[s][k][0], solo[k][0]) and same(mixed[s][k][1], solo[k][1]), (s, k)
+            assert same(mixed[s][k][0], want[k][0]), ("disparity", s, k)
+            assert same(mixed[s][k][1], want[k][1]), ("quality", s, k)
+
+
+def test_poisson_arrivals_complete_in_order_per_stream(frames):
+    """Arrivals at the streams' rates: every pair completes, and a stream's pairs complete in
+    the order they arrived (its segment computes them in order)."""
+    with segment.Context(0) as ctx:
+        segs = serving.configure_streams(ctx, SPECS)
+        records = serving.run_records(segs, frames)
+        arrivals = [serving.poisson_arrivals(spec.rate_hz * 20, 10.0, spec.seed)[:FRAMES]
+                    for spec in SPECS]
+        served = serving.serve(ctx, segs, records, arrivals, keep_replies=True)
+    for s in range(len(SPECS)):
+        mine = served.stream == s
+        order = np.argsort(served.arrival[mine], kind="stable")
+        assert (np.diff(served.done[mine][order]) >= 0).all(), s
+    assert (served.latency_ms > 0).all()
+    summary = served.summary()
+    assert summary["pairs"] == FRAMES * len(SPECS)
