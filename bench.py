@@ -143,7 +143,7 @@ def large_pair_leg(runs: int = 10) -> dict:
     cost = (n + 16 * pixels) / (us["cost"] * 1e-6) / 1e9
     for b in (l, r, o, ws):
         b.free()
-    return {
+    out = {
         "workload": "one rectified planar pair 1920x1080, 256 disparities (4..248 px), 8 paths, "
                     "Stage.Disparity's matcher alone",
         "ms_per_pair": round(ms, 4), "pairs_per_s": round(1e3 / ms, 2),
@@ -156,6 +156,12 @@ def large_pair_leg(runs: int = 10) -> dict:
                      "algorithmic_bytes": n + 16 * pixels},
             "peak": HBM_PEAK_GBS, "unit": "GB/s"},
     }
+    traffic = load_traffic("1080")
+    if traffic:
+        t = traffic["aggregation_traffic_bytes"]
+        out["roofline"]["aggregation"]["traffic"] = t
+        out["roofline"]["aggregation"]["traffic_gbs"] = round(t / (aggregation_us * 1e-6) / 1e9, 1)
+    return out
 
 
 BATCH_CLASSES = [(480, 270, 64, 24), (728, 544, 128, 48), (1280, 720, 192, 24)]
@@ -259,6 +265,96 @@ def served_leg(duration_s: float = 1.5) -> dict:
     return result
 
 
+# ---------------------------------------------------------------------------------------
+# HBM traffic from rocprofv3 counters. `--kernels-only SIZE` runs three matchings of the
+# given workload and a 256 MiB device-to-device copy (the calibration) and nothing else, for
+#   rocprofv3 --pmc FETCH_SIZE ... -- python bench.py --kernels-only 728
+#   rocprofv3 --pmc WRITE_SIZE ... -- python bench.py --kernels-only 728
+# and `--traffic FETCH_CSV WRITE_CSV OUT` turns the two counter_collection.csv files into
+# OUT (profiles/traffic_*.json), which the default run reads into roofline.traffic.
+
+TRAFFIC_FILES = {"728": os.path.join(ROOT, "profiles", "traffic_728x544x128.json"),
+                 "1080": os.path.join(ROOT, "profiles", "traffic_1920x1080x256.json")}
+CALIBRATION_BYTES = 256 << 20
+
+
+def kernels_only(size: str) -> int:
+    from nx_stereo_amd import _native, hip
+
+    # The kernels' accesses do not depend on the images' content: a planar pair of the size.
+    if size == "728":
+        w, h, d = WIDTH, HEIGHT, DISPARITIES
+        left, right = match_pair(w, h, d, SEED)
+    else:
+        w, h, d = 1920, 1080, 256
+        left, right = match_pair(w, h, d, SEED + 3)
+    l, r, o = hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h)
+    ws = hip.Buffer(_native.workspace_size(w, h, d, PATHS))
+    for _ in range(3):
+        _native.match_device(l.ptr, r.ptr, w, h, d, PATHS, 8, 64, 1.0526315789473684, o.ptr,
+                             ws.ptr, ws.nbytes)
+    a, b = hip.Buffer(CALIBRATION_BYTES), hip.Buffer(CALIBRATION_BYTES)
+    hip.check(hip.runtime().hipMemcpy(ctypes_ptr(b.ptr), ctypes_ptr(a.ptr), CALIBRATION_BYTES, 3),
+              "hipMemcpy")
+    hip.synchronize()
+    print(f"kernels-only {w}x{h}x{d} done", flush=True)
+    return 0
+
+
+def ctypes_ptr(address):
+    import ctypes
+
+    return ctypes.c_void_p(address)
+
+
+def traffic_from(fetch_csv: str, write_csv: str, out: str) -> int:
+    """Average FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of each matcher kernel, scaled
+    by what the same counter reads for the 256 MiB copy (which reads and writes 256 MiB)."""
+    import csv
+
+    def per_kernel(path, counter):
+        sums, counts = {}, {}
+        for row in csv.DictReader(open(path)):
+            if row["Counter_Name"] != counter:
+                continue
+            name = row["Kernel_Name"].split("(")[0].split("::")[-1].split("<")[0]
+            sums[name] = sums.get(name, 0.0) + float(row["Counter_Value"])
+            counts[name] = counts.get(name, 0) + 1
+        return {k: sums[k] / counts[k] for k in sums}
+
+    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    copy = [k for k in fetch if "copyBuffer" in k]
+    if not copy:
+        raise SystemExit("no calibration copy in the fetch counters")
+    kib = CALIBRATION_BYTES / 1024
+    fetch_factor, write_factor = kib / fetch[copy[0]], kib / write[copy[0]]
+    kernels = {}
+    for k in fetch:
+        if "copyBuffer" in k:
+            continue
+        kernels[k] = {"fetch_bytes": round(fetch[k] * 1024 * fetch_factor),
+                      "write_bytes": round(write.get(k, 0.0) * 1024 * write_factor)}
+    aggregation = sum(kernels[k]["fetch_bytes"] + kernels[k]["write_bytes"]
+                      for k in ("path_kernel", "select_kernel") if k in kernels)
+    result = {"fetch_factor": round(fetch_factor, 4), "write_factor": round(write_factor, 4),
+              "calibration": "hipMemcpy device-to-device of 256 MiB (reads and writes 256 MiB)",
+              "kernels": kernels, "aggregation_traffic_bytes": aggregation,
+              "sources": [fetch_csv, write_csv]}
+    with open(out, "w") as f:
+        json.dump(result, f, indent=1)
+        f.write("\n")
+    print(json.dumps(result))
+    return 0
+
+
+def load_traffic(size: str):
+    path = TRAFFIC_FILES[size]
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
@@ -267,7 +363,15 @@ def main() -> int:
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     parser.add_argument("--legs", default="large,batched,served",
                         help="comma-separated extra legs: large, batched, served (or none)")
+    parser.add_argument("--kernels-only", choices=sorted(TRAFFIC_FILES),
+                        help="run three matchings and the calibration copy, for rocprofv3 --pmc")
+    parser.add_argument("--traffic", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT"),
+                        help="HBM bytes per launch from two rocprofv3 --pmc runs into OUT")
     args = parser.parse_args()
+    if args.kernels_only:
+        return kernels_only(args.kernels_only)
+    if args.traffic:
+        return traffic_from(*args.traffic)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one, and pairs shard across GPUs "
               f"with no exchange (--gpus {args.gpus} is not run)", file=sys.stderr)
@@ -357,9 +461,18 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
             "kernel": "aggregation: path_kernel + select_kernel",
+            "algorithmic_bytes": algorithmic,
             "kernel_us": {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in kernels.items()},
         },
     }
+    traffic = load_traffic("728")
+    if traffic:
+        result["roofline"]["traffic"] = traffic["aggregation_traffic_bytes"]
+        result["roofline"]["traffic_note"] = (
+            "rocprofv3 FETCH_SIZE x %.2f + WRITE_SIZE x %.2f of path_kernel + select_kernel, "
+            "per pair (profiles/traffic_728x544x128.json); the working set fits the 256 MB "
+            "Infinity Cache, so part of it never reaches HBM" % (traffic["fetch_factor"],
+                                                                  traffic["write_factor"]))
     context.close()
     for frame in frames:
         frame.free()

@@ -117,7 +117,8 @@ int nxs_match_host(const uint8_t *left, const uint8_t *right, int width, int hei
  * over a descriptor table in device memory; the whole batch is captured
  * once into a hipGraph that nxs_batch_run replays. With `autotune`, each
  * class's aggregation schedule (prefetch depth 2/4/8, lanes per line 16/32/64,
- * orientations in one launch or one each) is timed on the batch itself and the fastest kept,
+ * one or two waves per line, orientations in one launch or one each) is timed on the
+ * batch itself and the fastest kept,
  * also for later single-pair launches of that size. The batch owns one
  * matcher workspace per pair; the pairs' images and outputs stay the
  * caller's, read and written at every replay. No counterpart in the
@@ -132,14 +133,18 @@ typedef struct nxs_pair {
 typedef struct nxs_batch nxs_batch;
 
 /* The aggregation schedule matcher launches of one size use: the autotuner's
- * pick, or what nxs_schedule_set says (prefetch 2, 4 or 8 steps; split 1 for
- * one launch per line orientation; lanes 16, 32 or 64 sharing one line, where
- * disparities / lanes is 2, 4, 8, 12 or 16). Results are the same under every
+ * pick, or what nxs_schedule_set says. Results are the same under every
  * schedule; only the time differs. */
-int nxs_schedule_set(int width, int height, int disparities, int paths, int prefetch, int split,
-                     int lanes);
-int nxs_schedule_get(int width, int height, int disparities, int paths, int *prefetch, int *split,
-                     int *lanes);
+typedef struct nxs_schedule {
+    int prefetch; /* steps of costs loaded ahead: 2, 4 or 8 */
+    int split;    /* 1: one launch per line orientation */
+    int lanes;    /* lanes sharing one line: 16, 32 or 64, where disparities / lanes is
+                     2, 4, 8, 12 or 16 */
+    int waves;    /* 2: a line's two directions in two waves (8 paths; 16 lanes) */
+} nxs_schedule;
+
+int nxs_schedule_set(int width, int height, int disparities, int paths, const nxs_schedule *s);
+int nxs_schedule_get(int width, int height, int disparities, int paths, nxs_schedule *s);
 
 int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch **out);
 int nxs_batch_run(nxs_batch *batch, void *stream);

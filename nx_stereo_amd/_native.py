@@ -47,6 +47,13 @@ class NxsPair(ctypes.Structure):
                 ("p2", ctypes.c_int), ("uniqueness", ctypes.c_float)]
 
 
+class NxsSchedule(ctypes.Structure):
+    """struct nxs_schedule: how matcher launches of one size aggregate."""
+
+    _fields_ = [("prefetch", ctypes.c_int), ("split", ctypes.c_int), ("lanes", ctypes.c_int),
+                ("waves", ctypes.c_int)]
+
+
 class NativeError(RuntimeError):
     """A failing call into libnxs_hip, with the library's reason."""
 
@@ -90,10 +97,9 @@ def load() -> ctypes.CDLL:
     lib.nxs_profile_start.restype = i
     lib.nxs_profile_stop.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i), i]
     lib.nxs_profile_stop.restype = i
-    lib.nxs_schedule_set.argtypes = [i, i, i, i, i, i, i]
+    lib.nxs_schedule_set.argtypes = [i, i, i, i, ctypes.POINTER(NxsSchedule)]
     lib.nxs_schedule_set.restype = i
-    lib.nxs_schedule_get.argtypes = [i, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i),
-                                     ctypes.POINTER(i)]
+    lib.nxs_schedule_get.argtypes = [i, i, i, i, ctypes.POINTER(NxsSchedule)]
     lib.nxs_schedule_get.restype = i
     lib.nxs_batch_create.argtypes = [ctypes.POINTER(NxsPair), i, i, ctypes.POINTER(P)]
     lib.nxs_batch_create.restype = i
@@ -162,16 +168,17 @@ def match_device(left_ptr: int, right_ptr: int, width: int, height: int, dispari
     _check(rc, "nxs_match_device")
 
 
-def set_schedule(width, height, disparities, paths, prefetch, split, lanes=16) -> None:
-    _check(load().nxs_schedule_set(width, height, disparities, paths, prefetch, int(split), lanes),
+def set_schedule(width, height, disparities, paths, prefetch, split, lanes=16, waves=1) -> None:
+    s = NxsSchedule(prefetch, int(split), lanes, waves)
+    _check(load().nxs_schedule_set(width, height, disparities, paths, ctypes.byref(s)),
            "nxs_schedule_set")
 
 
-def get_schedule(width, height, disparities, paths) -> tuple[int, bool, int]:
-    prefetch, split, lanes = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    load().nxs_schedule_get(width, height, disparities, paths, ctypes.byref(prefetch),
-                            ctypes.byref(split), ctypes.byref(lanes))
-    return prefetch.value, bool(split.value), lanes.value
+def get_schedule(width, height, disparities, paths) -> tuple[int, bool, int, int]:
+    s = NxsSchedule()
+    _check(load().nxs_schedule_get(width, height, disparities, paths, ctypes.byref(s)),
+           "nxs_schedule_get")
+    return s.prefetch, bool(s.split), s.lanes, s.waves
 
 
 class Batch:

@@ -43,12 +43,14 @@ struct PairDesc {
 };
 
 // How the aggregation is scheduled: the steps of costs the path kernel loads ahead,
-// whether the four line orientations go in one launch or in one launch each, and how many
-// lanes share a line (16, 32 or 64; D / lanes disparities each, an even count up to 16).
+// whether the four line orientations go in one launch or in one launch each, how many
+// lanes share a line (16, 32 or 64; D / lanes disparities each, an even count up to 16),
+// and whether a line's two directions share one wave or take one wave each.
 struct Schedule {
     int prefetch = 4;  // 2, 4 or 8
     bool split = false;
     int lanes = 16;
+    int waves = 1;  // 2: a line's two directions in two waves of one block (8 paths, 16 lanes)
 };
 
 // Whether `lanes` lanes per line fit D disparities (D / lanes is 2, 4, 8, 12 or 16).

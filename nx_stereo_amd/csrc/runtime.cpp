@@ -66,10 +66,25 @@ void profile_end(int slot, hipStream_t stream) {
 static std::mutex g_schedules_mu;
 static std::map<std::tuple<int, int, int, int>, Schedule> g_schedules;
 
+// Without a pick: below 256 disparities a line's sequential walk bounds the aggregation, and
+// two waves per line (one per direction) halve it; at 256 the aggregation is bound by HBM
+// traffic instead, and one wave per line with the deepest prefetch measured fastest
+// (1920x1080 pairs on MI355X).
+static Schedule default_schedule(int disparities, int paths) {
+    Schedule s;
+    if (paths == 8 && disparities < 256) {
+        s.prefetch = 2;
+        s.waves = 2;
+    } else {
+        s.prefetch = 8;
+    }
+    return s;
+}
+
 Schedule schedule_for(int width, int height, int disparities, int paths) {
     std::lock_guard<std::mutex> lock(g_schedules_mu);
     auto it = g_schedules.find(std::make_tuple(width, height, disparities, paths));
-    return it == g_schedules.end() ? Schedule{} : it->second;
+    return it == g_schedules.end() ? default_schedule(disparities, paths) : it->second;
 }
 
 void remember_schedule(int width, int height, int disparities, int paths, const Schedule &s) {
@@ -1423,8 +1438,10 @@ struct nxs_batch {
         std::vector<Schedule> candidates;
         for (int lanes : {16, 32, 64})
             if (lanes_fit(c.p.disparities, lanes))
-                for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, lanes});
-        candidates.push_back(Schedule{4, true, 16});
+                for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, lanes, 1});
+        if (c.p.paths == 8)
+            for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, 16, 2});
+        candidates.push_back(Schedule{4, true, 16, 1});
         for (const Schedule &s : candidates) {
             HIP_TRY(launch_match_batch(c.p, tables + c.first, c.count, s, stream));
             HIP_TRY(hipEventRecord(start, stream));
@@ -1435,7 +1452,8 @@ struct nxs_batch {
             float ms = 0;
             HIP_TRY(hipEventElapsedTime(&ms, start, stop));
             trials += "  prefetch " + std::to_string(s.prefetch) + ", " +
-                      std::to_string(s.lanes) + " lanes" + (s.split ? ", split" : "") + ": " +
+                      std::to_string(s.lanes) + " lanes, " + std::to_string(s.waves) +
+                      (s.waves == 2 ? " waves" : " wave") + (s.split ? ", split" : "") + ": " +
                       std::to_string(ms / 3) + " ms\n";
             if (ms < best) {
                 best = ms;
@@ -1539,7 +1557,8 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
         for (auto &c : batch->classes)
             text << c.count << " x " << c.p.width << "x" << c.p.height << "x" << c.p.disparities
                  << " paths " << c.p.paths << ": prefetch " << c.schedule.prefetch << ", "
-                 << c.schedule.lanes << " lanes per line, "
+                 << c.schedule.lanes << " lanes per line, " << c.schedule.waves
+                 << (c.schedule.waves == 2 ? " waves" : " wave") << " per line, "
                  << (c.schedule.split ? "one launch per orientation" : "orientations in one launch")
                  << (autotune ? " (tuned, " + std::to_string(c.tuned_ms) + " ms)" : "") << "\n";
         batch->description = text.str() + batch->trials;
@@ -1550,24 +1569,25 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
     return NXS_OK;
 }
 
-int nxs_schedule_set(int width, int height, int disparities, int paths, int prefetch, int split,
-                     int lanes) {
-    if (prefetch != 2 && prefetch != 4 && prefetch != 8)
+int nxs_schedule_set(int width, int height, int disparities, int paths, const nxs_schedule *s) {
+    if (!s) return nxs::fail(NXS_EINVAL, "no schedule");
+    if (s->prefetch != 2 && s->prefetch != 4 && s->prefetch != 8)
         return nxs::fail(NXS_EINVAL, "prefetch must be 2, 4 or 8");
-    if (!nxs::lanes_fit(disparities, lanes))
-        return nxs::fail(NXS_EINVAL, std::to_string(lanes) + " lanes per line do not fit " +
+    if (!nxs::lanes_fit(disparities, s->lanes))
+        return nxs::fail(NXS_EINVAL, std::to_string(s->lanes) + " lanes per line do not fit " +
                                          std::to_string(disparities) + " disparities");
+    if (s->waves != 1 && s->waves != 2) return nxs::fail(NXS_EINVAL, "waves must be 1 or 2");
+    if (s->waves == 2 && s->lanes != 16)
+        return nxs::fail(NXS_EINVAL, "two waves per line take 16 lanes per line");
     nxs::remember_schedule(width, height, disparities, paths,
-                           nxs::Schedule{prefetch, split != 0, lanes});
+                           nxs::Schedule{s->prefetch, s->split != 0, s->lanes, s->waves});
     return NXS_OK;
 }
 
-int nxs_schedule_get(int width, int height, int disparities, int paths, int *prefetch, int *split,
-                     int *lanes) {
-    const nxs::Schedule s = nxs::schedule_for(width, height, disparities, paths);
-    if (prefetch) *prefetch = s.prefetch;
-    if (split) *split = s.split ? 1 : 0;
-    if (lanes) *lanes = s.lanes;
+int nxs_schedule_get(int width, int height, int disparities, int paths, nxs_schedule *s) {
+    if (!s) return nxs::fail(NXS_EINVAL, "no schedule");
+    const nxs::Schedule k = nxs::schedule_for(width, height, disparities, paths);
+    *s = nxs_schedule{k.prefetch, k.split ? 1 : 0, k.lanes, k.waves};
     return NXS_OK;
 }
 

@@ -554,6 +554,120 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------
+// The same aggregation with a line's two directions in two waves of one block: wave 0 walks
+// forward, wave 1 backward, over the same four lines (16 lanes each). Each wave issues half
+// the instructions per step of the one-wave form, so a line's sequential walk, which bounds
+// the kernel, takes about half the time; the two waves meet through memory and LDS:
+//
+// * At every step each wave writes its own bytes into an LDS ring of its last TWO_RING
+//   steps, waits until its global stores from LAG steps ago have completed and its LDS
+//   writes are done, and meets the other wave at s_barrier. After the barrier of step i,
+//   the other's stores of steps <= i - LAG are visible, and its bytes of the last
+//   TWO_RING steps are in LDS.
+// * Past the middle, a wave needs the other's bytes of its own pixel, which the other made
+//   at step i - past (past = 2i - (L-1)): from memory, loaded PREFETCH steps ahead, when
+//   past >= PREFETCH + LAG; otherwise from the LDS ring (past < TWO_RING - 1, so the
+//   slot is not rewritten before it is read).
+// * At an odd line's middle both waves are at the same pixel in the same step: the forward
+//   wave stores the sum.
+//
+// Only the eight-path orientations use it (rows, columns, both diagonals).
+
+constexpr int TWO_BLOCK = 128;
+constexpr int TWO_RING = 16;
+
+template <int K, int PREFETCH>
+__global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
+    constexpr int LAG = PREFETCH >= 8 ? 6 : 8;  // steps after which a store has completed
+    static_assert(PREFETCH + LAG < TWO_RING - 1, "the LDS ring must hold the late steps");
+    constexpr int D = 16 * K;
+    constexpr int LINES = 4;
+    // Own bytes per direction, line, step (mod TWO_RING) and lane.
+    __shared__ Bytes<K> ring[2][LINES][TWO_RING][16];
+    const int dir = threadIdx.x >> 6;  // wave-uniform: 0 forward, 1 backward
+    const int lane = threadIdx.x & 15;
+    const int row = (threadIdx.x >> 4) & 3;
+    const int b = blockIdx.x;
+    int o = a.last;
+    for (int k = a.first; k < a.last; k++)
+        if (b < a.block_start[k + 1]) {
+            o = k;
+            break;
+        }
+    const int first_line = (b - a.block_start[o]) * LINES;
+    const int W = a.W, H = a.H;
+    const PairDesc pd = pair_of(a.single, a.table, blockIdx.y);
+    const LineGeometry g = line_geometry(o, first_line + row, W, H);
+    int longest = 0;
+#pragma unroll
+    for (int r = 0; r < LINES; r++) longest = max(longest, line_geometry(o, first_line + r, W, H).length);
+    if (longest == 0) return;  // block-uniform: both waves leave
+
+    const int dy = o == 0 ? 0 : 1;
+    const int dx = o == 0 ? 1 : (o == 1 ? 0 : (o == 2 ? 1 : -1));
+    const uint32_t stride = (uint32_t)((dy * W + dx) * D);
+    const int L = g.length;
+    const uint32_t first = ((uint32_t)g.y0 * W + (uint32_t)g.x0) * D + lane * K;
+    const uint32_t last = first + (uint32_t)max(L - 1, 0) * stride;
+    const uint32_t volume = (uint32_t)W * H * D;
+    const Rsrc costs = buffer_of(pd.b.costs, volume);
+    const Rsrc paired = buffer_of(pd.b.paired[o], volume);
+    const uint32_t p1x2 = (uint32_t)pd.p1 * 0x10001u;
+    const uint32_t p2 = (uint32_t)pd.p2;
+    // This wave's pixel at step 0 and the offset of one step (modulo 2^32 going backward).
+    const uint32_t start = dir == 0 ? first : last;
+    const uint32_t step = dir == 0 ? stride : 0u - stride;
+
+    PathState<K, 16> s;
+    s.reset();
+    Bytes<K> cost[PREFETCH], other[PREFETCH];
+    Bytes<K> none;
+#pragma unroll
+    for (int k = 0; k < (K + 3) / 4; k++) none.w[k] = 0;
+    // The prologue issues what a step issues: a (masked) store, then two loads.
+#pragma unroll
+    for (int p = 0; p < PREFETCH; p++) {
+        __builtin_amdgcn_sched_barrier(0);
+        vstore<K>(paired, MASKED, none);
+        __builtin_amdgcn_sched_barrier(0);
+        cost[p] = vload<K>(costs, start + p * step);
+        other[p] = vload<K>(paired, MASKED);
+    }
+    uint32_t at = start, ahead = start + PREFETCH * step;
+    for (int i0 = 0; i0 < longest; i0 += PREFETCH) {
+#pragma unroll
+        for (int p = 0; p < PREFETCH; p++) {
+            const int i = i0 + p;
+            __builtin_amdgcn_sched_barrier(0);
+            s.step(cost[p], p1x2, p2, lane);
+            const Bytes<K> mine = s.bytes();
+            ring[dir][row][i & (TWO_RING - 1)][lane] = mine;
+            // The store of step i - LAG is complete once at most its two loads and the three
+            // accesses of each later step are outstanding. LDS writes done. Meet the other wave.
+            asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(3 * LAG - 1)
+                         : "memory");
+            const int past = 2 * i - (L - 1);  // < 0 before the middle, 0 at an odd line's middle
+            Bytes<K> theirs = other[p];
+            if (past >= 0 && past < PREFETCH + LAG)
+                theirs = ring[dir ^ 1][row][(i - past) & (TWO_RING - 1)][lane];
+            Bytes<K> out;
+#pragma unroll
+            for (int k = 0; k < (K + 3) / 4; k++) out.w[k] = mine.w[k] + (past < 0 ? 0u : theirs.w[k]);
+            // At the middle only the forward wave stores (the sum of both). The store goes
+            // before this step's loads: the count above relies on that order.
+            vstore<K>(paired, i < L && !(past == 0 && dir == 1) ? at : MASKED, out);
+            __builtin_amdgcn_sched_barrier(0);
+            // Step i + PREFETCH: its costs, and the other's bytes when its store has completed.
+            cost[p] = vload<K>(costs, ahead);
+            const bool stored = 2 * (i + PREFETCH) - (L - 1) >= PREFETCH + LAG;
+            other[p] = vload<K>(paired, stored ? ahead : MASKED);
+            at += step;
+            ahead += step;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // Selection (reference.rs:148-186; nx_stereo.ex:608-641), 16 lanes per pixel.
 
 // Grid y: the pair.
@@ -823,7 +937,8 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
     a.W = W;
     a.H = H;
     a.backward_columns = p.paths == 8;
-    const int per_block = PATH_BLOCK / (lanes_fit(16 * K, sch.lanes) ? sch.lanes : 16);
+    const bool two = sch.waves == 2 && p.paths == 8;
+    const int per_block = two ? 4 : PATH_BLOCK / (lanes_fit(16 * K, sch.lanes) ? sch.lanes : 16);
     auto launch = [&](int first, int last) {
         a.first = first;
         a.last = last;
@@ -832,7 +947,15 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
             a.block_start[o] = blocks;
             if (o >= first && o <= last) blocks += (lines[o] + per_block - 1) / per_block;
         }
-        launch_paths_lanes<16 * K>(a, blocks, count, sch, stream);
+        if (two) {
+            switch (sch.prefetch) {
+                case 2: hipLaunchKernelGGL((path2_kernel<K, 2>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
+                case 8: hipLaunchKernelGGL((path2_kernel<K, 8>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
+                default: hipLaunchKernelGGL((path2_kernel<K, 4>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
+            }
+        } else {
+            launch_paths_lanes<16 * K>(a, blocks, count, sch, stream);
+        }
     };
     int slot = profile_begin(PK_PATH, stream);
     if (sch.split) {

@@ -64,15 +64,16 @@ def test_every_schedule_gives_the_same_disparity(disparities, paths):
     ws = hip.Buffer(_native.workspace_size(w, h, disparities, paths))
     kept = _native.get_schedule(w, h, disparities, paths)
     try:
-        for lanes in LANES[disparities]:
+        shapes = [(lanes, 1) for lanes in LANES[disparities]] + [(16, 2)]
+        for lanes, waves in shapes:
             for prefetch in (2, 4, 8):
                 for split in (False, True):
-                    _native.set_schedule(w, h, disparities, paths, prefetch, split, lanes)
+                    _native.set_schedule(w, h, disparities, paths, prefetch, split, lanes, waves)
                     _native.match_device(l.ptr, r.ptr, w, h, disparities, paths, 8, 64, 1.0526316,
                                          o.ptr, ws.ptr, ws.nbytes)
                     hip.synchronize()
                     got = o.download((h, w), np.float32)
-                    case = (lanes, prefetch, split)
+                    case = (lanes, waves, prefetch, split)
                     assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), case
                     sums = _native.match_host(left, right, disparities, paths, intermediates=True)["sums"]
                     assert np.array_equal(sums, want_sums), case
@@ -80,8 +81,26 @@ def test_every_schedule_gives_the_same_disparity(disparities, paths):
         _native.set_schedule(w, h, disparities, paths, *kept)
 
 
+def test_two_waves_on_odd_and_short_lines():
+    """Two waves per line: lines shorter than the prefetch and LDS windows, odd and even
+    lengths (the middle pixel), every diagonal length from 1 up."""
+    kept = _native.get_schedule(17, 40, 64, 8)
+    try:
+        for w, h in [(9, 7), (10, 8), (17, 40), (33, 17)]:
+            left, right = scenes.scene(w, h, 3)
+            want = oracle.aggregate(oracle.costs(oracle.census(left), oracle.census(right), 64), 8, 8, 64)
+            for prefetch in (2, 4, 8):
+                _native.set_schedule(w, h, 64, 8, prefetch, False, 16, 2)
+                sums = _native.match_host(left, right, 64, 8, intermediates=True)["sums"]
+                assert np.array_equal(sums, want), (w, h, prefetch)
+    finally:
+        _native.set_schedule(17, 40, 64, 8, *kept)
+
+
 def test_lanes_that_do_not_fit_are_refused():
     with pytest.raises(_native.NativeError, match="do not fit"):
         _native.set_schedule(97, 31, 192, 8, 4, False, 32)
     with pytest.raises(_native.NativeError, match="do not fit"):
         _native.set_schedule(97, 31, 64, 8, 4, False, 64)
+    with pytest.raises(_native.NativeError, match="16 lanes"):
+        _native.set_schedule(97, 31, 128, 8, 4, False, 32, 2)
