@@ -334,8 +334,8 @@ def traffic_from(fetch_csv: str, write_csv: str, out: str) -> int:
             continue
         kernels[k] = {"fetch_bytes": round(fetch[k] * 1024 * fetch_factor),
                       "write_bytes": round(write.get(k, 0.0) * 1024 * write_factor)}
-    aggregation = sum(kernels[k]["fetch_bytes"] + kernels[k]["write_bytes"]
-                      for k in ("path_kernel", "select_kernel") if k in kernels)
+    aggregation = sum(v["fetch_bytes"] + v["write_bytes"] for k, v in kernels.items()
+                      if k.startswith("path") or k == "select_kernel")
     result = {"fetch_factor": round(fetch_factor, 4), "write_factor": round(write_factor, 4),
               "calibration": "hipMemcpy device-to-device of 256 MiB (reads and writes 256 MiB)",
               "kernels": kernels, "aggregation_traffic_bytes": aggregation,
@@ -469,7 +469,7 @@ def main() -> int:
     if traffic:
         result["roofline"]["traffic"] = traffic["aggregation_traffic_bytes"]
         result["roofline"]["traffic_note"] = (
-            "rocprofv3 FETCH_SIZE x %.2f + WRITE_SIZE x %.2f of path_kernel + select_kernel, "
+            "rocprofv3 FETCH_SIZE x %.2f + WRITE_SIZE x %.2f of the path and select kernels, "
             "per pair (profiles/traffic_728x544x128.json); the working set fits the 256 MB "
             "Infinity Cache, so part of it never reaches HBM" % (traffic["fetch_factor"],
                                                                   traffic["write_factor"]))
