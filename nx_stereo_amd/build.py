@@ -1,7 +1,9 @@
-"""Builds libnxs_hip.so, the gfx950 library behind include/nxs_hip.h, in-tree.
+"""Builds libnxs_hip.so, the gfx950 library behind include/nxs_hip.h, in-tree,
+and stereo_match_hip, the Port executable over it.
 
-Each source compiles to an object with hipcc for gfx950, in parallel, and
-the objects link into nx_stereo_amd/libnxs_hip.so. Usage:
+Each library source compiles to an object with hipcc for gfx950, in parallel,
+and the objects link into nx_stereo_amd/libnxs_hip.so; port_main.cpp (host
+code) links against it into nx_stereo_amd/stereo_match_hip. Usage:
 ``python -m nx_stereo_amd.build``.
 """
 from __future__ import annotations
@@ -15,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIBRARY = os.path.join(HERE, "libnxs_hip.so")
+PORT = os.path.join(HERE, "stereo_match_hip")  # the Port executable over the library
 SOURCES = ["sgm.hip", "stages.hip", "runtime.cpp"]
 ARCH = "gfx950"
 
@@ -67,8 +70,17 @@ def build(verbose: bool = False) -> str:
     if result.returncode != 0:
         raise RuntimeError(f"link failed:\n{result.stderr}")
     os.replace(tmp, LIBRARY)
+    # The Port executable: host code only, linked against the library beside it.
+    tmp = PORT + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT}/include",
+           os.path.join(CSRC, "port_main.cpp"), "-o", tmp, f"-L{HERE}", "-lnxs_hip",
+           "-Wl,-rpath,$ORIGIN", "-lpthread"]
+    result = subprocess.run(cmd, capture_output=True, text=True)
+    if result.returncode != 0:
+        raise RuntimeError(f"building stereo_match_hip failed:\n{result.stderr}")
+    os.replace(tmp, PORT)
     if verbose:
-        print(f"built {LIBRARY}")
+        print(f"built {LIBRARY} and {PORT}")
     return LIBRARY
 
 

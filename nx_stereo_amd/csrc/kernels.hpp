@@ -85,7 +85,9 @@ hipError_t launch_despeckle(const float *in, float *out, int width, int height, 
 hipError_t launch_depth(const float *disparity, int pixels, float focal_length_px,
                         float baseline_m, uint16_t *depth_mm, float *depth_m, hipStream_t stream);
 
-// `partials` holds at least cloud_partials(width, height, decimation) ints.
+// `partials` holds at least cloud_partials(width, height, decimation) ints, the last of which
+// (for the largest such count in use) sits directly before `kept` and must start at zero: it
+// is the finishing counter, which each launch leaves at zero.
 int cloud_partials(int width, int height, int decimation);
 hipError_t launch_cloud(const float *depth_m, int width, int height, int decimation,
                         float focal_length_px, float cx, float cy, float nearest, float farthest,
@@ -95,7 +97,8 @@ hipError_t launch_temporal(float *disparity, float *previous, float *missed, int
                            float alpha, float delta, float hold, float max_missed,
                            hipStream_t stream);
 
-// `partials` holds quality_scratch_bytes(width, height) of scratch.
+// `partials` holds quality_scratch_bytes(width, height) of scratch, zeroed before the first
+// launch (its finishing counter; each launch leaves it at zero).
 size_t quality_scratch_bytes(int width, int height);
 hipError_t launch_quality(const float *disparity, float *previous, int width, int height,
                           double *partials, float *sums5, hipStream_t stream);

@@ -514,6 +514,12 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
     s->cloud_partials = reinterpret_cast<int32_t *>(carve(4 * (partials + 1)));
     s->cloud_kept = s->cloud_partials + partials;
     s->quality_partials = reinterpret_cast<double *>(carve(quality_scratch_bytes(g.width, g.height)));
+    // The cloud and quality kernels' finishing counters start at zero (each launch leaves
+    // them at zero again).
+    HIP_TRY(launch_fill_f32(reinterpret_cast<float *>(s->cloud_partials), 0.0f, partials + 1,
+                            s->stream));
+    HIP_TRY(launch_fill_f32(reinterpret_cast<float *>(s->quality_partials), 0.0f,
+                            quality_scratch_bytes(g.width, g.height) / sizeof(float), s->stream));
     s->quality_sums = reinterpret_cast<float *>(carve(4 * 8));
     s->workspace_bytes = match;
     for (auto &m : s->members) {

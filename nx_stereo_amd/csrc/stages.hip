@@ -7,6 +7,29 @@
 
 namespace nxs {
 
+__global__ void fill_f32_kernel(float *dst, float value, size_t count) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = value;
+}
+
+// Whether this block is the last of `blocks` to finish. Each block's threads have written
+// their share; thread 0 fences at device scope (the writes reach memory every XCD reads)
+// and counts the block in. The block that brings the count to `blocks` fences again (so
+// it sees every other block's writes) and sets the counter back to zero for the next
+// launch, so no clear is needed between launches.
+__device__ bool last_block(unsigned *counter, unsigned blocks) {
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(counter, 1u) == blocks - 1;
+        if (last) atomicExch(counter, 0u);
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
 // ------------------------------------------------------------------------------------------
 // Rectification (rectify.rs:34-128; rectify.ex:133-207): per output pixel the projection
 // through inverse(P3 R), plumb_bob distortion and K, a centre-aligned scale to the frame,
@@ -152,11 +175,14 @@ hipError_t launch_depth(const float *disparity, int pixels, float f, float b, ui
 // PointCloud (depth.rs:58-89; depth.ex:73-90): every decimation-th pixel unprojected where
 // nearest < z < farthest, NaN otherwise, and the kept count.
 
+// `partials` holds one count per block; the last block to finish adds them up into `kept`.
 __global__ __launch_bounds__(256) void cloud_kernel(const float *__restrict__ depth_m, int W,
                                                     int gw, int count, int dec, float f, float cx,
                                                     float cy, float nearest, float farthest,
                                                     float *__restrict__ points,
-                                                    int32_t *__restrict__ partials) {
+                                                    int32_t *__restrict__ partials,
+                                                    unsigned *__restrict__ counter,
+                                                    int32_t *__restrict__ kept_total) {
     __shared__ int32_t wave_kept[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int kept = 0;
@@ -184,36 +210,40 @@ __global__ __launch_bounds__(256) void cloud_kernel(const float *__restrict__ de
     __syncthreads();
     if (threadIdx.x == 0)
         partials[blockIdx.x] = wave_kept[0] + wave_kept[1] + wave_kept[2] + wave_kept[3];
-}
-
-__global__ __launch_bounds__(256) void sum_partials_kernel(const int32_t *__restrict__ partials,
-                                                           int n, int32_t *__restrict__ total) {
+    const unsigned blocks = gridDim.x;
+    if (!last_block(counter, blocks)) return;
     __shared__ int32_t sums[256];
-    int s = 0;
-    for (int i = threadIdx.x; i < n; i += 256) s += partials[i];
-    sums[threadIdx.x] = s;
+    int32_t total = 0;
+    for (unsigned b = threadIdx.x; b < blocks; b += 256) total += partials[b];
+    sums[threadIdx.x] = total;
     __syncthreads();
     for (int half = 128; half > 0; half >>= 1) {
         if ((int)threadIdx.x < half) sums[threadIdx.x] += sums[threadIdx.x + half];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *total = sums[0];
+    if (threadIdx.x == 0) *kept_total = sums[0];
 }
 
 int cloud_partials(int W, int H, int dec) {
     const int count = (W / dec) * (H / dec);
-    return (count + 255) / 256;
+    return (count + 255) / 256 + 1;  // one count per block, then room for the counter
 }
 
 hipError_t launch_cloud(const float *depth_m, int W, int H, int dec, float f, float cx, float cy,
                         float nearest, float farthest, float *points, int32_t *partials,
                         int32_t *kept, hipStream_t stream) {
     const int gw = W / dec, count = gw * (H / dec);
-    const int blocks = cloud_partials(W, H, dec);
-    if (blocks > 0)
+    const int blocks = cloud_partials(W, H, dec) - 1;
+    if (blocks > 0) {
+        // The finishing counter sits just before `kept`, past every member's counts.
         hipLaunchKernelGGL(cloud_kernel, dim3(blocks), dim3(256), 0, stream, depth_m, W, gw, count,
-                           dec, f, cx, cy, nearest, farthest, points, partials);
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, stream, partials, blocks, kept);
+                           dec, f, cx, cy, nearest, farthest, points, partials,
+                           reinterpret_cast<unsigned *>(kept - 1), kept);
+    } else {
+        // No grid point: the count is zero.
+        hipLaunchKernelGGL(fill_f32_kernel, dim3(1), dim3(64), 0, stream,
+                           reinterpret_cast<float *>(kept), 0.0f, (size_t)1);
+    }
     return hipGetLastError();
 }
 
@@ -254,81 +284,78 @@ hipError_t launch_temporal(float *disparity, float *previous, float *missed, int
 // of [matched, v, v^2, both, |c-p| * both]; each column's sum cast to f32; then an f64 sum
 // over columns, cast to f32. The carried previous disparity takes the current.
 //
-// The rows are summed in chunks of QUALITY_ROWS in parallel, then the chunks in order, a
-// fixed association: results repeat exactly from run to run and agree with the row-order
-// sum to far better than the reference's 1e-5 relative bar (f64 partials of f32 terms).
+// One launch. A block takes QUALITY_COLS columns; its QUALITY_PHASES thread rows sum every
+// QUALITY_PHASES-th image row, and the phases of a column are added in order. Every term is
+// an f32 value (matched and both are 0 or 1, and a matched v is at least 0.5) whose bits
+// fall within an f64 column sum's 53, so each column's f64 sum is exact in any order and
+// its f32 cast is the reference's. The columns' f32 sums are added by a fixed tree within a
+// block and over the blocks in order by the last block to finish.
 
-constexpr int QUALITY_ROWS = 16;
+constexpr int QUALITY_COLS = 64;
+constexpr int QUALITY_PHASES = 16;
 
-__global__ __launch_bounds__(256) void quality_rows_kernel(const float *__restrict__ disparity,
-                                                           float *__restrict__ previous, int W,
-                                                           int H, double *__restrict__ partials) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int chunk = blockIdx.y;
-    if (x >= W) return;
+__global__ __launch_bounds__(QUALITY_COLS * QUALITY_PHASES) void quality_kernel(
+    const float *__restrict__ disparity, float *__restrict__ previous, int W, int H,
+    double *__restrict__ partials, unsigned *__restrict__ counter, float *__restrict__ sums5) {
+    __shared__ double column[QUALITY_PHASES][5][QUALITY_COLS];
+    const int lx = threadIdx.x % QUALITY_COLS, phase = threadIdx.x / QUALITY_COLS;
+    const int x = blockIdx.x * QUALITY_COLS + lx;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
-    const int y_end = min(H, (chunk + 1) * QUALITY_ROWS);
-    for (int y = chunk * QUALITY_ROWS; y < y_end; y++) {
-        const size_t i = (size_t)y * W + x;
-        const float c = disparity[i], p = previous[i];
-        const float matched = c > 0.0f ? 1.0f : 0.0f;
-        const float both = __fmul_rn(p > 0.0f ? 1.0f : 0.0f, matched);
-        const float value = __fmul_rn(c, matched);
-        s0 += (double)matched;
-        s1 += (double)value;
-        s2 += (double)__fmul_rn(value, value);
-        s3 += (double)both;
-        s4 += (double)__fmul_rn(fabsf(__fsub_rn(c, p)), both);
-        previous[i] = c;
+    if (x < W) {
+        for (int y = phase; y < H; y += QUALITY_PHASES) {
+            const size_t i = (size_t)y * W + x;
+            const float c = disparity[i], p = previous[i];
+            const float matched = c > 0.0f ? 1.0f : 0.0f;
+            const float both = __fmul_rn(p > 0.0f ? 1.0f : 0.0f, matched);
+            const float value = __fmul_rn(c, matched);
+            s0 += (double)matched;
+            s1 += (double)value;
+            s2 += (double)__fmul_rn(value, value);
+            s3 += (double)both;
+            s4 += (double)__fmul_rn(fabsf(__fsub_rn(c, p)), both);
+            previous[i] = c;
+        }
     }
-    double *out = partials + (size_t)chunk * 5 * W + x;
-    out[0 * W] = s0;
-    out[1 * W] = s1;
-    out[2 * W] = s2;
-    out[3 * W] = s3;
-    out[4 * W] = s4;
-}
-
-// One block per plane: each column's chunks in order, cast to f32, then an f64 tree over
-// the columns.
-__global__ __launch_bounds__(256) void quality_total_kernel(const double *__restrict__ partials,
-                                                            int W, int chunks,
-                                                            float *__restrict__ sums5) {
-    __shared__ double partial[256];
-    const int plane = blockIdx.x;
-    double s = 0;
-    for (int x = threadIdx.x; x < W; x += 256) {
-        double column = 0;
-        for (int c = 0; c < chunks; c++) column += partials[((size_t)c * 5 + plane) * W + x];
-        s += (double)(float)column;
-    }
-    partial[threadIdx.x] = s;
+    column[phase][0][lx] = s0;
+    column[phase][1][lx] = s1;
+    column[phase][2][lx] = s2;
+    column[phase][3][lx] = s3;
+    column[phase][4][lx] = s4;
     __syncthreads();
-    for (int half = 128; half > 0; half >>= 1) {
-        if ((int)threadIdx.x < half) partial[threadIdx.x] += partial[threadIdx.x + half];
+    // Threads 0..5*64-1: plane k = t / 64 of column t % 64; the column's f32 sum, in f64.
+    const int t = threadIdx.x, k = t / QUALITY_COLS;
+    if (k < 5) {
+        double sum = 0;
+        for (int q = 0; q < QUALITY_PHASES; q++) sum += column[q][k][lx];
+        column[0][k][lx] = (double)(float)sum;
+    }
+    __syncthreads();
+    for (int half = QUALITY_COLS / 2; half > 0; half >>= 1) {
+        if (k < 5 && lx < half) column[0][k][lx] += column[0][k][lx + half];
         __syncthreads();
     }
-    if (threadIdx.x == 0) sums5[plane] = (float)partial[0];
+    if (t < 5) partials[(size_t)blockIdx.x * 5 + t] = column[0][t][0];
+    if (!last_block(counter, gridDim.x)) return;
+    if (t < 5) {
+        double total = 0;
+        for (unsigned b = 0; b < gridDim.x; b++) total += partials[(size_t)b * 5 + t];
+        sums5[t] = (float)total;
+    }
 }
 
 size_t quality_scratch_bytes(int W, int H) {
-    const size_t chunks = (size_t)(H + QUALITY_ROWS - 1) / QUALITY_ROWS;
-    return chunks * 5 * (size_t)W * sizeof(double);
+    (void)H;
+    const size_t blocks = (size_t)(W + QUALITY_COLS - 1) / QUALITY_COLS;
+    return blocks * 5 * sizeof(double) + sizeof(double);  // the block sums, then the counter
 }
 
 hipError_t launch_quality(const float *disparity, float *previous, int W, int H, double *partials,
                           float *sums5, hipStream_t stream) {
-    const int chunks = (H + QUALITY_ROWS - 1) / QUALITY_ROWS;
-    hipLaunchKernelGGL(quality_rows_kernel, dim3((W + 255) / 256, chunks), dim3(256), 0, stream,
-                       disparity, previous, W, H, partials);
-    hipLaunchKernelGGL(quality_total_kernel, dim3(5), dim3(256), 0, stream, partials, W, chunks,
-                       sums5);
+    const int blocks = (W + QUALITY_COLS - 1) / QUALITY_COLS;
+    unsigned *counter = reinterpret_cast<unsigned *>(partials + (size_t)blocks * 5);
+    hipLaunchKernelGGL(quality_kernel, dim3(blocks), dim3(QUALITY_COLS * QUALITY_PHASES), 0, stream,
+                       disparity, previous, W, H, partials, counter, sums5);
     return hipGetLastError();
-}
-
-__global__ void fill_f32_kernel(float *dst, float value, size_t count) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) dst[i] = value;
 }
 
 hipError_t launch_fill_f32(float *dst, float value, size_t count, hipStream_t stream) {
