@@ -13,9 +13,12 @@
 //   start and a failure to open it;
 // * fd 1 is pointed at stderr and records go to a duplicate of the real stdout, so nothing
 //   a library prints can cut a record (framing.rs:52-71);
-// * a closed pipe ends the program quietly (main.rs:21-26).
+// * a closed pipe ends the program quietly (main.rs:21-26);
+// * `--threads N`, which NxStereo.Matcher passes from its :threads option
+//   (matcher.ex:263-266), is checked as main.rs:28-38 checks it and otherwise unused: the
+//   work runs on the GPU.
 //
-// Usage: stereo_match_hip [--device N]
+// Usage: stereo_match_hip [--threads N] [--device N]
 #include <errno.h>
 #include <signal.h>
 #include <stdint.h>
@@ -104,19 +107,40 @@ struct Order {
 
 }  // namespace
 
+// An integer of at least `lowest`, all of `text`.
+bool integer(const char *text, int lowest, int *value) {
+    char *end = nullptr;
+    const long v = strtol(text, &end, 10);
+    if (end == text || *end != '\0' || v < lowest || v > 1 << 30) return false;
+    *value = (int)v;
+    return true;
+}
+
 int main(int argc, char **argv) {
-    int device = 0;
-    for (int i = 1; i < argc; i++) {
-        if (strcmp(argv[i], "--device") == 0 && i + 1 < argc) {
-            device = atoi(argv[++i]);
-        } else {
-            fprintf(stderr, "usage: stereo_match_hip [--device N]\n");
-            return 2;
-        }
-    }
     signal(SIGPIPE, SIG_IGN);  // a closed Port shows as a failed write
     const int out = dup(1);
     if (out < 0 || dup2(2, 1) < 0) return 1;
+    int device = 0, threads = 0;
+    for (int i = 1; i < argc; i++) {
+        const bool valued = i + 1 < argc;
+        if (strcmp(argv[i], "--threads") == 0 && valued) {
+            if (!integer(argv[++i], 1, &threads)) {
+                write_log(out, LOG_ERROR,
+                          std::string("--threads takes a positive integer, got ") + argv[i]);
+                return 2;
+            }
+        } else if (strcmp(argv[i], "--device") == 0 && valued) {
+            if (!integer(argv[++i], 0, &device)) {
+                write_log(out, LOG_ERROR, std::string("--device takes a device index, got ") + argv[i]);
+                return 2;
+            }
+        } else {
+            std::string got;
+            for (int k = 1; k < argc; k++) got += std::string(k > 1 ? " " : "") + argv[k];
+            write_log(out, LOG_ERROR, "usage: stereo_match_hip [--threads N] [--device N], got " + got);
+            return 2;
+        }
+    }
 
     nxs_ctx *ctx = nullptr;
     char err[512] = {0};
@@ -125,7 +149,10 @@ int main(int argc, char **argv) {
                                       ": " + err);
         return 1;
     }
-    if (!write_log(out, LOG_INFO, "computing on HIP device " + std::to_string(device))) {
+    if (!write_log(out, LOG_INFO, "computing on HIP device " + std::to_string(device) +
+                                      (threads ? " (--threads " + std::to_string(threads) +
+                                                     ": host threads are not used)"
+                                               : ""))) {
         nxs_close(ctx);
         return 0;
     }
