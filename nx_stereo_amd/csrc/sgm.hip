@@ -557,33 +557,40 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
 // The same aggregation with a line's two directions in two waves of one block: wave 0 walks
 // forward, wave 1 backward, over the same four lines (16 lanes each). Each wave issues half
 // the instructions per step of the one-wave form, so a line's sequential walk, which bounds
-// the kernel, takes about half the time; the two waves meet through memory and LDS:
+// the kernel, takes about half the time. The two waves meet through memory and LDS:
 //
-// * At every step each wave writes its own bytes into an LDS ring of its last TWO_RING
-//   steps, waits until its global stores from LAG steps ago have completed and its LDS
-//   writes are done, and meets the other wave at s_barrier. After the barrier of step i,
-//   the other's stores of steps <= i - LAG are visible, and its bytes of the last
-//   TWO_RING steps are in LDS.
+// * A wave that meets the other at s_barrier first waits until its global stores from LAG
+//   steps ago have completed (and its LDS writes are done). After a barrier at step b, the
+//   other's stores of steps <= b - LAG are visible.
 // * Past the middle, a wave needs the other's bytes of its own pixel, which the other made
-//   at step i - past (past = 2i - (L-1)): from memory, loaded PREFETCH steps ahead, when
-//   past >= PREFETCH + LAG; otherwise from the LDS ring (past < TWO_RING - 1, so the
-//   slot is not rewritten before it is read).
+//   at step i - past (past = 2i - (L-1)). When past >= X they come from memory, loaded
+//   PREFETCH steps ahead. Nearer the middle they come from an LDS ring of each wave's last
+//   RING steps, which past < RING - 1 keeps from being rewritten before it is read.
+// * The ring and a barrier at every step are needed only within X steps (in 2i) of the
+//   block's lines' middles, the window; elsewhere the waves meet once every PREFETCH
+//   steps. X = 2 PREFETCH + LAG + 2 covers a prefetch issued up to 2 PREFETCH - 1 steps
+//   after the last barrier.
 // * At an odd line's middle both waves are at the same pixel in the same step: the forward
 //   wave stores the sum.
 //
+// A CPU model of these rules (every line length up to 120, blocks of unequal lines, each
+// prefetch depth) checks that every read is of a completed store or a visible ring slot.
 // Only the eight-path orientations use it (rows, columns, both diagonals).
 
 constexpr int TWO_BLOCK = 128;
-constexpr int TWO_RING = 16;
 
 template <int K, int PREFETCH>
 __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
-    constexpr int LAG = PREFETCH >= 8 ? 6 : 8;  // steps after which a store has completed
-    static_assert(PREFETCH + LAG < TWO_RING - 1, "the LDS ring must hold the late steps");
+    // Steps after which a store has completed (the compiler's own wait for the loads of
+    // PREFETCH steps ago already implies it), the window's half-width, and the ring's depth.
+    constexpr int LAG = PREFETCH == 4 ? 5 : 6;
+    constexpr int X = 2 * PREFETCH + LAG + 2;
+    constexpr int RING = X + 1 <= 16 ? 16 : 32;
+    static_assert(X <= RING - 1, "the LDS ring must hold the window's late steps");
     constexpr int D = 16 * K;
     constexpr int LINES = 4;
-    // Own bytes per direction, line, step (mod TWO_RING) and lane.
-    __shared__ Bytes<K> ring[2][LINES][TWO_RING][16];
+    // Own bytes per direction, line, step (mod RING) and lane.
+    __shared__ Bytes<K> ring[2][LINES][RING][16];
     const int dir = threadIdx.x >> 6;  // wave-uniform: 0 forward, 1 backward
     const int lane = threadIdx.x & 15;
     const int row = (threadIdx.x >> 4) & 3;
@@ -598,10 +605,17 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
     const int W = a.W, H = a.H;
     const PairDesc pd = pair_of(a.single, a.table, blockIdx.y);
     const LineGeometry g = line_geometry(o, first_line + row, W, H);
-    int longest = 0;
+    // Block-uniform: the longest line, and the window around the lines' middles (in 2i).
+    int longest = 0, mid_lo = 1 << 30, mid_hi = -(1 << 30);
 #pragma unroll
-    for (int r = 0; r < LINES; r++) longest = max(longest, line_geometry(o, first_line + r, W, H).length);
-    if (longest == 0) return;  // block-uniform: both waves leave
+    for (int r = 0; r < LINES; r++) {
+        const int len = line_geometry(o, first_line + r, W, H).length;
+        longest = max(longest, len);
+        mid_lo = min(mid_lo, len - 1);
+        mid_hi = max(mid_hi, len - 1);
+    }
+    if (longest == 0) return;  // both waves leave
+    const int window_lo = mid_lo - X, window_hi = mid_hi + X;
 
     const int dy = o == 0 ? 0 : 1;
     const int dx = o == 0 ? 1 : (o == 1 ? 0 : (o == 2 ? 1 : -1));
@@ -641,15 +655,17 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
             __builtin_amdgcn_sched_barrier(0);
             s.step(cost[p], p1x2, p2, lane);
             const Bytes<K> mine = s.bytes();
-            ring[dir][row][i & (TWO_RING - 1)][lane] = mine;
-            // The store of step i - LAG is complete once at most its two loads and the three
-            // accesses of each later step are outstanding. LDS writes done. Meet the other wave.
-            asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(3 * LAG - 1)
-                         : "memory");
+            const bool window = 2 * i >= window_lo && 2 * i <= window_hi;  // block-uniform
+            if (window) ring[dir][row][i & (RING - 1)][lane] = mine;
+            if (window || p == 0) {
+                // The store of step i - LAG is complete once at most its two loads and the
+                // three accesses of each later step are outstanding; LDS writes done.
+                asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(3 * LAG - 1)
+                             : "memory");
+            }
             const int past = 2 * i - (L - 1);  // < 0 before the middle, 0 at an odd line's middle
             Bytes<K> theirs = other[p];
-            if (past >= 0 && past < PREFETCH + LAG)
-                theirs = ring[dir ^ 1][row][(i - past) & (TWO_RING - 1)][lane];
+            if (past >= 0 && past < X) theirs = ring[dir ^ 1][row][(i - past) & (RING - 1)][lane];
             Bytes<K> out;
 #pragma unroll
             for (int k = 0; k < (K + 3) / 4; k++) out.w[k] = mine.w[k] + (past < 0 ? 0u : theirs.w[k]);
@@ -659,7 +675,7 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
             __builtin_amdgcn_sched_barrier(0);
             // Step i + PREFETCH: its costs, and the other's bytes when its store has completed.
             cost[p] = vload<K>(costs, ahead);
-            const bool stored = 2 * (i + PREFETCH) - (L - 1) >= PREFETCH + LAG;
+            const bool stored = 2 * (i + PREFETCH) - (L - 1) >= X;
             other[p] = vload<K>(paired, stored ? ahead : MASKED);
             at += step;
             ahead += step;
