@@ -140,7 +140,9 @@ typedef struct nxs_schedule {
     int split;    /* 1: one launch per line orientation */
     int lanes;    /* lanes sharing one line: 16, 32 or 64, where disparities / lanes is
                      2, 4, 8, 12 or 16 */
-    int waves;    /* 2: a line's two directions in two waves (8 paths; 16 lanes) */
+    int waves;    /* waves per line (8 paths; 16 lanes): 2, a wave per direction; 4 or 8,
+                     each direction's disparities over 2 or 4 waves as well, where
+                     disparities / (8 x waves) is 2, 4 or 8 */
 } nxs_schedule;
 
 int nxs_schedule_set(int width, int height, int disparities, int paths, const nxs_schedule *s);

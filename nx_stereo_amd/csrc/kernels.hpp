@@ -50,8 +50,12 @@ struct Schedule {
     int prefetch = 4;  // 2, 4 or 8
     bool split = false;
     int lanes = 16;
-    int waves = 1;  // 2: a line's two directions in two waves of one block (8 paths, 16 lanes)
+    int waves = 1;  // 8 paths, 16 lanes: 2 = a line's two directions in two waves of one block;
+                    // 4 or 8 = each direction's D split over 2 or 4 waves as well
 };
+
+// Slices per direction (2 or 4) that fit D disparities, or 1 where they do not.
+int slices_fit(int disparities, int slices);
 
 // Whether `lanes` lanes per line fit D disparities (D / lanes is 2, 4, 8, 12 or 16).
 bool lanes_fit(int disparities, int lanes);

@@ -1446,7 +1446,9 @@ struct nxs_batch {
             if (lanes_fit(c.p.disparities, lanes))
                 for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, lanes, 1});
         if (c.p.paths == 8)
-            for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, 16, 2});
+            for (int waves : {2, 4, 8})
+                if (waves == 2 || slices_fit(c.p.disparities, waves / 2) > 1)
+                    for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, 16, waves});
         candidates.push_back(Schedule{4, true, 16, 1});
         for (const Schedule &s : candidates) {
             HIP_TRY(launch_match_batch(c.p, tables + c.first, c.count, s, stream));
@@ -1582,9 +1584,13 @@ int nxs_schedule_set(int width, int height, int disparities, int paths, const nx
     if (!nxs::lanes_fit(disparities, s->lanes))
         return nxs::fail(NXS_EINVAL, std::to_string(s->lanes) + " lanes per line do not fit " +
                                          std::to_string(disparities) + " disparities");
-    if (s->waves != 1 && s->waves != 2) return nxs::fail(NXS_EINVAL, "waves must be 1 or 2");
-    if (s->waves == 2 && s->lanes != 16)
-        return nxs::fail(NXS_EINVAL, "two waves per line take 16 lanes per line");
+    if (s->waves != 1 && s->waves != 2 && s->waves != 4 && s->waves != 8)
+        return nxs::fail(NXS_EINVAL, "waves must be 1, 2, 4 or 8");
+    if (s->waves > 1 && s->lanes != 16)
+        return nxs::fail(NXS_EINVAL, "two or more waves per line take 16 lanes per line");
+    if (s->waves > 2 && nxs::slices_fit(disparities, s->waves / 2) == 1)
+        return nxs::fail(NXS_EINVAL, std::to_string(s->waves) + " waves per line do not fit " +
+                                         std::to_string(disparities) + " disparities");
     nxs::remember_schedule(width, height, disparities, paths,
                            nxs::Schedule{s->prefetch, s->split != 0, s->lanes, s->waves});
     return NXS_OK;

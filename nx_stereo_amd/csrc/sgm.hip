@@ -684,6 +684,176 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------
+// The two-wave form with each direction's D disparities also split into G slices of 16
+// lanes, one wave each: 2G waves per block over the same four lines, each issuing about
+// 1/G of a direction's per-step work. At every step the slices of a direction exchange,
+// through LDS and one s_barrier, what the next step needs from the others: each slice's
+// minimum (the line's minimum is theirs), its first lane's first pair (the right neighbour
+// of the slice before) and its last lane's last pair (the left neighbour of the slice
+// after). The exchange is double-buffered by step parity: a wave writes step t+1's values
+// only after the barrier of step t, which every reader of step t-1's values has passed.
+// The two directions meet as in the two-wave form, with a barrier at every step (X below).
+
+struct SliceShare {
+    uint32_t minimum, first, last;
+};
+
+template <int K, int PREFETCH, int G>
+__global__ __launch_bounds__(128 * G) void path_sliced_kernel(PathArgs a) {
+    constexpr int LAG = PREFETCH == 8 ? 6 : 8;  // steps after which a store has completed
+    constexpr int X = PREFETCH + LAG;            // past >= X: the other's bytes from memory
+    constexpr int RING = X + 1 <= 15 ? 16 : 32;  // past < RING - 1: the ring slot still holds them
+    static_assert(X <= RING - 2, "the LDS ring must hold the late steps");
+    constexpr int R = K / 2;
+    constexpr int D = 16 * K * G;
+    constexpr int LINES = 4;
+    constexpr uint32_t SENT = 0x3FFF3FFFu;
+    __shared__ Bytes<K> ring[2][LINES][RING][16 * G];
+    __shared__ SliceShare share[2][2][LINES][G];  // [step parity][direction][line][slice]
+    const int wave = threadIdx.x >> 6;
+    const int dir = wave / G, slice = wave % G;  // wave-uniform
+    const int lane = threadIdx.x & 15;
+    const int row = (threadIdx.x >> 4) & 3;
+    const int b = blockIdx.x;
+    int o = a.last;
+    for (int k = a.first; k < a.last; k++)
+        if (b < a.block_start[k + 1]) {
+            o = k;
+            break;
+        }
+    const int first_line = (b - a.block_start[o]) * LINES;
+    const int W = a.W, H = a.H;
+    const PairDesc pd = pair_of(a.single, a.table, blockIdx.y);
+    const LineGeometry g = line_geometry(o, first_line + row, W, H);
+    int longest = 0, mid_lo = 1 << 30, mid_hi = -(1 << 30);
+#pragma unroll
+    for (int r = 0; r < LINES; r++) {
+        const int len = line_geometry(o, first_line + r, W, H).length;
+        longest = max(longest, len);
+        mid_lo = min(mid_lo, len - 1);
+        mid_hi = max(mid_hi, len - 1);
+    }
+    if (longest == 0) return;  // every wave of the block leaves
+    const int window_lo = mid_lo - X - 2, window_hi = mid_hi + X + 2;  // ring writes (in 2i)
+
+    const int dy = o == 0 ? 0 : 1;
+    const int dx = o == 0 ? 1 : (o == 1 ? 0 : (o == 2 ? 1 : -1));
+    const uint32_t stride = (uint32_t)((dy * W + dx) * D);
+    const int L = g.length;
+    const int column = slice * 16 + lane;  // this lane's disparities: [column*K, column*K+K)
+    const uint32_t first = ((uint32_t)g.y0 * W + (uint32_t)g.x0) * D + column * K;
+    const uint32_t last = first + (uint32_t)max(L - 1, 0) * stride;
+    const uint32_t volume = (uint32_t)W * H * D;
+    const Rsrc costs = buffer_of(pd.b.costs, volume);
+    const Rsrc paired = buffer_of(pd.b.paired[o], volume);
+    const uint32_t p1x2 = (uint32_t)pd.p1 * 0x10001u;
+    const uint32_t p2 = (uint32_t)pd.p2;
+    const uint32_t start = dir == 0 ? first : last;
+    const uint32_t step = dir == 0 ? stride : 0u - stride;
+
+    // The path entering a line has prev = 0 everywhere; past the line's two ends, the
+    // sentinel.
+    uint32_t v[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) v[k] = 0;
+    uint32_t pm = 0;
+    uint32_t left_edge = slice == 0 ? SENT : 0u, right_edge = slice == G - 1 ? SENT : 0u;
+
+    Bytes<K> cost[PREFETCH], other[PREFETCH];
+    Bytes<K> none;
+#pragma unroll
+    for (int k = 0; k < (K + 3) / 4; k++) none.w[k] = 0;
+#pragma unroll
+    for (int p = 0; p < PREFETCH; p++) {
+        __builtin_amdgcn_sched_barrier(0);
+        vstore<K>(paired, MASKED, none);
+        __builtin_amdgcn_sched_barrier(0);
+        cost[p] = vload<K>(costs, start + p * step);
+        other[p] = vload<K>(paired, MASKED);
+    }
+    uint32_t at = start, ahead = start + PREFETCH * step;
+    for (int i0 = 0; i0 < longest; i0 += PREFETCH) {
+#pragma unroll
+        for (int p = 0; p < PREFETCH; p++) {
+            const int i = i0 + p;
+            __builtin_amdgcn_sched_barrier(0);
+            // The recurrence on this slice (as PathState::step), its neighbours across the
+            // slice's ends being the other slices' pairs exchanged last step.
+            const uint32_t left_in = dpp_or<DPP_ROW_SHR1>(left_edge, v[R - 1]);
+            const uint32_t right_in = dpp_or<DPP_ROW_SHL1>(right_edge, v[0]);
+            uint32_t sh[R + 1];
+            sh[0] = (v[0] << 16) | (left_in >> 16);
+#pragma unroll
+            for (int k = 1; k < R; k++) sh[k] = (v[k] << 16) | (v[k - 1] >> 16);
+            sh[R] = (right_in << 16) | (v[R - 1] >> 16);
+            const uint32_t PM = pm * 0x10001u;
+            const uint32_t JUMP = (pm + p2) * 0x10001u;
+            uint32_t c[R];
+            if constexpr (K == 2) {
+                c[0] = lo_pair(cost[p].w[0]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < R / 2; k++) {
+                    c[2 * k] = lo_pair(cost[p].w[k]);
+                    c[2 * k + 1] = hi_pair(cost[p].w[k]);
+                }
+            }
+            uint32_t mn = 0xFFFFFFFFu;
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                uint32_t t = pk_min(sh[k], sh[k + 1]);
+                t = pk_add(t, p1x2);
+                t = pk_min(t, v[k]);
+                t = pk_min(t, JUMP);
+                t = pk_add(pk_sub(t, PM), c[k]);
+                v[k] = t;
+                mn = pk_min(mn, t);
+            }
+            const uint32_t m = row_min_u32(min(mn & 0xFFFFu, mn >> 16));
+            Bytes<K> mine;
+            if constexpr (K == 2) {
+                mine.w[0] = __builtin_amdgcn_perm(0u, v[0], 0x0C0C0200u);
+            } else {
+#pragma unroll
+                for (int k = 0; k < R / 2; k++) mine.w[k] = pack_pairs(v[2 * k], v[2 * k + 1]);
+            }
+            // Share this step's slice minimum and end pairs; publish own bytes near the middle.
+            SliceShare &out_share = share[i & 1][dir][row][slice];
+            if (lane == 0) {
+                out_share.minimum = m;
+                out_share.first = v[0];
+            }
+            if (lane == 15) out_share.last = v[R - 1];
+            const bool window = 2 * i >= window_lo && 2 * i <= window_hi;  // block-uniform
+            if (window) ring[dir][row][i & (RING - 1)][column] = mine;
+            // The store of step i - LAG is complete once at most its two loads and the three
+            // accesses of each later step are outstanding; LDS writes done.
+            asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(3 * LAG - 1) : "memory");
+            uint32_t line_min = share[i & 1][dir][row][0].minimum;
+#pragma unroll
+            for (int q = 1; q < G; q++) line_min = min(line_min, share[i & 1][dir][row][q].minimum);
+            pm = line_min;
+            left_edge = slice == 0 ? SENT : share[i & 1][dir][row][slice > 0 ? slice - 1 : 0].last;
+            right_edge = slice == G - 1 ? SENT : share[i & 1][dir][row][slice < G - 1 ? slice + 1 : 0].first;
+
+            const int past = 2 * i - (L - 1);
+            Bytes<K> theirs = other[p];
+            if (past >= 0 && past < X) theirs = ring[dir ^ 1][row][(i - past) & (RING - 1)][column];
+            Bytes<K> sum;
+#pragma unroll
+            for (int k = 0; k < (K + 3) / 4; k++) sum.w[k] = mine.w[k] + (past < 0 ? 0u : theirs.w[k]);
+            vstore<K>(paired, i < L && !(past == 0 && dir == 1) ? at : MASKED, sum);
+            __builtin_amdgcn_sched_barrier(0);
+            cost[p] = vload<K>(costs, ahead);
+            const bool stored = 2 * (i + PREFETCH) - (L - 1) >= X;
+            other[p] = vload<K>(paired, stored ? ahead : MASKED);
+            at += step;
+            ahead += step;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // Selection (reference.rs:148-186; nx_stereo.ex:608-641), 16 lanes per pixel.
 
 // Grid y: the pair.
@@ -940,6 +1110,36 @@ static void launch_paths_lanes(const PathArgs &a, int blocks, int count, const S
     launch_paths_prefetch<D / 16, 16>(a, blocks, count, sch.prefetch, stream);
 }
 
+// Slices per direction that fit D disparities (D / (16 slices) an even count up to 16), or 1.
+int slices_fit(int disparities, int slices) {
+    if (slices != 2 && slices != 4) return 1;
+    const int k = disparities / (16 * slices);
+    return disparities % (16 * slices) == 0 && (k == 2 || k == 4 || k == 8) ? slices : 1;
+}
+
+template <int K, int G>
+static void launch_sliced_prefetch(const PathArgs &a, int blocks, int count, int prefetch,
+                                   hipStream_t stream) {
+    switch (prefetch) {
+        case 2: hipLaunchKernelGGL((path_sliced_kernel<K, 2, G>), dim3(blocks, count), dim3(128 * G), 0, stream, a); break;
+        case 8: hipLaunchKernelGGL((path_sliced_kernel<K, 8, G>), dim3(blocks, count), dim3(128 * G), 0, stream, a); break;
+        default: hipLaunchKernelGGL((path_sliced_kernel<K, 4, G>), dim3(blocks, count), dim3(128 * G), 0, stream, a); break;
+    }
+}
+
+template <int D>
+static void launch_sliced(const PathArgs &a, int blocks, int count, int prefetch, int slices,
+                          hipStream_t stream) {
+    if constexpr (D == 64) {
+        launch_sliced_prefetch<2, 2>(a, blocks, count, prefetch, stream);
+    } else if constexpr (D == 128 || D == 256) {
+        if (slices == 4)
+            launch_sliced_prefetch<D / 64, 4>(a, blocks, count, prefetch, stream);
+        else
+            launch_sliced_prefetch<D / 32, 2>(a, blocks, count, prefetch, stream);
+    }
+}
+
 template <int K>
 static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &single,
                                           const PairDesc *table, int count, const Schedule &sch,
@@ -953,7 +1153,8 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
     a.W = W;
     a.H = H;
     a.backward_columns = p.paths == 8;
-    const bool two = sch.waves == 2 && p.paths == 8;
+    const bool two = sch.waves >= 2 && p.paths == 8;
+    const int slices = two ? slices_fit(16 * K, sch.waves / 2) : 1;
     const int per_block = two ? 4 : PATH_BLOCK / (lanes_fit(16 * K, sch.lanes) ? sch.lanes : 16);
     auto launch = [&](int first, int last) {
         a.first = first;
@@ -963,7 +1164,9 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
             a.block_start[o] = blocks;
             if (o >= first && o <= last) blocks += (lines[o] + per_block - 1) / per_block;
         }
-        if (two) {
+        if (two && slices > 1) {
+            launch_sliced<16 * K>(a, blocks, count, sch.prefetch, slices, stream);
+        } else if (two) {
             switch (sch.prefetch) {
                 case 2: hipLaunchKernelGGL((path2_kernel<K, 2>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
                 case 8: hipLaunchKernelGGL((path2_kernel<K, 8>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;

@@ -139,10 +139,52 @@ __global__ __launch_bounds__(256) void despeckle_kernel(const float *__restrict_
     out[(size_t)y * W + x] = (c > 0.0f && (float)support >= min_support) ? c : -1.0f;
 }
 
+// The stage's own radius (3, stage/speckle_filter.ex): a 32x8 block of pixels reads its
+// (32+2R)x(8+2R) window into LDS once (outside the image: -1, which never counts), then
+// counts from LDS.
+constexpr int SPECKLE_TX = 32, SPECKLE_TY = 8;
+
+template <int R>
+__global__ __launch_bounds__(SPECKLE_TX * SPECKLE_TY) void despeckle_tiled_kernel(
+    const float *__restrict__ in, float *__restrict__ out, int W, int H, float min_support,
+    float max_diff) {
+    constexpr int TW = SPECKLE_TX + 2 * R, TH = SPECKLE_TY + 2 * R;
+    __shared__ float tile[TH][TW];
+    const int x0 = blockIdx.x * SPECKLE_TX, y0 = blockIdx.y * SPECKLE_TY;
+    for (int i = threadIdx.x; i < TW * TH; i += SPECKLE_TX * SPECKLE_TY) {
+        const int r = i / TW, c = i - r * TW;
+        const int yy = y0 - R + r, xx = x0 - R + c;
+        tile[r][c] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? in[(size_t)yy * W + xx] : -1.0f;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % SPECKLE_TX, ty = threadIdx.x / SPECKLE_TX;
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= W || y >= H) return;
+    const float c = tile[ty + R][tx + R];
+    unsigned support = 0;
+#pragma unroll
+    for (int dy = -R; dy <= R; dy++) {
+#pragma unroll
+        for (int dx = -R; dx <= R; dx++) {
+            if (dy == 0 && dx == 0) continue;
+            const float n = tile[ty + R + dy][tx + R + dx];
+            support += (n > 0.0f) && (fabsf(__fsub_rn(n, c)) <= max_diff);
+        }
+    }
+    out[(size_t)y * W + x] = (c > 0.0f && (float)support >= min_support) ? c : -1.0f;
+}
+
 hipError_t launch_despeckle(const float *in, float *out, int W, int H, float min_support,
                             float max_diff, int radius, hipStream_t stream) {
-    hipLaunchKernelGGL(despeckle_kernel, dim3((W + 255) / 256, H), dim3(256), 0, stream, in, out, W,
-                       H, min_support, max_diff, radius);
+    if (radius == 3) {
+        hipLaunchKernelGGL(despeckle_tiled_kernel<3>,
+                           dim3((W + SPECKLE_TX - 1) / SPECKLE_TX, (H + SPECKLE_TY - 1) / SPECKLE_TY),
+                           dim3(SPECKLE_TX * SPECKLE_TY), 0, stream, in, out, W, H, min_support,
+                           max_diff);
+    } else {
+        hipLaunchKernelGGL(despeckle_kernel, dim3((W + 255) / 256, H), dim3(256), 0, stream, in, out,
+                           W, H, min_support, max_diff, radius);
+    }
     return hipGetLastError();
 }
 

@@ -49,6 +49,7 @@ def test_batch_matches_the_oracle_on_every_replay():
 
 
 LANES = {64: (16, 32), 128: (16, 32, 64), 192: (16,), 256: (16, 32, 64)}
+WAVES = {64: (2, 4), 128: (2, 4, 8), 192: (2,), 256: (2, 4, 8)}  # more than one wave per line
 
 
 @pytest.mark.parametrize("disparities,paths", [(64, 8), (128, 8), (128, 3), (192, 8), (256, 8)])
@@ -64,7 +65,7 @@ def test_every_schedule_gives_the_same_disparity(disparities, paths):
     ws = hip.Buffer(_native.workspace_size(w, h, disparities, paths))
     kept = _native.get_schedule(w, h, disparities, paths)
     try:
-        shapes = [(lanes, 1) for lanes in LANES[disparities]] + [(16, 2)]
+        shapes = [(lanes, 1) for lanes in LANES[disparities]] + [(16, w) for w in WAVES[disparities]]
         for lanes, waves in shapes:
             for prefetch in (2, 4, 8):
                 for split in (False, True):
@@ -81,20 +82,25 @@ def test_every_schedule_gives_the_same_disparity(disparities, paths):
         _native.set_schedule(w, h, disparities, paths, *kept)
 
 
-def test_two_waves_on_odd_and_short_lines():
-    """Two waves per line: lines shorter than the prefetch and LDS windows, odd and even
-    lengths (the middle pixel), every diagonal length from 1 up."""
-    kept = _native.get_schedule(17, 40, 64, 8)
+@pytest.mark.parametrize("disparities", [64, 128])
+def test_several_waves_on_odd_and_short_lines(disparities):
+    """Two or more waves per line: lines shorter than the prefetch and LDS windows, odd and
+    even lengths (the middle pixel), every diagonal length from 1 up."""
+    sizes = [(9, 7), (10, 8), (17, 40), (33, 17)]
+    kept = {size: _native.get_schedule(*size, disparities, 8) for size in sizes}
     try:
-        for w, h in [(9, 7), (10, 8), (17, 40), (33, 17)]:
+        for w, h in sizes:
             left, right = scenes.scene(w, h, 3)
-            want = oracle.aggregate(oracle.costs(oracle.census(left), oracle.census(right), 64), 8, 8, 64)
-            for prefetch in (2, 4, 8):
-                _native.set_schedule(w, h, 64, 8, prefetch, False, 16, 2)
-                sums = _native.match_host(left, right, 64, 8, intermediates=True)["sums"]
-                assert np.array_equal(sums, want), (w, h, prefetch)
+            want = oracle.aggregate(oracle.costs(oracle.census(left), oracle.census(right),
+                                                 disparities), 8, 8, 64)
+            for waves in WAVES[disparities]:
+                for prefetch in (2, 4, 8):
+                    _native.set_schedule(w, h, disparities, 8, prefetch, False, 16, waves)
+                    sums = _native.match_host(left, right, disparities, 8, intermediates=True)["sums"]
+                    assert np.array_equal(sums, want), (w, h, waves, prefetch)
     finally:
-        _native.set_schedule(17, 40, 64, 8, *kept)
+        for size, schedule in kept.items():
+            _native.set_schedule(*size, disparities, 8, *schedule)
 
 
 def test_lanes_that_do_not_fit_are_refused():
@@ -104,3 +110,5 @@ def test_lanes_that_do_not_fit_are_refused():
         _native.set_schedule(97, 31, 64, 8, 4, False, 64)
     with pytest.raises(_native.NativeError, match="16 lanes"):
         _native.set_schedule(97, 31, 128, 8, 4, False, 32, 2)
+    with pytest.raises(_native.NativeError, match="8 waves per line do not fit"):
+        _native.set_schedule(97, 31, 64, 8, 4, False, 16, 8)

@@ -109,6 +109,23 @@ def test_speckle_depth_cloud_quality_chain(context):
         np.testing.assert_allclose(out["quality_sums"], want, rtol=1e-5)
 
 
+@pytest.mark.parametrize("radius", [1, 2, 3, 5])
+def test_speckle_radius(context, radius):
+    """Radius is a build constant (3 from Stage.SpeckleFilter; segment.rs takes any): the
+    LDS-tiled kernel serves 3, the direct one the others. Sizes off the 32x8 tiles."""
+    h, w = 45, 77
+    rng = np.random.default_rng(radius)
+    entry = np.where(rng.random((h, w)) < 0.25, -1.0, rng.uniform(0.5, 9.0, (h, w))).astype(np.float32)
+    g = {"width": w, "height": h, "frame_width": w, "frame_height": h}
+    segment_id = 7100 + radius
+    settings = [("radius", float(radius))]
+    assert context.request(protocol.configure(segment_id, g, [("speckle_filter", settings)])) is True
+    inputs = [[("min_support", np.float32(5)), ("max_diff", np.float32(1.0))]]
+    reply = context.request(protocol.run(segment_id, inputs, [("disparity", entry)]))
+    same_bits(reply.tensors["disparity"], oracle.despeckle(entry, 5, 1.0, radius))
+    context.request(protocol.release(segment_id))
+
+
 def test_temporal_filter_carries_across_pairs(context):
     geometry, _ = rig_geometry(64, 32)
     seg = build(context, [("temporal_filter", stages.TemporalFilter(alpha=0.5, delta=2.0, hold=1))],
