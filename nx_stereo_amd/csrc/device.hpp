@@ -35,6 +35,15 @@ __device__ __forceinline__ uint32_t dpp_or(uint32_t edge, uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)v, CTRL, 0xF, 0xF, false);
 }
 
+// A wave-uniform 64-bit value as scalar registers: each half through readfirstlane. The
+// builtin returns int, so each half goes back through uint32_t before it is widened; a
+// widened int would be sign-extended and corrupt the high half when bit 31 is set.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return hi << 32 | lo;
+}
+
 // Raw buffer access (MUBUF) with 32-bit byte offsets: an offset at or past `bytes` loads
 // zeros and drops stores, which masks an access without a branch.
 typedef __amdgpu_buffer_rsrc_t Rsrc;
@@ -43,12 +52,9 @@ constexpr uint32_t MASKED = 0x80000000u;  // past every buffer made here (all un
 // `base` must be the same in every lane of the wave: it is read from the first active lane,
 // so the descriptor lives in scalar registers.
 __device__ __forceinline__ Rsrc buffer_of(const void *base, uint32_t bytes) {
-    const uint64_t v = reinterpret_cast<uint64_t>(base);
-    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
-    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(hi << 32 | lo), (short)0,
-                                             (int)__builtin_amdgcn_readfirstlane((int)bytes),
-                                             0x00020000);
+    return __builtin_amdgcn_make_buffer_rsrc(
+        reinterpret_cast<void *>(uniform_u64(reinterpret_cast<uint64_t>(base))), (short)0,
+        (int)__builtin_amdgcn_readfirstlane((int)bytes), 0x00020000);
 }
 
 // Minimum over the 16 lanes of a DPP row, left in every lane of the row.

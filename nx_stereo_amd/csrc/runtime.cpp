@@ -66,18 +66,14 @@ void profile_end(int slot, hipStream_t stream) {
 static std::mutex g_schedules_mu;
 static std::map<std::tuple<int, int, int, int>, Schedule> g_schedules;
 
-// Without a pick: below 256 disparities a line's sequential walk bounds the aggregation, and
-// two waves per line (one per direction) halve it; at 256 the aggregation is bound by HBM
-// traffic instead, and one wave per line with the deepest prefetch measured fastest
-// (1920x1080 pairs on MI355X).
+// Without a pick, with 8 paths: two waves per line (one per direction), which halves a
+// line's sequential walk. Loads 4 steps ahead below 256 disparities, 8 at 256, where the
+// aggregation is bound by HBM traffic (the fastest measured on MI355X at 728x544x128,
+// 1280x720x192 and 1920x1080x256). With 3 paths: one wave per line.
 static Schedule default_schedule(int disparities, int paths) {
     Schedule s;
-    if (paths == 8 && disparities < 256) {
-        s.prefetch = 2;
-        s.waves = 2;
-    } else {
-        s.prefetch = 8;
-    }
+    s.prefetch = paths == 8 && disparities < 256 ? 4 : 8;
+    s.waves = paths == 8 ? 2 : 1;
     return s;
 }
 

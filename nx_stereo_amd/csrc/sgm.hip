@@ -187,6 +187,18 @@ typedef uint32_t v2u __attribute__((ext_vector_type(2)));
 typedef uint32_t v3u __attribute__((ext_vector_type(3)));
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 
+// Orientation o's byte volume. A choice among the four pointers as scalar values: a
+// run-time index into the descriptor's array (or selects the compiler folds back into one)
+// would put the descriptor in scratch memory.
+__device__ __forceinline__ uint8_t *orientation_volume(const MatchBuffers &b, int o) {
+    uint64_t v = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        v = o == k ? uniform_u64((uint64_t)b.paired[k]) : v;
+    }
+    return (uint8_t *)v;
+}
+
 // K bytes of one lane at a byte offset of a buffer, as whole dwords (MUBUF loads/stores).
 template <int K>
 __device__ __forceinline__ Bytes<K> vload(Rsrc r, uint32_t at) {
@@ -430,7 +442,7 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
     const uint32_t last = first + (uint32_t)max(L - 1, 0) * stride;
     const uint32_t volume = (uint32_t)W * H * D;
     const Rsrc costs = buffer_of(pd.b.costs, volume);
-    const Rsrc paired = buffer_of(pd.b.paired[o], volume);
+    const Rsrc paired = buffer_of(orientation_volume(pd.b, o), volume);
     const uint32_t p1x2 = (uint32_t)pd.p1 * 0x10001u;
     const uint32_t p2 = (uint32_t)pd.p2;
     // Every step issues the same loads and stores, and the loops run whole groups of PREFETCH
@@ -567,15 +579,27 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
 //   PREFETCH steps ahead. Nearer the middle they come from an LDS ring of each wave's last
 //   RING steps, which past < RING - 1 keeps from being rewritten before it is read.
 // * The ring and a barrier at every step are needed only within X steps (in 2i) of the
-//   block's lines' middles, the window; elsewhere the waves meet once every PREFETCH
-//   steps. X = 2 PREFETCH + LAG + 2 covers a prefetch issued up to 2 PREFETCH - 1 steps
-//   after the last barrier.
+//   block's lines' middles, the window. After it the waves meet once every PREFETCH steps.
+//   X = 2 PREFETCH + LAG + 2 covers a prefetch issued up to 2 PREFETCH - 1 steps after the
+//   last barrier. Before it nothing is read from the other wave, so they do not meet.
 // * At an odd line's middle both waves are at the same pixel in the same step: the forward
 //   wave stores the sum.
+// * A step of the window or after it issues one store and two loads (a load of nothing
+//   where there is nothing to read), which the count in the barrier's wait relies on. A
+//   step of phase A (below) issues a store and one load; the first LAG barriers after it
+//   therefore wait for less, but every read of the other's bytes from memory is prefetched
+//   after a barrier with LAG three-access steps before it.
 //
-// A CPU model of these rules (every line length up to 120, blocks of unequal lines, each
-// prefetch depth) checks that every read is of a completed store or a visible ring slot.
-// Only the eight-path orientations use it (rows, columns, both diagonals).
+// The walk runs in block-uniform phases, each a whole number of PREFETCH steps:
+//   A  before the window: own bytes out, costs in; nothing to mask and no barrier;
+//   the general step around the window and wherever a line of the block has ended;
+//   C  after the window while every line of the block goes on: pair sums out, costs and the
+//      other's bytes in, a barrier every PREFETCH steps.
+// A CPU model of these rules (tests/test_two_wave_rules.py: line lengths up to 260, blocks
+// of unequal lines, each prefetch depth, the wait's count over the accesses actually issued)
+// checks that every read is of a completed store or a visible ring slot, and that phases A
+// and C need no mask. Only the eight-path orientations use it (rows, columns, both
+// diagonals).
 
 constexpr int TWO_BLOCK = 128;
 
@@ -625,7 +649,7 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
     const uint32_t last = first + (uint32_t)max(L - 1, 0) * stride;
     const uint32_t volume = (uint32_t)W * H * D;
     const Rsrc costs = buffer_of(pd.b.costs, volume);
-    const Rsrc paired = buffer_of(pd.b.paired[o], volume);
+    const Rsrc paired = buffer_of(orientation_volume(pd.b, o), volume);
     const uint32_t p1x2 = (uint32_t)pd.p1 * 0x10001u;
     const uint32_t p2 = (uint32_t)pd.p2;
     // This wave's pixel at step 0 and the offset of one step (modulo 2^32 going backward).
@@ -648,7 +672,13 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
         other[p] = vload<K>(paired, MASKED);
     }
     uint32_t at = start, ahead = start + PREFETCH * step;
-    for (int i0 = 0; i0 < longest; i0 += PREFETCH) {
+    // The store of step i - LAG is complete once at most its two loads and the three
+    // accesses of each later step are outstanding; LDS writes done.
+    auto meet = []() __attribute__((always_inline)) {
+        asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(3 * LAG - 1) : "memory");
+    };
+    // Any step: the window's ring and barriers, the middle, lines that have ended.
+    auto general = [&](int i0) __attribute__((always_inline)) {
 #pragma unroll
         for (int p = 0; p < PREFETCH; p++) {
             const int i = i0 + p;
@@ -657,12 +687,7 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
             const Bytes<K> mine = s.bytes();
             const bool window = 2 * i >= window_lo && 2 * i <= window_hi;  // block-uniform
             if (window) ring[dir][row][i & (RING - 1)][lane] = mine;
-            if (window || p == 0) {
-                // The store of step i - LAG is complete once at most its two loads and the
-                // three accesses of each later step are outstanding; LDS writes done.
-                asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(3 * LAG - 1)
-                             : "memory");
-            }
+            if (window || p == 0) meet();
             const int past = 2 * i - (L - 1);  // < 0 before the middle, 0 at an odd line's middle
             Bytes<K> theirs = other[p];
             if (past >= 0 && past < X) theirs = ring[dir ^ 1][row][(i - past) & (RING - 1)][lane];
@@ -670,7 +695,7 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
 #pragma unroll
             for (int k = 0; k < (K + 3) / 4; k++) out.w[k] = mine.w[k] + (past < 0 ? 0u : theirs.w[k]);
             // At the middle only the forward wave stores (the sum of both). The store goes
-            // before this step's loads: the count above relies on that order.
+            // before this step's loads: the count in meet() relies on that order.
             vstore<K>(paired, i < L && !(past == 0 && dir == 1) ? at : MASKED, out);
             __builtin_amdgcn_sched_barrier(0);
             // Step i + PREFETCH: its costs, and the other's bytes when its store has completed.
@@ -680,7 +705,49 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
             at += step;
             ahead += step;
         }
-    }
+    };
+    // Phase A: every line of the block goes on and is before its middle.
+    auto before_window = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int p = 0; p < PREFETCH; p++) {
+            __builtin_amdgcn_sched_barrier(0);
+            s.step(cost[p], p1x2, p2, lane);
+            vstore<K>(paired, at, s.bytes());
+            __builtin_amdgcn_sched_barrier(0);
+            cost[p] = vload<K>(costs, ahead);
+            at += step;
+            ahead += step;
+        }
+    };
+    // Phase C: every line of the block goes on, with the other's bytes in memory.
+    auto after_window = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int p = 0; p < PREFETCH; p++) {
+            __builtin_amdgcn_sched_barrier(0);
+            s.step(cost[p], p1x2, p2, lane);
+            const Bytes<K> mine = s.bytes();
+            if (p == 0) meet();
+            Bytes<K> out;
+#pragma unroll
+            for (int k = 0; k < (K + 3) / 4; k++) out.w[k] = mine.w[k] + other[p].w[k];
+            vstore<K>(paired, at, out);
+            __builtin_amdgcn_sched_barrier(0);
+            cost[p] = vload<K>(costs, ahead);
+            other[p] = vload<K>(paired, ahead);
+            at += step;
+            ahead += step;
+        }
+    };
+    // Block-uniform phase bounds, multiples of PREFETCH: A is [0, a_end), C is
+    // [c_start, c_end) (empty when c_end <= c_start).
+    const int a_end = max(0, (window_lo + 1) >> 1) & ~(PREFETCH - 1);
+    const int c_start = ((window_hi >> 1) + PREFETCH) & ~(PREFETCH - 1);
+    const int c_end = (mid_lo + 1) & ~(PREFETCH - 1);
+    int i0 = 0;
+    for (; i0 < a_end; i0 += PREFETCH) before_window();
+    for (; i0 < c_start && i0 < longest; i0 += PREFETCH) general(i0);
+    for (; i0 < c_end; i0 += PREFETCH) after_window();
+    for (; i0 < longest; i0 += PREFETCH) general(i0);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -745,7 +812,7 @@ __global__ __launch_bounds__(128 * G) void path_sliced_kernel(PathArgs a) {
     const uint32_t last = first + (uint32_t)max(L - 1, 0) * stride;
     const uint32_t volume = (uint32_t)W * H * D;
     const Rsrc costs = buffer_of(pd.b.costs, volume);
-    const Rsrc paired = buffer_of(pd.b.paired[o], volume);
+    const Rsrc paired = buffer_of(orientation_volume(pd.b, o), volume);
     const uint32_t p1x2 = (uint32_t)pd.p1 * 0x10001u;
     const uint32_t p2 = (uint32_t)pd.p2;
     const uint32_t start = dir == 0 ? first : last;
