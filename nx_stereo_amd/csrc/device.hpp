@@ -17,6 +17,11 @@ __device__ __forceinline__ uint32_t pk_min(uint32_t a, uint32_t b) {
     return as_u(__builtin_elementwise_min(as_v(a), as_v(b)));
 }
 __device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) { return as_u(as_v(a) + as_v(b)); }
+// min(lo, hi) in both halves (v_pk_min_u16 with its second operand's halves swapped).
+__device__ __forceinline__ uint32_t pk_min_halves(uint32_t a) {
+    const u16x2 v = as_v(a);
+    return as_u(__builtin_elementwise_min(v, __builtin_shufflevector(v, v, 1, 0)));
+}
 __device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) { return as_u(as_v(a) - as_v(b)); }
 
 // DPP controls (GFX9): row_shl:1, row_shr:1, row_ror:n. A DPP row is 16 lanes.

@@ -299,33 +299,48 @@ struct LineLanes<64> {
 template <int K, int LANES>
 struct PathState {
     static constexpr int R = K / 2;
+    static constexpr uint32_t SENT = 0x3FFF3FFFu;
     uint32_t v[R];  // disparities (2k, 2k+1) of this lane's K in the low and high halves
-    uint32_t pm;    // the minimum over the line's D values, same in all its lanes
+    uint32_t PM;    // the minimum over the line's D values in both halves, same in all its lanes
+    // The neighbouring lanes' values at this lane's two edges. The lanes at a line's two ends
+    // are never written by the moves, so they keep the sentinel from reset().
+    uint32_t left_in, right_in;
+    // The next step's min(prev[d], min(prev[d-1], prev[d+1]) + p1), which needs no minimum.
+    uint32_t part[R];
 
-    __device__ void reset() {
+    // The path entering a line: prev = 0 everywhere.
+    __device__ void reset(uint32_t p1x2, int lane) {
 #pragma unroll
         for (int k = 0; k < R; k++) v[k] = 0;
-        pm = 0;
+        PM = 0;
+        left_in = right_in = SENT;
+        prepare(p1x2, lane);
     }
 
-    // One step of L(p,d) = min(a[d], p2, min(a[d-1], a[d+1]) + p1) + C(p,d) with
+    // Steps compute L(p,d) = min(a[d], p2, min(a[d-1], a[d+1]) + p1) + C(p,d) with
     // a = prev - min(prev), written unnormalised: min(prev[d], pm + p2,
     // min(prev[d-1], prev[d+1]) + p1) - pm + C. Out-of-range neighbours are a
     // sentinel that only ever loses to the pm + p2 term, as libSGM's p2 - p1
-    // padding does.
-    __device__ __forceinline__ void step(const Bytes<K> &cost, uint32_t p1x2, uint32_t p2,
-                                         int lane) {
+    // padding does. The part without pm is made as soon as prev is known, so that it
+    // overlaps the reduction that makes pm.
+    __device__ __forceinline__ void prepare(uint32_t p1x2, int lane) {
         // The line's first and last lanes have no neighbour past its ends: the sentinel.
-        constexpr uint32_t SENT = 0x3FFF3FFFu;
-        const uint32_t left_in = LineLanes<LANES>::from_left(SENT, v[R - 1], lane);
-        const uint32_t right_in = LineLanes<LANES>::from_right(SENT, v[0], lane);
+        left_in = LineLanes<LANES>::from_left(left_in, v[R - 1], lane);
+        right_in = LineLanes<LANES>::from_right(right_in, v[0], lane);
         uint32_t s[R + 1];
         s[0] = (v[0] << 16) | (left_in >> 16);
 #pragma unroll
         for (int k = 1; k < R; k++) s[k] = (v[k] << 16) | (v[k - 1] >> 16);
         s[R] = (right_in << 16) | (v[R - 1] >> 16);
-        const uint32_t PM = pm * 0x10001u;
-        const uint32_t JUMP = (pm + p2) * 0x10001u;
+#pragma unroll
+        for (int k = 0; k < R; k++) part[k] = pk_min(pk_add(pk_min(s[k], s[k + 1]), p1x2), v[k]);
+    }
+
+    // One step: the pm + p2 term, the normalisation and this pixel's costs; then the next
+    // step's part and this step's minimum over the line.
+    __device__ __forceinline__ void step(const Bytes<K> &cost, uint32_t p1x2, uint32_t p2,
+                                         int lane) {
+        const uint32_t JUMP = pk_add(PM, p2 * 0x10001u);
         uint32_t c[R];
         if constexpr (K == 2) {
             c[0] = lo_pair(cost.w[0]);
@@ -339,16 +354,12 @@ struct PathState {
         uint32_t mn = 0xFFFFFFFFu;
 #pragma unroll
         for (int k = 0; k < R; k++) {
-            uint32_t t = pk_min(s[k], s[k + 1]);
-            t = pk_add(t, p1x2);
-            t = pk_min(t, v[k]);
-            t = pk_min(t, JUMP);
-            t = pk_add(pk_sub(t, PM), c[k]);
-            v[k] = t;
-            mn = pk_min(mn, t);
+            v[k] = pk_add(pk_sub(pk_min(part[k], JUMP), PM), c[k]);
+            mn = pk_min(mn, v[k]);
         }
-        uint32_t m = min(mn & 0xFFFFu, mn >> 16);
-        pm = LineLanes<LANES>::min(m);
+        prepare(p1x2, lane);
+        // (m << 16 | m) orders as m does, so the u32 minimum over lanes is the packed one.
+        PM = LineLanes<LANES>::min(pk_min_halves(mn));
     }
 
     __device__ __forceinline__ Bytes<K> bytes() const {
@@ -453,7 +464,7 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
     if (o == 1 && !a.backward_columns) {
         // Columns with 3 paths: down only, its own path costs.
         PathState<K, LANES> s;
-        s.reset();
+        s.reset(p1x2, lane);
         Bytes<K> ring[PREFETCH];
         // The prologue issues what a step issues (a masked store, then the load), so the
         // loop is entered with the same accesses outstanding as it goes round with.
@@ -495,8 +506,8 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
     // (0 < 2j - (L-1) < PREFETCH) they are younger than that, and come instead from the
     // history ring of each direction's last PREFETCH own bytes.
     PathState<K, LANES> fwd, bwd;
-    fwd.reset();
-    bwd.reset();
+    fwd.reset(p1x2, lane);
+    bwd.reset(p1x2, lane);
     Bytes<K> ring_f[PREFETCH], ring_b[PREFETCH];    // costs, PREFETCH steps ahead
     Bytes<K> other_f[PREFETCH], other_b[PREFETCH];  // the other direction's bytes, ditto
     Bytes<K> hist_f[PREFETCH], hist_b[PREFETCH];    // own bytes of the last PREFETCH steps
@@ -657,7 +668,7 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
     const uint32_t step = dir == 0 ? stride : 0u - stride;
 
     PathState<K, 16> s;
-    s.reset();
+    s.reset(p1x2, lane);
     Bytes<K> cost[PREFETCH], other[PREFETCH];
     Bytes<K> none;
 #pragma unroll

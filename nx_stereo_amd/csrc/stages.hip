@@ -327,14 +327,17 @@ hipError_t launch_temporal(float *disparity, float *previous, float *missed, int
 // over columns, cast to f32. The carried previous disparity takes the current.
 //
 // One launch. A block takes QUALITY_COLS columns; its QUALITY_PHASES thread rows sum every
-// QUALITY_PHASES-th image row, and the phases of a column are added in order. Every term is
-// an f32 value (matched and both are 0 or 1, and a matched v is at least 0.5) whose bits
-// fall within an f64 column sum's 53, so each column's f64 sum is exact in any order and
-// its f32 cast is the reference's. The columns' f32 sums are added by a fixed tree within a
-// block and over the blocks in order by the last block to finish.
+// QUALITY_PHASES-th image row, QUALITY_BATCH rows' loads at a time so that their latencies
+// overlap, and the phases of a column are added in order. Every term is an f32 value
+// (matched and both are 0 or 1, and a matched v is at least 0.5) whose bits fall within an
+// f64 column sum's 53 for up to 4096 rows, so each column's f64 sum is exact in any order
+// and its f32 cast is the reference's. The columns' f32 sums are added by a fixed tree
+// within a block and over the blocks in order by the last block to finish. Few blocks: the
+// finish needs a device-scope fence in each, which writes back the L2 cache.
 
 constexpr int QUALITY_COLS = 64;
 constexpr int QUALITY_PHASES = 16;
+constexpr int QUALITY_BATCH = 8;
 
 __global__ __launch_bounds__(QUALITY_COLS * QUALITY_PHASES) void quality_kernel(
     const float *__restrict__ disparity, float *__restrict__ previous, int W, int H,
@@ -344,18 +347,29 @@ __global__ __launch_bounds__(QUALITY_COLS * QUALITY_PHASES) void quality_kernel(
     const int x = blockIdx.x * QUALITY_COLS + lx;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
     if (x < W) {
-        for (int y = phase; y < H; y += QUALITY_PHASES) {
-            const size_t i = (size_t)y * W + x;
-            const float c = disparity[i], p = previous[i];
-            const float matched = c > 0.0f ? 1.0f : 0.0f;
-            const float both = __fmul_rn(p > 0.0f ? 1.0f : 0.0f, matched);
-            const float value = __fmul_rn(c, matched);
-            s0 += (double)matched;
-            s1 += (double)value;
-            s2 += (double)__fmul_rn(value, value);
-            s3 += (double)both;
-            s4 += (double)__fmul_rn(fabsf(__fsub_rn(c, p)), both);
-            previous[i] = c;
+        for (int y0 = phase; y0 < H; y0 += QUALITY_BATCH * QUALITY_PHASES) {
+            float c[QUALITY_BATCH], p[QUALITY_BATCH];
+#pragma unroll
+            for (int j = 0; j < QUALITY_BATCH; j++) {
+                const int y = y0 + j * QUALITY_PHASES;
+                const size_t i = (size_t)min(y, H - 1) * W + x;
+                c[j] = disparity[i];
+                p[j] = previous[i];
+            }
+#pragma unroll
+            for (int j = 0; j < QUALITY_BATCH; j++) {
+                const int y = y0 + j * QUALITY_PHASES;
+                if (y >= H) break;
+                const float matched = c[j] > 0.0f ? 1.0f : 0.0f;
+                const float both = __fmul_rn(p[j] > 0.0f ? 1.0f : 0.0f, matched);
+                const float value = __fmul_rn(c[j], matched);
+                s0 += (double)matched;
+                s1 += (double)value;
+                s2 += (double)__fmul_rn(value, value);
+                s3 += (double)both;
+                s4 += (double)__fmul_rn(fabsf(__fsub_rn(c[j], p[j])), both);
+                previous[(size_t)y * W + x] = c[j];
+            }
         }
     }
     column[phase][0][lx] = s0;
