@@ -23,6 +23,10 @@ __device__ __forceinline__ uint32_t pk_min_halves(uint32_t a) {
     return as_u(__builtin_elementwise_min(v, __builtin_shufflevector(v, v, 1, 0)));
 }
 __device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) { return as_u(as_v(a) - as_v(b)); }
+__device__ __forceinline__ uint32_t pk_mul(uint32_t a, uint32_t b) { return as_u(as_v(a) * as_v(b)); }
+__device__ __forceinline__ uint32_t pk_max(uint32_t a, uint32_t b) {
+    return as_u(__builtin_elementwise_max(as_v(a), as_v(b)));
+}
 
 // DPP controls (GFX9): row_shl:1, row_shr:1, row_ror:n. A DPP row is 16 lanes.
 constexpr int DPP_ROW_SHL1 = 0x101;

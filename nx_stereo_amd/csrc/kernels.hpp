@@ -29,6 +29,7 @@ void match_carve(MatchBuffers &b, void *base, int width, int height, int dispari
 struct MatchParams {
     int width, height, disparities, paths, p1, p2;
     float uniqueness;
+    bool keep_stages = false;  // also store the census images and the checked map (stage checks)
 };
 
 // One pair of a matching launch: its images, output, workspace and live settings. A launch
@@ -102,7 +103,7 @@ hipError_t launch_temporal(float *disparity, float *previous, float *missed, int
                            hipStream_t stream);
 
 // `partials` holds quality_scratch_bytes(width, height) of scratch, zeroed before the first
-// launch (its finishing counter; each launch leaves it at zero).
+// launch (the column totals; each launch leaves them at zero).
 size_t quality_scratch_bytes(int width, int height);
 hipError_t launch_quality(const float *disparity, float *previous, int width, int height,
                           double *partials, float *sums5, hipStream_t stream);

@@ -6,7 +6,8 @@
 
 namespace nxs {
 
-enum ProfiledKernel { PK_CENSUS, PK_COST, PK_PATH, PK_SELECT, PK_CONSISTENCY, PK_MEDIAN, PK_COUNT };
+// PK_COST: census and costs, one kernel; PK_CHECK_MEDIAN: the left-right check and the median.
+enum ProfiledKernel { PK_COST, PK_PATH, PK_SELECT, PK_CHECK_MEDIAN, PK_COUNT };
 
 // Brackets one launch: begin() before it, end() after it, on the same stream.
 // Both are no-ops while profiling is off.

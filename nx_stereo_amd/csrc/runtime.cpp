@@ -540,8 +540,14 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
         slot.download_bytes = 2 * P + 4 * P + 2 * P + 4 * cloud + 256;
         HIP_TRY(hipHostMalloc((void **)&slot.upload, slot.upload_bytes));
         HIP_TRY(hipHostMalloc((void **)&slot.download, slot.download_bytes));
+        // The events between members only time them: no system-scope fence (each one would
+        // write back and invalidate the caches between the members' kernels). The last one
+        // also tells the host the run is done, downloads included: it keeps the fence.
         slot.events.resize(s->members.size() + 1);
-        for (auto &e : slot.events) HIP_TRY(hipEventCreate(&e));
+        for (size_t k = 0; k < slot.events.size(); k++)
+            HIP_TRY(hipEventCreateWithFlags(&slot.events[k], k + 1 < slot.events.size()
+                                                                  ? hipEventDisableSystemFence
+                                                                  : hipEventDefault));
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     return s;
@@ -1273,6 +1279,7 @@ int nxs_match_host(const uint8_t *left, const uint8_t *right, int width, int hei
     nxs::MatchBuffers b;
     nxs::match_carve(b, dev, width, height, disparities, paths);
     nxs::MatchParams mp{width, height, disparities, paths, p1, p2, uniqueness};
+    mp.keep_stages = true;
     int rc = NXS_OK;
     auto check = [&](hipError_t err) {
         if (err != hipSuccess && rc == NXS_OK) rc = nxs::fail(NXS_EDEVICE, hipGetErrorString(err));

@@ -39,91 +39,103 @@ __device__ __forceinline__ PairDesc pair_of(const PairDesc &single, const PairDe
     return table ? table[index] : single;
 }
 
-// Grid z: the pair times two, plus the image (0 left, 1 right).
-__global__ __launch_bounds__(256) void census_kernel(PairDesc single, const PairDesc *table, int W,
-                                                     int H) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    const int side = blockIdx.z & 1;
-    const PairDesc pd = pair_of(single, table, blockIdx.z >> 1);
-    const uint8_t *img = side ? pd.right : pd.left;
-    uint64_t *out = side ? pd.b.census_right : pd.b.census_left;
-    if (x >= W) return;
-    const int centre = img[(size_t)y * W + x];
-    uint64_t f = 0;
-    int bit = 0;
-#pragma unroll
-    for (int dy = -3; dy <= 3; dy++) {
-        if (dy == 0) continue;
-        const int sy = min(max(y + dy, 0), H - 1);
-        const uint8_t *row = img + (size_t)sy * W;
-#pragma unroll
-        for (int dx = -4; dx <= 4; dx++) {
-            if (dx == 0) continue;
-            const int sx = min(max(x + dx, 0), W - 1);
-            f |= (uint64_t)((int)row[sx] + 1 < centre) << bit;
-            bit++;
-        }
-    }
-    out[(size_t)y * W + x] = f;
-}
-
 // ------------------------------------------------------------------------------------------
-// Costs (reference.rs:41-63): C[y,x,d] = popcount(FL[y,x] ^ FR[y,x-d]), FR = 0 left of
-// the image. A block takes a run of `tile` pixels of one row. The right features they
-// compare with, [x0-D+1, x0+tile), are staged in LDS once. Thread t takes pixel t % tile
-// and the 16 disparities of chunk t / tile, so the lanes of a wave read consecutive LDS
-// words (no bank conflicts); the bytes go through an LDS tile and leave as one contiguous
-// run of tile*D bytes, 16 bytes per thread.
+// Census and costs (reference.rs:20-63; census.rs:18-84), one block per image row. The
+// seven rows around row y of both images come into LDS (rows clamped at the top and
+// bottom), and the block makes row y's 48-bit census of both there: bit k is set when
+// neighbour + 1 < centre, row-major over dy in -3..3 and dx in -4..4 (0 skipped), columns
+// clamped. Then C[y,x,d] = popcount(FL[y,x] ^ FR[y,x-d]), with FR = 0 left of the image
+// (the right census row has D zero features in front). Costs leave `tile` pixels at a
+// time: thread t takes pixel t % tile and the 16 disparities of chunk t / tile, so a
+// wave's LDS reads are of consecutive features (no bank conflicts); the bytes go through
+// an LDS tile and leave as one contiguous run of tile*D bytes, 16 bytes per thread.
 
-constexpr int COST_BLOCK = 256;
+constexpr int COST_BLOCK = 1024;
 constexpr int MAX_D = 256;
 constexpr int COST_PITCH_PAD = 16;  // LDS bytes between pixels of the staged tile, spreading banks
+// The staged tile: 16 cost bytes per thread, and a pad per pixel (at most 256 pixels, D = 64).
+constexpr int COST_STAGED_BYTES = COST_BLOCK * 16 + COST_BLOCK / 4 * COST_PITCH_PAD;
 
-// Grid z: the pair.
-__global__ __launch_bounds__(COST_BLOCK) void cost_kernel(PairDesc single, const PairDesc *table,
-                                                          int W, int D) {
-    __shared__ uint64_t right[COST_BLOCK + MAX_D];
-    // tile*(D + pad) bytes: at most 64 pixels (D = 64) or 4096 bytes of costs plus pads.
-    __shared__ uint4 staged[(COST_BLOCK * 16 + 64 * COST_PITCH_PAD) / 16];
+// LDS bytes of one block: the staged tile, the two census rows (the right one with D zero
+// features in front), then 7 rows of each image.
+__host__ __device__ constexpr size_t census_cost_lds(int W, int D) {
+    return (size_t)COST_STAGED_BYTES + 8 * (size_t)(2 * W + D) + 14 * (size_t)W;
+}
+
+// Grid y: the row; z: the pair. keep_census: also store both census images.
+__global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single,
+                                                                 const PairDesc *table, int W,
+                                                                 int H, int D, int keep_census) {
+    extern __shared__ __align__(16) uint8_t lds[];
     const PairDesc pd = pair_of(single, table, blockIdx.z);
-    const uint64_t *__restrict__ cl = pd.b.census_left;
-    const uint64_t *__restrict__ cr = pd.b.census_right;
+    const int y = blockIdx.y;
+    uint4 *staged = reinterpret_cast<uint4 *>(lds);
+    uint64_t *cl = reinterpret_cast<uint64_t *>(lds + COST_STAGED_BYTES);  // [W]
+    uint64_t *cr = cl + W;                                                  // [D zeros, W]
+    uint8_t *rows = reinterpret_cast<uint8_t *>(cr + D + W);                // [2][7][W]
+    for (int side = 0; side < 2; side++) {
+        const uint8_t *img = side ? pd.right : pd.left;
+        for (int r = 0; r < 7; r++) {
+            const uint8_t *src = img + (size_t)min(max(y + r - 3, 0), H - 1) * W;
+            uint8_t *dst = rows + (side * 7 + r) * W;
+            for (int x = threadIdx.x; x < W; x += COST_BLOCK) dst[x] = src[x];
+        }
+    }
+    for (int i = threadIdx.x; i < D; i += COST_BLOCK) cr[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2 * W; i += COST_BLOCK) {
+        const int side = i >= W ? 1 : 0, x = i - side * W;
+        const uint8_t *img = rows + side * 7 * W;
+        const int centre = img[3 * W + x];
+        uint64_t f = 0;
+        int bit = 0;
+#pragma unroll
+        for (int dy = -3; dy <= 3; dy++) {
+            if (dy == 0) continue;
+            const uint8_t *row = img + (3 + dy) * W;
+#pragma unroll
+            for (int dx = -4; dx <= 4; dx++) {
+                if (dx == 0) continue;
+                const int sx = min(max(x + dx, 0), W - 1);
+                f |= (uint64_t)((int)row[sx] + 1 < centre) << bit;
+                bit++;
+            }
+        }
+        if (side) cr[D + x] = f;
+        else cl[x] = f;
+        if (keep_census) (side ? pd.b.census_right : pd.b.census_left)[(size_t)y * W + x] = f;
+    }
+    __syncthreads();
     uint8_t *__restrict__ costs = pd.b.costs;
     const int chunks = D >> 4;
     const int tile = COST_BLOCK / chunks;
     const int pitch = D + COST_PITCH_PAD;  // LDS bytes per pixel
-    const int y = blockIdx.y;
-    const int x0 = blockIdx.x * tile;
-    const int pixels = min(tile, W - x0);
-    const uint64_t *row = cr + (size_t)y * W;
-    for (int i = threadIdx.x; i < tile + D - 1; i += COST_BLOCK) {
-        const int x = x0 - D + 1 + i;
-        right[i] = (x >= 0 && x < W) ? row[x] : 0;
-    }
-    __syncthreads();
     const int p = threadIdx.x % tile, chunk = threadIdx.x / tile;
-    if (chunk < chunks && p < pixels) {
-        const uint64_t l = cl[(size_t)y * W + x0 + p];
-        // FR[x - d] for x = x0 + p sits at right[p + D - 1 - d].
-        const uint64_t *r = right + p + D - 1 - chunk * 16;
-        uint32_t words[4];
+    for (int x0 = 0; x0 < W; x0 += tile) {
+        const int pixels = min(tile, W - x0);
+        if (chunk < chunks && p < pixels) {
+            const uint64_t l = cl[x0 + p];
+            // FR[x - d] for x = x0 + p sits at cr[D + x - d].
+            const uint64_t *r = cr + D + x0 + p - chunk * 16;
+            uint32_t words[4];
 #pragma unroll
-        for (int w = 0; w < 4; w++) {
-            uint32_t word = 0;
+            for (int w = 0; w < 4; w++) {
+                uint32_t word = 0;
 #pragma unroll
-            for (int j = 0; j < 4; j++)
-                word |= (uint32_t)__builtin_popcountll(l ^ r[-(w * 4 + j)]) << (8 * j);
-            words[w] = word;
+                for (int j = 0; j < 4; j++)
+                    word |= (uint32_t)__builtin_popcountll(l ^ r[-(w * 4 + j)]) << (8 * j);
+                words[w] = word;
+            }
+            staged[(p * pitch + chunk * 16) / 16] = make_uint4(words[0], words[1], words[2], words[3]);
         }
-        staged[(p * pitch + chunk * 16) / 16] = make_uint4(words[0], words[1], words[2], words[3]);
-    }
-    __syncthreads();
-    // The run of pixels*D bytes at (y*W + x0)*D, 16 bytes per thread.
-    uint4 *out = reinterpret_cast<uint4 *>(costs + ((size_t)y * W + x0) * D);
-    for (int q = threadIdx.x; q < pixels * chunks; q += COST_BLOCK) {
-        const int pixel = q / chunks, c = q - pixel * chunks;
-        out[q] = staged[(pixel * pitch + c * 16) / 16];
+        __syncthreads();
+        // The run of pixels*D bytes at (y*W + x0)*D, 16 bytes per thread.
+        uint4 *out = reinterpret_cast<uint4 *>(costs + ((size_t)y * W + x0) * D);
+        for (int q = threadIdx.x; q < pixels * chunks; q += COST_BLOCK) {
+            const int pixel = q / chunks, c = q - pixel * chunks;
+            out[q] = staged[(pixel * pitch + c * 16) / 16];
+        }
+        __syncthreads();
     }
 }
 
@@ -933,43 +945,46 @@ __global__ __launch_bounds__(128 * G) void path_sliced_kernel(PathArgs a) {
 
 // ------------------------------------------------------------------------------------------
 // Selection (reference.rs:148-186; nx_stereo.ex:608-641), 16 lanes per pixel.
+//
+// V[d] = S[d] + 0x8000 for d > x. The winner is the lowest d of the least V; d = 0 is never
+// past the left edge and S <= 1016, so the winner is never a penalised d, and (V << 4 | j)
+// for the lane's j-th disparity orders the unpenalised ones in 14 bits (penalised ones take
+// 0xFFFF). Sums, keys and the second-best search run on u16 pairs. The second best is the
+// least V outside [below, above]: those entries are lifted to 0xFFFD..0xFFFF, above every
+// V (at most 0x83F8). The pixel's first lane reads the winner's two neighbours back from
+// the four volumes.
 
-// Grid y: the pair.
 template <int K>
 __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const PairDesc *table,
-                                                     int orientations, int W, int pixels) {
-    constexpr int D = 16 * K;
-    const PairDesc pd = pair_of(single, table, blockIdx.y);
-    const uint8_t *__restrict__ p0 = pd.b.paired[0];
-    const uint8_t *__restrict__ p1 = pd.b.paired[1];
-    const uint8_t *__restrict__ p2 = pd.b.paired[2];
-    const uint8_t *__restrict__ p3 = pd.b.paired[3];
+                                                     int orientations, int W) {
+    constexpr int D = 16 * K, R = K / 2;
+    const PairDesc pd = pair_of(single, table, blockIdx.z);
+    const uint32_t volume = (uint32_t)W * gridDim.y * D;
+    const Rsrc vol[4] = {buffer_of(pd.b.paired[0], volume), buffer_of(pd.b.paired[1], volume),
+                         buffer_of(pd.b.paired[2], volume), buffer_of(pd.b.paired[3], volume)};
     const float uniqueness = pd.uniqueness;
     float *__restrict__ selected = pd.b.selected;
     uint32_t *__restrict__ keys = pd.b.keys;
     uint16_t *__restrict__ sums_out = pd.sums;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane16 = threadIdx.x & 15;
-    const long pix_raw = t >> 4;
-    const bool valid = pix_raw < pixels;
-    const long pix = valid ? pix_raw : pixels - 1;
-    const int x = (int)(pix % W);
-    const size_t at = (size_t)pix * D + (size_t)lane16 * K;
+    const int x = blockIdx.x * 16 + (threadIdx.x >> 4), y = blockIdx.y;
+    const bool valid = x < W;
+    const uint32_t pix = (uint32_t)y * W + (valid ? x : W - 1);
+    const uint32_t at = pix * D + (uint32_t)lane16 * K;  // the volume is under 2^31 bytes
+    const int d0 = lane16 * K;
 
-    uint32_t sum[K / 2];
+    uint32_t sum[R];
     {
-        Bytes<K> b;
-        b.load(p0 + at);
+        Bytes<K> b = vload<K>(vol[0], at);
 #pragma unroll
         for (int k = 0; k < K / 4; k++) {
             sum[2 * k] = lo_pair(b.w[k]);
             sum[2 * k + 1] = hi_pair(b.w[k]);
         }
-        const uint8_t *more[3] = {p1, p2, p3};
 #pragma unroll
-        for (int m = 0; m < 3; m++) {
-            if (m + 1 >= orientations) break;
-            b.load(more[m] + at);
+        for (int m = 1; m < 4; m++) {
+            if (m >= orientations) break;
+            b = vload<K>(vol[m], at);
 #pragma unroll
             for (int k = 0; k < K / 4; k++) {
                 sum[2 * k] = pk_add(sum[2 * k], lo_pair(b.w[k]));
@@ -980,43 +995,67 @@ __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const Pair
     if (sums_out && valid) {
         uint32_t *out = reinterpret_cast<uint32_t *>(sums_out + at);
 #pragma unroll
-        for (int k = 0; k < K / 2; k++) out[k] = sum[k];
+        for (int k = 0; k < R; k++) out[k] = sum[k];
     }
 
-    // V[d] = S[d] + 0x8000 past the left edge; keys (V << 8) | d order by cost, then d.
-    uint32_t value[K];
-    uint32_t best = 0xFFFFFFFFu;
+    uint32_t value[R], key[R];
+    // (j, j + 1) of register r, as u16 pairs.
+    auto pair_index = [](int r) { return (uint32_t)(2 * r) | (uint32_t)(2 * r + 1) << 16; };
+    if (__any(x < D - 1)) {  // wave-uniform: a pixel of this wave has disparities past x
+        const uint32_t X1 = (uint32_t)(x + 1) * 0x10001u;
+        const uint32_t D0 = (uint32_t)d0 * 0x10001u;
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        const int d = lane16 * K + j;
-        const uint32_t s = (j & 1) ? (sum[j >> 1] >> 16) : (sum[j >> 1] & 0xFFFFu);
-        value[j] = s + (d > x ? 0x8000u : 0u);
-        best = min(best, (value[j] << 8) | (uint32_t)d);
+        for (int r = 0; r < R; r++) {
+            // d - (x + 1) wraps past 0xF000 for d <= x and stays under 0x100 for d > x.
+            const uint32_t q = pk_sub(pk_add(D0, pair_index(r)), X1);
+            const uint32_t past = pk_sub(pk_min(q, 0x01000100u), 0x01000100u);  // 0xFF01.. or 0
+            value[r] = sum[r] | (past & 0x80008000u);
+            key[r] = pk_max(pk_add(pk_mul(sum[r], 0x00100010u), pair_index(r)), past);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            value[r] = sum[r];
+            key[r] = pk_add(pk_mul(sum[r], 0x00100010u), pair_index(r));
+        }
     }
-    best = row_min_u32(best);
+    uint32_t kmin = key[0];
+#pragma unroll
+    for (int r = 1; r < R; r++) kmin = pk_min(kmin, key[r]);
+    const uint32_t k16 = pk_min_halves(kmin) & 0xFFFFu;
+    const uint32_t best = row_min_u32((k16 >> 4) << 8 | (uint32_t)(d0 + (int)(k16 & 15u)));
     const uint32_t minimum = best >> 8;
     const int winner = (int)(best & 0xFFu);
     const int below = max(winner - 1, 0), above = min(winner + 1, D - 1);
-    uint32_t second = 0xFFFFFFFFu, neighbours = 0xFFFFFFFFu;
+
+    // Entries with below <= d0 + j <= above: (j - (below - d0)) <= span as u16.
+    const uint32_t T = (uint32_t)(below - d0) & 0xFFFFu, S1 = (uint32_t)(above - below + 1);
+    const uint32_t T2 = T * 0x10001u, S2 = S1 * 0x10001u;
+    uint32_t smin = 0xFFFFFFFFu;
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        const int d = lane16 * K + j;
-        if (d < below || d > above) second = min(second, value[j]);
-        if (d == below) neighbours = (neighbours & 0xFFFF0000u) | value[j];
-        if (d == above) neighbours = (neighbours & 0x0000FFFFu) | (value[j] << 16);
+    for (int r = 0; r < R; r++) {
+        const uint32_t lifted = pk_sub(pk_min(pk_sub(pair_index(r), T2), S2), S2);
+        smin = pk_min(smin, pk_max(value[r], lifted));
     }
-    second = row_min_u32(second);
-    neighbours = row_min_pk(neighbours);
+    const uint32_t second = row_min_u32(pk_min_halves(smin)) & 0xFFFFu;
     if (lane16 != 0 || !valid) return;
 
-    const float lower = (float)(neighbours & 0xFFFFu);
-    const float upper = (float)(neighbours >> 16);
+    uint32_t lower = x < below ? 0x8000u : 0u, upper = x < above ? 0x8000u : 0u;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        if (m >= orientations) break;
+        lower += __builtin_amdgcn_raw_buffer_load_b8(vol[m], pix * D + below, 0, 0);
+        upper += __builtin_amdgcn_raw_buffer_load_b8(vol[m], pix * D + above, 0, 0);
+    }
+    const float lower_f = (float)lower;
+    const float upper_f = (float)upper;
     const float minimum_f32 = (float)minimum;
     // An empty "second" set would be i32::MAX in the reference; D >= 64 never leaves it empty.
     const bool accepted = (float)second > __fmul_rn(minimum_f32, uniqueness);
-    const float denominator = __fsub_rn(__fadd_rn(lower, upper), __fmul_rn(2.0f, minimum_f32));
+    const float denominator = __fsub_rn(__fadd_rn(lower_f, upper_f), __fmul_rn(2.0f, minimum_f32));
     const float correction = fminf(
-        fmaxf(__fdiv_rn(__fsub_rn(lower, upper), __fmul_rn(2.0f, fmaxf(denominator, 1.0f))), -0.5f),
+        fmaxf(__fdiv_rn(__fsub_rn(lower_f, upper_f), __fmul_rn(2.0f, fmaxf(denominator, 1.0f))),
+              -0.5f),
         0.5f);
     const bool refine = denominator > 0.0f && winner > 0 && winner < min(D - 1, x);
     const float disparity = __fadd_rn((float)winner, refine ? correction : 0.0f);
@@ -1025,93 +1064,106 @@ __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const Pair
 }
 
 // ------------------------------------------------------------------------------------------
-// Left-right check (reference.rs:188-235; check.rs:7-47), one block per row: each
-// right pixel keeps the cheapest left match projected onto it, ties to the larger
-// disparity, via an LDS atomic minimum of (cost << 16) | (65535 - d).
+// Left-right check (reference.rs:188-235; check.rs:7-47), per row: each right pixel keeps
+// the cheapest left match projected onto it, ties to the larger disparity, via an LDS
+// atomic minimum of (cost << 16) | (65535 - d) into the row's map.
 
 constexpr int MAX_WIDTH = 4096;
 
-// Grid y: the pair.
-__global__ __launch_bounds__(256) void consistency_kernel(PairDesc single, const PairDesc *table,
-                                                          int W) {
-    __shared__ uint32_t right[MAX_WIDTH];
-    const PairDesc pd = pair_of(single, table, blockIdx.y);
-    const float *__restrict__ selected = pd.b.selected;
-    const uint32_t *__restrict__ keys = pd.b.keys;
-    float *__restrict__ checked = pd.b.checked;
-    const int y = blockIdx.x;
-    const float *sel = selected + (size_t)y * W;
-    const uint32_t *row = keys + (size_t)y * W;
-    for (int x = threadIdx.x; x < W; x += blockDim.x) right[x] = INVALID_KEY;
-    __syncthreads();
-    for (int x = threadIdx.x; x < W; x += blockDim.x) {
-        const uint32_t key = row[x];
-        if (key != INVALID_KEY) {
-            const uint32_t w = key & 0xFFFFu;
-            atomicMin(&right[x - (int)w], (key & 0xFFFF0000u) | (65535u - w));
-        }
-    }
-    __syncthreads();
-    for (int x = threadIdx.x; x < W; x += blockDim.x) {
-        const float value = sel[x];
-        float out = -1.0f;
-        if (value >= 0.0f) {
-            const int lo = (int)floorf(value), hi = (int)ceilf(value);
-            bool rejected = true;
+// The checked value of pixel x of a row: `sel` is the row's selected disparities and
+// `right` its map, already made.
+__device__ __forceinline__ float checked_value(const float *sel, const uint32_t *right, int x,
+                                               int W) {
+    const float value = sel[x];
+    if (value < 0.0f) return -1.0f;
+    const int lo = (int)floorf(value), hi = (int)ceilf(value);
+    bool rejected = true;
 #pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const int d = k ? hi : lo;
-                const int rx = x - d;
-                bool dis = false;
-                if (rx >= 0 && rx < W) {
-                    const uint32_t found = right[rx];
-                    dis = found != INVALID_KEY && abs(65535 - (int)(found & 0xFFFFu) - d) > 1;
-                }
-                rejected = rejected && dis;
-            }
-            if (!rejected) out = value;
+    for (int k = 0; k < 2; k++) {
+        const int d = k ? hi : lo;
+        const int rx = x - d;
+        bool dis = false;
+        if (rx >= 0 && rx < W) {
+            const uint32_t found = right[rx];
+            dis = found != INVALID_KEY && abs(65535 - (int)(found & 0xFFFFu) - d) > 1;
         }
-        checked[(size_t)y * W + x] = out;
+        rejected = rejected && dis;
     }
+    return rejected ? -1.0f : value;
 }
 
 // ------------------------------------------------------------------------------------------
 // 3x3 median with replicated borders (reference.rs:237-254; filters.rs:10-30 network),
 // on i32 bit patterns, which order -1.0 and non-negative values as f32 does; then the
 // Disparity stage's d > 0 ? d : -1 (stage/disparity.ex:162).
+//
+// One block per output row y: it checks rows y-1, y and y+1 (clamped) in LDS, each with
+// its own map, then takes the median of row y from them. The checked map is stored too
+// when asked for (stage-by-stage checks).
 
-// Grid z: the pair.
-__global__ __launch_bounds__(256) void median_kernel(PairDesc single, const PairDesc *table, int W,
-                                                     int H) {
-    const PairDesc pd = pair_of(single, table, blockIdx.z);
-    const float *__restrict__ in = pd.b.checked;
-    float *__restrict__ out = pd.disparity;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= W) return;
-    int v[9];
-    int n = 0;
+__host__ __device__ constexpr size_t check_median_lds(int W) { return 6 * 4 * (size_t)W; }
+
+// Grid x: the row; y: the pair.
+constexpr int CHECK_BLOCK = 1024;
+
+__global__ __launch_bounds__(CHECK_BLOCK) void check_median_kernel(PairDesc single,
+                                                                   const PairDesc *table, int W,
+                                                                   int H, int keep_checked) {
+    extern __shared__ __align__(16) uint32_t check_lds[];
+    uint32_t *right = check_lds;                                    // [3][W]
+    float *checked = reinterpret_cast<float *>(check_lds + 3 * W);  // [3][W]
+    const PairDesc pd = pair_of(single, table, blockIdx.y);
+    const int y = blockIdx.x;
+    int rows[3];
 #pragma unroll
-    for (int dy = -1; dy <= 1; dy++) {
-        const int yy = min(max(y + dy, 0), H - 1);
+    for (int r = 0; r < 3; r++) rows[r] = min(max(y - 1 + r, 0), H - 1);
+    for (int i = threadIdx.x; i < 3 * W; i += blockDim.x) right[i] = INVALID_KEY;
+    __syncthreads();
 #pragma unroll
-        for (int dx = -1; dx <= 1; dx++) {
-            const int xx = min(max(x + dx, 0), W - 1);
-            v[n++] = __float_as_int(in[(size_t)yy * W + xx]);
+    for (int r = 0; r < 3; r++) {
+        const uint32_t *row = pd.b.keys + (size_t)rows[r] * W;
+        for (int x = threadIdx.x; x < W; x += blockDim.x) {
+            const uint32_t key = row[x];
+            if (key != INVALID_KEY) {
+                const uint32_t w = key & 0xFFFFu;
+                atomicMin(&right[r * W + x - (int)w], (key & 0xFFFF0000u) | (65535u - w));
+            }
         }
     }
-#define CX(a, b)                    \
-    {                               \
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const float *sel = pd.b.selected + (size_t)rows[r] * W;
+        for (int x = threadIdx.x; x < W; x += blockDim.x) {
+            const float c = checked_value(sel, right + r * W, x, W);
+            checked[r * W + x] = c;
+            if (r == 1 && keep_checked) pd.b.checked[(size_t)y * W + x] = c;
+        }
+    }
+    __syncthreads();
+    float *__restrict__ out = pd.disparity + (size_t)y * W;
+    for (int x = threadIdx.x; x < W; x += blockDim.x) {
+        int v[9];
+        int n = 0;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++)
+                v[n++] = __float_as_int(checked[r * W + min(max(x + dx, 0), W - 1)]);
+        }
+#define CX(a, b)                        \
+    {                                   \
         const int lo = min(v[a], v[b]); \
         const int hi = max(v[a], v[b]); \
-        v[a] = lo;                  \
-        v[b] = hi;                  \
+        v[a] = lo;                      \
+        v[b] = hi;                      \
     }
-    CX(1, 2) CX(4, 5) CX(7, 8) CX(0, 1) CX(3, 4) CX(6, 7) CX(1, 2) CX(4, 5) CX(7, 8) CX(0, 3)
-    CX(5, 8) CX(4, 7) CX(3, 6) CX(1, 4) CX(2, 5) CX(4, 7) CX(4, 2) CX(6, 4) CX(4, 2)
+        CX(1, 2) CX(4, 5) CX(7, 8) CX(0, 1) CX(3, 4) CX(6, 7) CX(1, 2) CX(4, 5) CX(7, 8) CX(0, 3)
+        CX(5, 8) CX(4, 7) CX(3, 6) CX(1, 4) CX(2, 5) CX(4, 7) CX(4, 2) CX(6, 4) CX(4, 2)
 #undef CX
-    const float m = __int_as_float(v[4]);
-    out[(size_t)y * W + x] = m > 0.0f ? m : -1.0f;
+        const float m = __int_as_float(v[4]);
+        out[x] = m > 0.0f ? m : -1.0f;
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1261,10 +1313,9 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
         launch(0, n - 1);
     }
     profile_end(slot, stream);
-    const long threads = (long)W * H * 16;
     slot = profile_begin(PK_SELECT, stream);
-    hipLaunchKernelGGL(select_kernel<K>, dim3((unsigned)((threads + 255) / 256), count), dim3(256), 0,
-                       stream, single, table, n, W, W * H);
+    hipLaunchKernelGGL(select_kernel<K>, dim3((W + 15) / 16, H, count), dim3(256), 0, stream, single,
+                       table, n, W);
     profile_end(slot, stream);
     return hipGetLastError();
 }
@@ -1273,14 +1324,14 @@ static hipError_t launch_all(const MatchParams &p, const PairDesc &single, const
                              int count, const Schedule &sch, hipStream_t stream) {
     const int W = p.width, H = p.height, D = p.disparities;
     if (W > MAX_WIDTH || W < 1 || H < 1 || count < 1) return hipErrorInvalidValue;
-    int slot = profile_begin(PK_CENSUS, stream);
-    hipLaunchKernelGGL(census_kernel, dim3((W + 255) / 256, H, 2 * count), dim3(256), 0, stream,
-                       single, table, W, H);
-    profile_end(slot, stream);
-    const int tile = COST_BLOCK / (D / 16);
-    slot = profile_begin(PK_COST, stream);
-    hipLaunchKernelGGL(cost_kernel, dim3((W + tile - 1) / tile, H, count), dim3(COST_BLOCK), 0,
-                       stream, single, table, W, D);
+    // Above 64 KB of LDS a kernel must say so first (up to the CU's 160 KB).
+    static const hipError_t lds_ok = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(census_cost_kernel),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)census_cost_lds(MAX_WIDTH, MAX_D));
+    if (lds_ok != hipSuccess) return lds_ok;
+    int slot = profile_begin(PK_COST, stream);
+    hipLaunchKernelGGL(census_cost_kernel, dim3(1, H, count), dim3(COST_BLOCK),
+                       census_cost_lds(W, D), stream, single, table, W, H, D, p.keep_stages ? 1 : 0);
     profile_end(slot, stream);
     hipError_t e;
     switch (D) {
@@ -1291,12 +1342,13 @@ static hipError_t launch_all(const MatchParams &p, const PairDesc &single, const
         default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
-    slot = profile_begin(PK_CONSISTENCY, stream);
-    hipLaunchKernelGGL(consistency_kernel, dim3(H, count), dim3(256), 0, stream, single, table, W);
-    profile_end(slot, stream);
-    slot = profile_begin(PK_MEDIAN, stream);
-    hipLaunchKernelGGL(median_kernel, dim3((W + 255) / 256, H, count), dim3(256), 0, stream, single,
-                       table, W, H);
+    static const hipError_t check_lds_ok = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(check_median_kernel),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)check_median_lds(MAX_WIDTH));
+    if (check_lds_ok != hipSuccess) return check_lds_ok;
+    slot = profile_begin(PK_CHECK_MEDIAN, stream);
+    hipLaunchKernelGGL(check_median_kernel, dim3(H, count), dim3(CHECK_BLOCK), check_median_lds(W), stream,
+                       single, table, W, H, p.keep_stages ? 1 : 0);
     profile_end(slot, stream);
     return hipGetLastError();
 }

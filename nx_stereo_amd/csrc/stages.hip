@@ -326,50 +326,54 @@ hipError_t launch_temporal(float *disparity, float *previous, float *missed, int
 // of [matched, v, v^2, both, |c-p| * both]; each column's sum cast to f32; then an f64 sum
 // over columns, cast to f32. The carried previous disparity takes the current.
 //
-// One launch. A block takes QUALITY_COLS columns; its QUALITY_PHASES thread rows sum every
-// QUALITY_PHASES-th image row, QUALITY_BATCH rows' loads at a time so that their latencies
-// overlap, and the phases of a column are added in order. Every term is an f32 value
-// (matched and both are 0 or 1, and a matched v is at least 0.5) whose bits fall within an
-// f64 column sum's 53 for up to 4096 rows, so each column's f64 sum is exact in any order
-// and its f32 cast is the reference's. The columns' f32 sums are added by a fixed tree
-// within a block and over the blocks in order by the last block to finish. Few blocks: the
-// finish needs a device-scope fence in each, which writes back the L2 cache.
+// Two launches, with no fence between blocks. Every term is an f32 value (matched and both
+// are 0 or 1, and a matched v is at least 0.5) whose bits fall within an f64 column sum's 53
+// for up to 4096 rows, so each column's f64 sum is exact in any order and its f32 cast is
+// the reference's (past 4096 rows a sum may round, and its value then depends on the
+// chunking, the same from run to run).
+//  1. A grid of QUALITY_COLS columns by QUALITY_ROWS rows per block: its QUALITY_PHASES
+//     thread rows sum every QUALITY_PHASES-th row of the chunk, and the block adds its
+//     columns' chunk sums into per-column f64 totals with atomics (exact, so in any order).
+//     It also moves the current disparity into the carried previous.
+//  2. One block: each column's total cast to f32 (and the total set back to zero for the
+//     next launch), then the columns of each group of QUALITY_COLS added by a fixed
+//     pairwise tree, and the groups in order.
 
 constexpr int QUALITY_COLS = 64;
 constexpr int QUALITY_PHASES = 16;
-constexpr int QUALITY_BATCH = 8;
+constexpr int QUALITY_ROWS = 64;
+constexpr int QUALITY_FINISH = 1024;
 
-__global__ __launch_bounds__(QUALITY_COLS * QUALITY_PHASES) void quality_kernel(
+__global__ __launch_bounds__(QUALITY_COLS * QUALITY_PHASES) void quality_chunks_kernel(
     const float *__restrict__ disparity, float *__restrict__ previous, int W, int H,
-    double *__restrict__ partials, unsigned *__restrict__ counter, float *__restrict__ sums5) {
+    double *__restrict__ totals) {
     __shared__ double column[QUALITY_PHASES][5][QUALITY_COLS];
     const int lx = threadIdx.x % QUALITY_COLS, phase = threadIdx.x / QUALITY_COLS;
     const int x = blockIdx.x * QUALITY_COLS + lx;
+    const int y0 = blockIdx.y * QUALITY_ROWS;
+    constexpr int PER_THREAD = QUALITY_ROWS / QUALITY_PHASES;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
     if (x < W) {
-        for (int y0 = phase; y0 < H; y0 += QUALITY_BATCH * QUALITY_PHASES) {
-            float c[QUALITY_BATCH], p[QUALITY_BATCH];
+        float c[PER_THREAD], p[PER_THREAD];
 #pragma unroll
-            for (int j = 0; j < QUALITY_BATCH; j++) {
-                const int y = y0 + j * QUALITY_PHASES;
-                const size_t i = (size_t)min(y, H - 1) * W + x;
-                c[j] = disparity[i];
-                p[j] = previous[i];
-            }
+        for (int j = 0; j < PER_THREAD; j++) {
+            const size_t i = (size_t)min(y0 + phase + j * QUALITY_PHASES, H - 1) * W + x;
+            c[j] = disparity[i];
+            p[j] = previous[i];
+        }
 #pragma unroll
-            for (int j = 0; j < QUALITY_BATCH; j++) {
-                const int y = y0 + j * QUALITY_PHASES;
-                if (y >= H) break;
-                const float matched = c[j] > 0.0f ? 1.0f : 0.0f;
-                const float both = __fmul_rn(p[j] > 0.0f ? 1.0f : 0.0f, matched);
-                const float value = __fmul_rn(c[j], matched);
-                s0 += (double)matched;
-                s1 += (double)value;
-                s2 += (double)__fmul_rn(value, value);
-                s3 += (double)both;
-                s4 += (double)__fmul_rn(fabsf(__fsub_rn(c[j], p[j])), both);
-                previous[(size_t)y * W + x] = c[j];
-            }
+        for (int j = 0; j < PER_THREAD; j++) {
+            const int y = y0 + phase + j * QUALITY_PHASES;
+            if (y >= H) break;
+            const float matched = c[j] > 0.0f ? 1.0f : 0.0f;
+            const float both = __fmul_rn(p[j] > 0.0f ? 1.0f : 0.0f, matched);
+            const float value = __fmul_rn(c[j], matched);
+            s0 += (double)matched;
+            s1 += (double)value;
+            s2 += (double)__fmul_rn(value, value);
+            s3 += (double)both;
+            s4 += (double)__fmul_rn(fabsf(__fsub_rn(c[j], p[j])), both);
+            previous[(size_t)y * W + x] = c[j];
         }
     }
     column[phase][0][lx] = s0;
@@ -378,39 +382,74 @@ __global__ __launch_bounds__(QUALITY_COLS * QUALITY_PHASES) void quality_kernel(
     column[phase][3][lx] = s3;
     column[phase][4][lx] = s4;
     __syncthreads();
-    // Threads 0..5*64-1: plane k = t / 64 of column t % 64; the column's f32 sum, in f64.
-    const int t = threadIdx.x, k = t / QUALITY_COLS;
-    if (k < 5) {
+    const int k = threadIdx.x / QUALITY_COLS;  // plane k of column lx, for the first 5*64 threads
+    if (k < 5 && x < W) {
         double sum = 0;
         for (int q = 0; q < QUALITY_PHASES; q++) sum += column[q][k][lx];
-        column[0][k][lx] = (double)(float)sum;
+        atomicAdd(&totals[(size_t)k * W + x], sum);
     }
-    __syncthreads();
-    for (int half = QUALITY_COLS / 2; half > 0; half >>= 1) {
-        if (k < 5 && lx < half) column[0][k][lx] += column[0][k][lx + half];
+}
+
+constexpr int QUALITY_FINISH_COLUMNS = 4;  // per thread: up to 4096 columns
+
+__global__ __launch_bounds__(QUALITY_FINISH) void quality_finish_kernel(
+    double *__restrict__ totals, int W, float *__restrict__ sums5) {
+    extern __shared__ double columns[];  // up to a whole group past W, at zero
+    const int groups = (W + QUALITY_COLS - 1) / QUALITY_COLS, padded = groups * QUALITY_COLS;
+    // Every total this thread finishes, read (and set back to zero) at once.
+    float mine[5][QUALITY_FINISH_COLUMNS];
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+#pragma unroll
+        for (int j = 0; j < QUALITY_FINISH_COLUMNS; j++) {
+            const int x = threadIdx.x + j * QUALITY_FINISH;
+            mine[k][j] = x < W ? (float)totals[(size_t)k * W + x] : 0.0f;
+        }
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+#pragma unroll
+        for (int j = 0; j < QUALITY_FINISH_COLUMNS; j++) {
+            const int x = threadIdx.x + j * QUALITY_FINISH;
+            if (x < W) totals[(size_t)k * W + x] = 0.0;
+        }
+    for (int k = 0; k < 5; k++) {
+#pragma unroll
+        for (int j = 0; j < QUALITY_FINISH_COLUMNS; j++) {
+            const int x = threadIdx.x + j * QUALITY_FINISH;
+            if (x < padded) columns[x] = (double)mine[k][j];
+        }
         __syncthreads();
-    }
-    if (t < 5) partials[(size_t)blockIdx.x * 5 + t] = column[0][t][0];
-    if (!last_block(counter, gridDim.x)) return;
-    if (t < 5) {
-        double total = 0;
-        for (unsigned b = 0; b < gridDim.x; b++) total += partials[(size_t)b * 5 + t];
-        sums5[t] = (float)total;
+        for (int half = QUALITY_COLS / 2; half > 0; half >>= 1) {
+            for (int i = threadIdx.x; i < groups * half; i += QUALITY_FINISH) {
+                const int g = i / half, lx = i - g * half;
+                columns[g * QUALITY_COLS + lx] += columns[g * QUALITY_COLS + lx + half];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            double total = 0;
+            for (int g = 0; g < groups; g++) total += columns[g * QUALITY_COLS];
+            sums5[k] = (float)total;
+        }
+        __syncthreads();
     }
 }
 
 size_t quality_scratch_bytes(int W, int H) {
     (void)H;
-    const size_t blocks = (size_t)(W + QUALITY_COLS - 1) / QUALITY_COLS;
-    return blocks * 5 * sizeof(double) + sizeof(double);  // the block sums, then the counter
+    return 5 * (size_t)W * sizeof(double);  // the column totals
 }
 
 hipError_t launch_quality(const float *disparity, float *previous, int W, int H, double *partials,
                           float *sums5, hipStream_t stream) {
-    const int blocks = (W + QUALITY_COLS - 1) / QUALITY_COLS;
-    unsigned *counter = reinterpret_cast<unsigned *>(partials + (size_t)blocks * 5);
-    hipLaunchKernelGGL(quality_kernel, dim3(blocks), dim3(QUALITY_COLS * QUALITY_PHASES), 0, stream,
-                       disparity, previous, W, H, partials, counter, sums5);
+    if (W > QUALITY_FINISH * QUALITY_FINISH_COLUMNS) return hipErrorInvalidValue;
+    const int groups = (W + QUALITY_COLS - 1) / QUALITY_COLS;
+    const int n_chunks = (H + QUALITY_ROWS - 1) / QUALITY_ROWS;
+    hipLaunchKernelGGL(quality_chunks_kernel, dim3(groups, n_chunks),
+                       dim3(QUALITY_COLS * QUALITY_PHASES), 0, stream, disparity, previous, W, H,
+                       partials);
+    hipLaunchKernelGGL(quality_finish_kernel, dim3(1), dim3(QUALITY_FINISH),
+                       (size_t)groups * QUALITY_COLS * sizeof(double), stream, partials, W, sums5);
     return hipGetLastError();
 }
 

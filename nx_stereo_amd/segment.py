@@ -112,7 +112,7 @@ class Context:
 
 NxsTensor = _native.NxsTensor
 
-PROFILED_KERNELS = ("census", "cost", "path", "select", "consistency", "median")
+PROFILED_KERNELS = ("census_cost", "path", "select", "check_median")
 
 
 def profile_start(capacity: int = 4096) -> None:

@@ -160,6 +160,26 @@ def test_stateful_pipeline_closed_forms(context):
     assert float(out["quality_sums"][3]) == 0.0
 
 
+def test_quality_of_a_tall_image(context):
+    """A 1200-row image: each column's totals gather 19 chunks of rows, added from blocks
+    anywhere on the device. The sums are the oracle's (each column's sum is exact), and two
+    segments fed the same frames give the same bits."""
+    geometry, _ = rig_geometry(130, 1200)
+    rng = np.random.default_rng(11)
+    frames = [np.where(rng.random((1200, 130)) < 0.3, -1.0,
+                       rng.uniform(0.5, 250.0, (1200, 130))).astype(np.float32) for _ in range(4)]
+    quality = oracle.Quality((1200, 130))
+    got = []
+    for _ in range(2):
+        seg = build(context, [("quality", stages.Quality())], geometry)
+        got.append([run_ok(seg, {"disparity": f})["quality_sums"] for f in frames])
+        seg.release()
+    for g, f in zip(got[0], frames):
+        np.testing.assert_allclose(g, quality.sums(f), rtol=1e-6)
+    for a, b in zip(*got):
+        same_bits(a, b)
+
+
 def test_full_pipeline_728x544x128(context):
     """The benchmark's workload, every output against the oracle's chain."""
     geometry, cal = rig_geometry(728, 544)
