@@ -954,6 +954,11 @@ __global__ __launch_bounds__(128 * G) void path_sliced_kernel(PathArgs a) {
 // V (at most 0x83F8). The pixel's first lane reads the winner's two neighbours back from
 // the four volumes.
 
+// A block takes SELECT_GROUPS groups of 16 pixels of a row, one after another, loading the
+// next group's four volumes while it selects in the current one (so that each wave keeps
+// loads in flight).
+constexpr int SELECT_GROUPS = 4;
+
 template <int K>
 __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const PairDesc *table,
                                                      int orientations, int W) {
@@ -967,100 +972,117 @@ __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const Pair
     uint32_t *__restrict__ keys = pd.b.keys;
     uint16_t *__restrict__ sums_out = pd.sums;
     const int lane16 = threadIdx.x & 15;
-    const int x = blockIdx.x * 16 + (threadIdx.x >> 4), y = blockIdx.y;
-    const bool valid = x < W;
-    const uint32_t pix = (uint32_t)y * W + (valid ? x : W - 1);
-    const uint32_t at = pix * D + (uint32_t)lane16 * K;  // the volume is under 2^31 bytes
+    const int x_first = blockIdx.x * 16 * SELECT_GROUPS + (threadIdx.x >> 4), y = blockIdx.y;
     const int d0 = lane16 * K;
-
-    uint32_t sum[R];
-    {
-        Bytes<K> b = vload<K>(vol[0], at);
+    // This lane's bytes of group g's pixel (the last pixel of the row past its end).
+    auto offset_of = [&](int g) {
+        const uint32_t pix = (uint32_t)y * W + min(x_first + 16 * g, W - 1);
+        return pix * D + (uint32_t)lane16 * K;  // the volume is under 2^31 bytes
+    };
+    Bytes<K> cur[4], next[4];
+    auto load = [&](Bytes<K>(&b)[4], int g) __attribute__((always_inline)) {
+        const uint32_t at = offset_of(g);
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+            if (m < orientations) b[m] = vload<K>(vol[m], at);
+    };
+    load(cur, 0);
+#pragma unroll
+    for (int g = 0; g < SELECT_GROUPS; g++) {
+        if (g + 1 < SELECT_GROUPS) load(next, g + 1);
+        const int x = x_first + 16 * g;
+        const bool valid = x < W;
+        const uint32_t at = offset_of(g), pix = (uint32_t)y * W + min(x, W - 1);
+        uint32_t sum[R];
 #pragma unroll
         for (int k = 0; k < K / 4; k++) {
-            sum[2 * k] = lo_pair(b.w[k]);
-            sum[2 * k + 1] = hi_pair(b.w[k]);
+            sum[2 * k] = lo_pair(cur[0].w[k]);
+            sum[2 * k + 1] = hi_pair(cur[0].w[k]);
         }
 #pragma unroll
         for (int m = 1; m < 4; m++) {
             if (m >= orientations) break;
-            b = vload<K>(vol[m], at);
 #pragma unroll
             for (int k = 0; k < K / 4; k++) {
-                sum[2 * k] = pk_add(sum[2 * k], lo_pair(b.w[k]));
-                sum[2 * k + 1] = pk_add(sum[2 * k + 1], hi_pair(b.w[k]));
+                sum[2 * k] = pk_add(sum[2 * k], lo_pair(cur[m].w[k]));
+                sum[2 * k + 1] = pk_add(sum[2 * k + 1], hi_pair(cur[m].w[k]));
             }
         }
-    }
-    if (sums_out && valid) {
-        uint32_t *out = reinterpret_cast<uint32_t *>(sums_out + at);
+        if (sums_out && valid) {
+            uint32_t *out = reinterpret_cast<uint32_t *>(sums_out + at);
 #pragma unroll
-        for (int k = 0; k < R; k++) out[k] = sum[k];
-    }
+            for (int k = 0; k < R; k++) out[k] = sum[k];
+        }
+        uint32_t value[R], key[R];
+        // (j, j + 1) of register r, as u16 pairs.
+        auto pair_index = [](int r) { return (uint32_t)(2 * r) | (uint32_t)(2 * r + 1) << 16; };
+        if (__any(x < D - 1)) {  // wave-uniform: a pixel of the wave has disparities past x
+            const uint32_t X1 = (uint32_t)(x + 1) * 0x10001u;
+            const uint32_t D0 = (uint32_t)d0 * 0x10001u;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                // d - (x + 1) wraps past 0xF000 for d <= x and stays under 0x100 for d > x.
+                const uint32_t q = pk_sub(pk_add(D0, pair_index(r)), X1);
+                const uint32_t past = pk_sub(pk_min(q, 0x01000100u), 0x01000100u);  // 0xFF.. or 0
+                value[r] = sum[r] | (past & 0x80008000u);
+                key[r] = pk_max(pk_add(pk_mul(sum[r], 0x00100010u), pair_index(r)), past);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                value[r] = sum[r];
+                key[r] = pk_add(pk_mul(sum[r], 0x00100010u), pair_index(r));
+            }
+        }
+        uint32_t kmin = key[0];
+#pragma unroll
+        for (int r = 1; r < R; r++) kmin = pk_min(kmin, key[r]);
+        const uint32_t k16 = pk_min_halves(kmin) & 0xFFFFu;
+        const uint32_t best = row_min_u32((k16 >> 4) << 8 | (uint32_t)(d0 + (int)(k16 & 15u)));
+        const uint32_t minimum = best >> 8;
+        const int winner = (int)(best & 0xFFu);
+        const int below = max(winner - 1, 0), above = min(winner + 1, D - 1);
 
-    uint32_t value[R], key[R];
-    // (j, j + 1) of register r, as u16 pairs.
-    auto pair_index = [](int r) { return (uint32_t)(2 * r) | (uint32_t)(2 * r + 1) << 16; };
-    if (__any(x < D - 1)) {  // wave-uniform: a pixel of this wave has disparities past x
-        const uint32_t X1 = (uint32_t)(x + 1) * 0x10001u;
-        const uint32_t D0 = (uint32_t)d0 * 0x10001u;
+        // Entries with below <= d0 + j <= above: (j - (below - d0)) <= span as u16.
+        const uint32_t T = (uint32_t)(below - d0) & 0xFFFFu;
+        const uint32_t S1 = (uint32_t)(above - below + 1);
+        const uint32_t T2 = T * 0x10001u, S2 = S1 * 0x10001u;
+        uint32_t smin = 0xFFFFFFFFu;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            // d - (x + 1) wraps past 0xF000 for d <= x and stays under 0x100 for d > x.
-            const uint32_t q = pk_sub(pk_add(D0, pair_index(r)), X1);
-            const uint32_t past = pk_sub(pk_min(q, 0x01000100u), 0x01000100u);  // 0xFF01.. or 0
-            value[r] = sum[r] | (past & 0x80008000u);
-            key[r] = pk_max(pk_add(pk_mul(sum[r], 0x00100010u), pair_index(r)), past);
+            const uint32_t lifted = pk_sub(pk_min(pk_sub(pair_index(r), T2), S2), S2);
+            smin = pk_min(smin, pk_max(value[r], lifted));
         }
-    } else {
+        const uint32_t second = row_min_u32(pk_min_halves(smin)) & 0xFFFFu;
+        if (lane16 == 0 && valid) {
+            uint32_t lower = x < below ? 0x8000u : 0u, upper = x < above ? 0x8000u : 0u;
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            value[r] = sum[r];
-            key[r] = pk_add(pk_mul(sum[r], 0x00100010u), pair_index(r));
+            for (int m = 0; m < 4; m++) {
+                if (m >= orientations) break;
+                lower += __builtin_amdgcn_raw_buffer_load_b8(vol[m], pix * D + below, 0, 0);
+                upper += __builtin_amdgcn_raw_buffer_load_b8(vol[m], pix * D + above, 0, 0);
+            }
+            const float lower_f = (float)lower;
+            const float upper_f = (float)upper;
+            const float minimum_f32 = (float)minimum;
+            // An empty "second" set would be i32::MAX in the reference; D >= 64 never
+            // leaves it empty.
+            const bool accepted = (float)second > __fmul_rn(minimum_f32, uniqueness);
+            const float denominator =
+                __fsub_rn(__fadd_rn(lower_f, upper_f), __fmul_rn(2.0f, minimum_f32));
+            const float correction =
+                fminf(fmaxf(__fdiv_rn(__fsub_rn(lower_f, upper_f),
+                                      __fmul_rn(2.0f, fmaxf(denominator, 1.0f))),
+                            -0.5f),
+                      0.5f);
+            const bool refine = denominator > 0.0f && winner > 0 && winner < min(D - 1, x);
+            const float disparity = __fadd_rn((float)winner, refine ? correction : 0.0f);
+            selected[pix] = accepted ? disparity : -1.0f;
+            keys[pix] = accepted ? ((minimum << 16) | (uint32_t)winner) : INVALID_KEY;
         }
-    }
-    uint32_t kmin = key[0];
 #pragma unroll
-    for (int r = 1; r < R; r++) kmin = pk_min(kmin, key[r]);
-    const uint32_t k16 = pk_min_halves(kmin) & 0xFFFFu;
-    const uint32_t best = row_min_u32((k16 >> 4) << 8 | (uint32_t)(d0 + (int)(k16 & 15u)));
-    const uint32_t minimum = best >> 8;
-    const int winner = (int)(best & 0xFFu);
-    const int below = max(winner - 1, 0), above = min(winner + 1, D - 1);
-
-    // Entries with below <= d0 + j <= above: (j - (below - d0)) <= span as u16.
-    const uint32_t T = (uint32_t)(below - d0) & 0xFFFFu, S1 = (uint32_t)(above - below + 1);
-    const uint32_t T2 = T * 0x10001u, S2 = S1 * 0x10001u;
-    uint32_t smin = 0xFFFFFFFFu;
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        const uint32_t lifted = pk_sub(pk_min(pk_sub(pair_index(r), T2), S2), S2);
-        smin = pk_min(smin, pk_max(value[r], lifted));
+        for (int m = 0; m < 4; m++) cur[m] = next[m];
     }
-    const uint32_t second = row_min_u32(pk_min_halves(smin)) & 0xFFFFu;
-    if (lane16 != 0 || !valid) return;
-
-    uint32_t lower = x < below ? 0x8000u : 0u, upper = x < above ? 0x8000u : 0u;
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-        if (m >= orientations) break;
-        lower += __builtin_amdgcn_raw_buffer_load_b8(vol[m], pix * D + below, 0, 0);
-        upper += __builtin_amdgcn_raw_buffer_load_b8(vol[m], pix * D + above, 0, 0);
-    }
-    const float lower_f = (float)lower;
-    const float upper_f = (float)upper;
-    const float minimum_f32 = (float)minimum;
-    // An empty "second" set would be i32::MAX in the reference; D >= 64 never leaves it empty.
-    const bool accepted = (float)second > __fmul_rn(minimum_f32, uniqueness);
-    const float denominator = __fsub_rn(__fadd_rn(lower_f, upper_f), __fmul_rn(2.0f, minimum_f32));
-    const float correction = fminf(
-        fmaxf(__fdiv_rn(__fsub_rn(lower_f, upper_f), __fmul_rn(2.0f, fmaxf(denominator, 1.0f))),
-              -0.5f),
-        0.5f);
-    const bool refine = denominator > 0.0f && winner > 0 && winner < min(D - 1, x);
-    const float disparity = __fadd_rn((float)winner, refine ? correction : 0.0f);
-    selected[pix] = accepted ? disparity : -1.0f;
-    keys[pix] = accepted ? ((minimum << 16) | (uint32_t)winner) : INVALID_KEY;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1314,8 +1336,9 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
     }
     profile_end(slot, stream);
     slot = profile_begin(PK_SELECT, stream);
-    hipLaunchKernelGGL(select_kernel<K>, dim3((W + 15) / 16, H, count), dim3(256), 0, stream, single,
-                       table, n, W);
+    const int select_pixels = 16 * SELECT_GROUPS;
+    hipLaunchKernelGGL(select_kernel<K>, dim3((W + select_pixels - 1) / select_pixels, H, count),
+                       dim3(256), 0, stream, single, table, n, W);
     profile_end(slot, stream);
     return hipGetLastError();
 }
