@@ -108,6 +108,20 @@ size_t quality_scratch_bytes(int width, int height);
 hipError_t launch_quality(const float *disparity, float *previous, int width, int height,
                           double *partials, float *sums5, hipStream_t stream);
 
+// MedianFilter: kernel_size 3 or 5.
+hipError_t launch_median_filter(const float *in, float *out, int width, int height,
+                                int kernel_size, hipStream_t stream);
+
+// Sharpness and RowAlignment sums. `totals` holds the scratch bytes named below, zeroed
+// before the first launch (each launch leaves them at zero).
+size_t sharpness_scratch_bytes(int width);
+hipError_t launch_sharpness(const uint8_t *left, const uint8_t *right, int width, int height,
+                            uint64_t *totals, float *sums6, hipStream_t stream);
+size_t alignment_scratch_bytes(int bands, int reach);
+hipError_t launch_alignment(const uint8_t *left, const uint8_t *right, const float *disparity,
+                            int width, int height, int bands, int reach, int row_step,
+                            uint64_t *totals, float *sums, hipStream_t stream);
+
 hipError_t launch_fill_f32(float *dst, float value, size_t count, hipStream_t stream);
 
 }  // namespace nxs

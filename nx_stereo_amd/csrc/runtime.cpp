@@ -259,13 +259,26 @@ static std::string number_text(double v) {
 // ------------------------------------------------------------------------------------------
 // Stages (segment.rs:244-333): kinds, build settings and the constants they share with Nx.
 
-enum class Kind { Rectification, Disparity, SpeckleFilter, TemporalFilter, Depth, PointCloud, Quality };
+enum class Kind {
+    Rectification,
+    Sharpness,
+    Disparity,
+    MedianFilter,
+    SpeckleFilter,
+    RowAlignment,
+    TemporalFilter,
+    Depth,
+    PointCloud,
+    Quality
+};
 
 struct Member {
     Kind kind;
     std::string name;
     int disparities = 0, paths = 0;
     int radius = 0;
+    int kernel_size = 0;
+    int bands = 0, reach = 0, row_step = 0;
     int decimation = 0;
     float nearest = 0, farthest = 0;
     float max_missed = 0;
@@ -350,6 +363,19 @@ static Member make_member(const Geometry &g, const std::string &kind, const Opti
         m.paths = integer_option(o, kind, "paths", {3, 8});
         check_matching(W, H, m.disparities, m.paths, 8, 64, 1.0f);
         m.kind = Kind::Disparity;
+    } else if (kind == "sharpness") {
+        takes_only(o, kind, {});
+        m.kind = Kind::Sharpness;
+    } else if (kind == "median_filter") {
+        takes_only(o, kind, {"kernel_size"});
+        m.kernel_size = integer_option(o, kind, "kernel_size", {3, 5});
+        m.kind = Kind::MedianFilter;
+    } else if (kind == "row_alignment") {
+        takes_only(o, kind, {"bands", "reach", "row_step"});
+        m.bands = count_option(o, kind, "bands", 1, W);
+        m.reach = count_option(o, kind, "reach", 0, H);
+        m.row_step = count_option(o, kind, "row_step", 1, H);
+        m.kind = Kind::RowAlignment;
     } else if (kind == "speckle_filter") {
         takes_only(o, kind, {"radius"});
         m.radius = count_option(o, kind, "radius", 1, std::min(W, H));
@@ -378,11 +404,14 @@ static Member make_member(const Geometry &g, const std::string &kind, const Opti
 
 static const std::vector<const char *> &member_outputs(Kind k) {
     static const std::vector<const char *> rect = {"left", "right"}, disp = {"disparity"},
-                                           depth = {"depth_mm"},
+                                           sharp = {"sharpness_sums"},
+                                           align = {"alignment_sums"}, depth = {"depth_mm"},
                                            cloud = {"cloud_grid", "cloud_points"},
                                            quality = {"quality_sums"};
     switch (k) {
         case Kind::Rectification: return rect;
+        case Kind::Sharpness: return sharp;
+        case Kind::RowAlignment: return align;
         case Kind::Depth: return depth;
         case Kind::PointCloud: return cloud;
         case Kind::Quality: return quality;
@@ -440,6 +469,8 @@ struct Segment {
     float *disparity[2]{};
     uint16_t *depth_mm = nullptr;
     float *depth_m = nullptr, *cloud = nullptr, *quality_sums = nullptr;
+    uint64_t *sharpness_totals = nullptr, *alignment_totals = nullptr;
+    float *sharpness_sums = nullptr, *alignment_sums = nullptr;
     double *quality_partials = nullptr;
     int32_t *cloud_partials = nullptr, *cloud_kept = nullptr;
     std::vector<float *> carry_a, carry_b;  // per member: temporal previous/missed, quality previous
@@ -484,11 +515,20 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
         }
     }
     (void)max_dec;
+    size_t sharp = 0, align = 0, align_sums = 0;  // scratch bytes; the alignment sums (floats)
+    for (auto &m : s->members) {
+        if (m.kind == Kind::Sharpness) sharp = sharpness_scratch_bytes(g.width);
+        if (m.kind == Kind::RowAlignment) {
+            align = std::max(align, alignment_scratch_bytes(m.bands, m.reach));
+            align_sums = std::max(align_sums, (size_t)m.bands * (2 * m.reach + 2));
+        }
+    }
     int carries = 0;
     for (auto &m : s->members) carries += m.kind == Kind::TemporalFilter ? 2 : (m.kind == Kind::Quality);
     size_t bytes = up256(F) * 2 + up256(P) * 2 + up256(4 * P) * 2 + up256(2 * P) + up256(4 * P) +
                    up256(4 * cloud) + up256(4 * (partials + 1)) +
                    up256(quality_scratch_bytes(g.width, g.height)) + up256(32) +
+                   up256(sharp) + up256(32) + up256(align) + up256(4 * align_sums) +
                    up256(4 * P) * carries + 256;
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&s->arena, bytes));
@@ -517,6 +557,17 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
     HIP_TRY(launch_fill_f32(reinterpret_cast<float *>(s->quality_partials), 0.0f,
                             quality_scratch_bytes(g.width, g.height) / sizeof(float), s->stream));
     s->quality_sums = reinterpret_cast<float *>(carve(4 * 8));
+    // The sharpness and alignment totals start at zero (each launch leaves them at zero).
+    s->sharpness_totals = reinterpret_cast<uint64_t *>(carve(sharp));
+    s->sharpness_sums = reinterpret_cast<float *>(carve(4 * 8));
+    s->alignment_totals = reinterpret_cast<uint64_t *>(carve(align));
+    s->alignment_sums = reinterpret_cast<float *>(carve(4 * align_sums));
+    if (sharp)
+        HIP_TRY(launch_fill_f32(reinterpret_cast<float *>(s->sharpness_totals), 0.0f,
+                                sharp / sizeof(float), s->stream));
+    if (align)
+        HIP_TRY(launch_fill_f32(reinterpret_cast<float *>(s->alignment_totals), 0.0f,
+                                align / sizeof(float), s->stream));
     s->workspace_bytes = match;
     for (auto &m : s->members) {
         float *a = nullptr, *b = nullptr;
@@ -537,7 +588,7 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
     // tensors, each span rounded up to 16 bytes.
     for (auto &slot : s->slots) {
         slot.upload_bytes = 2 * F + 2 * P + 8 * P + 256;
-        slot.download_bytes = 2 * P + 4 * P + 2 * P + 4 * cloud + 256;
+        slot.download_bytes = 2 * P + 4 * P + 2 * P + 4 * cloud + 4 * align_sums + 256;
         HIP_TRY(hipHostMalloc((void **)&slot.upload, slot.upload_bytes));
         HIP_TRY(hipHostMalloc((void **)&slot.download, slot.download_bytes));
         // The events between members only time them: no system-scope fence (each one would
@@ -701,6 +752,31 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
                 HIP_TRY(launch_match(mp, l, r, b, out, nullptr, stream));
                 tensors["disparity"] = Slot{TYPE_F32, image, out};
                 free_disparity ^= 1;
+                break;
+            }
+            case Kind::Sharpness: {
+                auto *l = (const uint8_t *)shaped(tensors, "left", TYPE_U8, image, "u8 image");
+                auto *r = (const uint8_t *)shaped(tensors, "right", TYPE_U8, image, "u8 image");
+                HIP_TRY(launch_sharpness(l, r, W, H, s.sharpness_totals, s.sharpness_sums, stream));
+                tensors["sharpness_sums"] = Slot{TYPE_F32, {6}, s.sharpness_sums};
+                break;
+            }
+            case Kind::MedianFilter: {
+                auto *d = (const float *)shaped(tensors, "disparity", TYPE_F32, image, "f32 map");
+                float *out = s.disparity[0] == d ? s.disparity[1] : s.disparity[0];
+                HIP_TRY(launch_median_filter(d, out, W, H, m.kernel_size, stream));
+                tensors["disparity"] = Slot{TYPE_F32, image, out};
+                free_disparity = s.disparity[0] == out ? 1 : 0;
+                break;
+            }
+            case Kind::RowAlignment: {
+                auto *l = (const uint8_t *)shaped(tensors, "left", TYPE_U8, image, "u8 image");
+                auto *r = (const uint8_t *)shaped(tensors, "right", TYPE_U8, image, "u8 image");
+                auto *d = (const float *)shaped(tensors, "disparity", TYPE_F32, image, "f32 map");
+                HIP_TRY(launch_alignment(l, r, d, W, H, m.bands, m.reach, m.row_step,
+                                         s.alignment_totals, s.alignment_sums, stream));
+                tensors["alignment_sums"] =
+                    Slot{TYPE_F32, {(uint32_t)m.bands, (uint32_t)(2 * m.reach + 2)}, s.alignment_sums};
                 break;
             }
             case Kind::SpeckleFilter: {

@@ -453,6 +453,239 @@ hipError_t launch_quality(const float *disparity, float *previous, int W, int H,
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------------------
+// MedianFilter (filters.rs:134-216; image.ex:479-505): the middle of each pixel's k x k
+// window (3 or 5), borders replicated, in f32 comparisons. Every value but a NaN or a signed
+// zero has one place in the order, so the middle is the reference's (the maps this stage
+// filters hold -1 and positive disparities). 3x3: the classic 19-exchange median network.
+// 5x5: Batcher's odd-even merge sorting network on 32 slots, the 7 past the window at
+// +infinity, then the 13th value.
+
+struct MergeNetwork32 {
+    int count;
+    unsigned char lo[192], hi[192];
+    // Batcher's odd-even merge sort on n = 32 inputs (191 compare-exchanges).
+    constexpr MergeNetwork32() : count(0), lo(), hi() {
+        const int n = 32;
+        for (int p = 1; p < n; p += p)
+            for (int k = p; k > 0; k /= 2)
+                for (int j = k % p; j + k < n; j += 2 * k)
+                    for (int i = 0; i < k && i + j + k < n; i++)
+                        if ((i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+                            lo[count] = (unsigned char)(i + j);
+                            hi[count] = (unsigned char)(i + j + k);
+                            count++;
+                        }
+    }
+};
+constexpr MergeNetwork32 MERGE32{};
+static_assert(MERGE32.count == 191, "Batcher's network on 32 inputs has 191 exchanges");
+
+template <int SIDE>
+__global__ __launch_bounds__(256) void median_filter_kernel(const float *__restrict__ in,
+                                                            float *__restrict__ out, int W,
+                                                            int H) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W) return;
+    constexpr int R = SIDE / 2, SLOTS = SIDE == 3 ? 9 : 32;
+    float v[SLOTS];
+    int n = 0;
+#pragma unroll
+    for (int dy = -R; dy <= R; dy++) {
+        const float *row = in + (size_t)min(max(y + dy, 0), H - 1) * W;
+#pragma unroll
+        for (int dx = -R; dx <= R; dx++) v[n++] = row[min(max(x + dx, 0), W - 1)];
+    }
+#define MX(a, b)                          \
+    {                                     \
+        const float lo_ = fminf(v[a], v[b]); \
+        const float hi_ = fmaxf(v[a], v[b]); \
+        v[a] = lo_;                       \
+        v[b] = hi_;                       \
+    }
+    if constexpr (SIDE == 3) {
+        MX(1, 2) MX(4, 5) MX(7, 8) MX(0, 1) MX(3, 4) MX(6, 7) MX(1, 2) MX(4, 5) MX(7, 8) MX(0, 3)
+        MX(5, 8) MX(4, 7) MX(3, 6) MX(1, 4) MX(2, 5) MX(4, 7) MX(4, 2) MX(6, 4) MX(4, 2)
+    } else {
+#pragma unroll
+        for (int i = SIDE * SIDE; i < SLOTS; i++) v[i] = __builtin_inff();
+#pragma unroll
+        for (int c = 0; c < MERGE32.count; c++) MX(MERGE32.lo[c], MERGE32.hi[c])
+    }
+#undef MX
+    out[(size_t)y * W + x] = v[SIDE * SIDE / 2];
+}
+
+hipError_t launch_median_filter(const float *in, float *out, int W, int H, int kernel_size,
+                                hipStream_t stream) {
+    const dim3 grid((W + 255) / 256, H);
+    if (kernel_size == 3)
+        hipLaunchKernelGGL(median_filter_kernel<3>, grid, dim3(256), 0, stream, in, out, W, H);
+    else if (kernel_size == 5)
+        hipLaunchKernelGGL(median_filter_kernel<5>, grid, dim3(256), 0, stream, in, out, W, H);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// Sharpness (measures.rs:11-51): per column x < W-1 and image, over the rows, the pairs
+// (x, x+1) with both above black: their squared difference, the left grey and the count.
+// Then, per plane, each column's sum cast to f32, those summed in f64 and cast to f32. The
+// column sums are integers and their f32 values integers under 2^33, so the f64 sum of at
+// most 4095 of them is exact in any order. Planes: [left energy, right energy, left grey,
+// right grey, left pairs, right pairs]. Two launches: row chunks add into u64 column totals
+// (atomics), then one block finishes and sets the totals back to zero.
+
+constexpr int SHARP_ROWS = 64;
+constexpr int SHARP_FINISH = 1024;
+
+__global__ __launch_bounds__(256) void sharpness_chunks_kernel(
+    const uint8_t *__restrict__ left, const uint8_t *__restrict__ right, int W, int H,
+    unsigned long long *__restrict__ totals) {
+    const int n = W - 1, x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int y0 = blockIdx.y * SHARP_ROWS, y1 = min(H, y0 + SHARP_ROWS);
+    uint32_t energy[2] = {0, 0}, grey[2] = {0, 0}, pairs[2] = {0, 0};
+    for (int y = y0; y < y1; y++) {
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const uint8_t *row = (side ? right : left) + (size_t)y * W;
+            const int a = row[x], b = row[x + 1];
+            if (a > 0 && b > 0) {
+                energy[side] += (uint32_t)((b - a) * (b - a));
+                grey[side] += (uint32_t)a;
+                pairs[side] += 1;
+            }
+        }
+    }
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+        if (energy[side]) atomicAdd(&totals[(size_t)(0 + side) * n + x], (unsigned long long)energy[side]);
+        if (grey[side]) atomicAdd(&totals[(size_t)(2 + side) * n + x], (unsigned long long)grey[side]);
+        if (pairs[side]) atomicAdd(&totals[(size_t)(4 + side) * n + x], (unsigned long long)pairs[side]);
+    }
+}
+
+__global__ __launch_bounds__(SHARP_FINISH) void sharpness_finish_kernel(
+    unsigned long long *__restrict__ totals, int n, float *__restrict__ sums6) {
+    __shared__ double part[SHARP_FINISH];
+    for (int plane = 0; plane < 6; plane++) {
+        double sum = 0.0;
+        for (int x = threadIdx.x; x < n; x += SHARP_FINISH) {
+            sum += (double)(float)totals[(size_t)plane * n + x];
+            totals[(size_t)plane * n + x] = 0;
+        }
+        part[threadIdx.x] = sum;
+        __syncthreads();
+        for (int half = SHARP_FINISH / 2; half > 0; half >>= 1) {
+            if ((int)threadIdx.x < half) part[threadIdx.x] += part[threadIdx.x + half];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) sums6[plane] = (float)part[0];
+        __syncthreads();
+    }
+}
+
+size_t sharpness_scratch_bytes(int W) { return 6 * (size_t)max(W - 1, 1) * sizeof(uint64_t); }
+
+hipError_t launch_sharpness(const uint8_t *left, const uint8_t *right, int W, int H,
+                            uint64_t *totals, float *sums6, hipStream_t stream) {
+    auto *t = reinterpret_cast<unsigned long long *>(totals);
+    const int n = W - 1;
+    if (n > 0)
+        hipLaunchKernelGGL(sharpness_chunks_kernel, dim3((n + 255) / 256, (H + SHARP_ROWS - 1) / SHARP_ROWS),
+                           dim3(256), 0, stream, left, right, W, H, t);
+    hipLaunchKernelGGL(sharpness_finish_kernel, dim3(1), dim3(SHARP_FINISH), 0, stream, t, max(n, 0),
+                       sums6);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// RowAlignment (measures.rs:53-108): per band of W / bands columns, over rows 0, row_step,
+// ...: each matched left pixel's absolute difference with the right pixel at its rounded
+// disparity (half away from zero, clamped to the image) on each of the 2 reach + 1 rows
+// from reach above to reach below (clamped), then the matched count; each sum cast to f32.
+// A block takes one band and ALIGN_ROWS sampled rows; each thread sums up to ALIGN_PLANES
+// planes at a time in registers (planes past that in further passes), the waves reduce
+// them, and the block adds them into u64 totals. One more launch casts the totals and sets
+// them back to zero.
+
+constexpr int ALIGN_ROWS = 8;
+constexpr int ALIGN_PLANES = 16;
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void alignment_chunks_kernel(
+    const uint8_t *__restrict__ left, const uint8_t *__restrict__ right,
+    const float *__restrict__ disparity, int W, int H, int bands, int reach, int row_step,
+    unsigned long long *__restrict__ totals) {
+    const int planes = 2 * reach + 2, band = blockIdx.x, bw = W / bands;
+    const int x0 = band * bw, rows = (H + row_step - 1) / row_step;
+    const int r0 = blockIdx.y * ALIGN_ROWS, r1 = min(rows, r0 + ALIGN_ROWS);
+    for (int p0 = 0; p0 < planes; p0 += ALIGN_PLANES) {
+        uint32_t acc[ALIGN_PLANES];
+#pragma unroll
+        for (int p = 0; p < ALIGN_PLANES; p++) acc[p] = 0;
+        for (int r = r0; r < r1; r++) {
+            const int y = r * row_step;
+            for (int x = x0 + (int)threadIdx.x; x < x0 + bw; x += blockDim.x) {
+                const float value = disparity[(size_t)y * W + x];
+                if (!(value > 0.0f)) continue;
+                const float sx = fminf(fmaxf(__fsub_rn((float)x, roundf(value)), 0.0f), (float)(W - 1));
+                const int source_x = (int)sx;
+                const int grey = left[(size_t)y * W + x];
+#pragma unroll
+                for (int p = 0; p < ALIGN_PLANES; p++) {
+                    const int plane = p0 + p;
+                    if (plane < 2 * reach + 1) {
+                        const int sy = min(max(y + plane - reach, 0), H - 1);
+                        acc[p] += (uint32_t)abs((int)right[(size_t)sy * W + source_x] - grey);
+                    } else if (plane == 2 * reach + 1) {
+                        acc[p] += 1;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < ALIGN_PLANES; p++) {
+            const uint32_t v = wave_sum(acc[p]);
+            if ((threadIdx.x & 63) == 0 && v != 0 && p0 + p < planes)
+                atomicAdd(&totals[(size_t)band * planes + p0 + p], (unsigned long long)v);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void alignment_finish_kernel(unsigned long long *__restrict__ totals,
+                                                               int count, float *__restrict__ sums) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    sums[i] = __ull2float_rn(totals[i]);
+    totals[i] = 0;
+}
+
+size_t alignment_scratch_bytes(int bands, int reach) {
+    return (size_t)bands * (2 * reach + 2) * sizeof(uint64_t);
+}
+
+hipError_t launch_alignment(const uint8_t *left, const uint8_t *right, const float *disparity,
+                            int W, int H, int bands, int reach, int row_step, uint64_t *totals,
+                            float *sums, hipStream_t stream) {
+    auto *t = reinterpret_cast<unsigned long long *>(totals);
+    const int rows = (H + row_step - 1) / row_step, count = bands * (2 * reach + 2);
+    if (W / bands > 0)
+        hipLaunchKernelGGL(alignment_chunks_kernel, dim3(bands, (rows + ALIGN_ROWS - 1) / ALIGN_ROWS),
+                           dim3(256), 0, stream, left, right, disparity, W, H, bands, reach,
+                           row_step, t);
+    hipLaunchKernelGGL(alignment_finish_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, t,
+                       count, sums);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_f32(float *dst, float value, size_t count, hipStream_t stream) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream,

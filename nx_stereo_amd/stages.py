@@ -158,6 +158,36 @@ class Disparity(Stage):
         return self.state["matcher"]
 
 
+class Sharpness(Stage):
+    """stage/sharpness.ex: each rectified image's sharpness sums, from left and right."""
+
+    KIND = "sharpness"
+    OUTPUTS = ("sharpness_sums",)
+
+
+class MedianFilter(Stage):
+    """stage/median_filter.ex: the median of each pixel's kernel_size^2 window."""
+
+    KIND = "median_filter"
+    OUTPUTS = ("disparity",)
+    SETTINGS = {"kernel_size": {"default": 5, "type": "integer", "values": [3, 5]}}
+
+    def native_settings(self):
+        return [("kernel_size", self.state["kernel_size"])]
+
+
+class RowAlignment(Stage):
+    """stage/row_alignment.ex: the row-offset sums, with the diagnostics' default bands,
+    reach and row step as constants."""
+
+    KIND = "row_alignment"
+    OUTPUTS = ("alignment_sums",)
+    BANDS, REACH, ROW_STEP = 4, 2, 4
+
+    def native_constants(self):
+        return {"bands": self.BANDS, "reach": self.REACH, "row_step": self.ROW_STEP}
+
+
 class SpeckleFilter(Stage):
     """stage/speckle_filter.ex: 7x7 support count."""
 
@@ -233,8 +263,8 @@ class Quality(Stage):
     OUTPUTS = ("quality_sums",)
 
 
-KINDS = {cls.KIND: cls for cls in (Rectification, Disparity, SpeckleFilter, TemporalFilter, Depth,
-                                   PointCloud, Quality)}
+KINDS = {cls.KIND: cls for cls in (Rectification, Sharpness, Disparity, MedianFilter, SpeckleFilter,
+                                   RowAlignment, TemporalFilter, Depth, PointCloud, Quality)}
 
 
 def check_geometry(stage: Stage, geometry: dict) -> None:

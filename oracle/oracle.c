@@ -377,3 +377,90 @@ void orc_quality(const float *disparity, float *previous, int width, int height,
     }
     free(columns);
 }
+
+/* filters.rs:134-216: each window position's value with the edges repeated, then the
+ * middle of the window; sorted here by f32::total_cmp instead of a network, which gives
+ * the same middle for every value that is not a NaN or a signed zero. */
+void orc_median(const float *image, int width, int height, int kernel, float *out) {
+    const int radius = kernel / 2;
+    float values[25];
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < width; x++) {
+            int n = 0;
+            for (int dy = -radius; dy <= radius; dy++)
+                for (int dx = -radius; dx <= radius; dx++)
+                    values[n++] = image[(size_t)clampi(y + dy, 0, height - 1) * width +
+                                        clampi(x + dx, 0, width - 1)];
+            qsort(values, (size_t)n, sizeof(float), total_cmp);
+            out[(size_t)y * width + x] = values[n / 2];
+        }
+}
+
+/* measures.rs:49-51 (sum_columns): each column's sum rounded to f32, summed in f64 in
+ * column order, rounded to f32. */
+static float sum_columns(const uint64_t *columns, int n) {
+    double total = 0.0;
+    for (int i = 0; i < n; i++) total += (double)(float)columns[i];
+    return (float)total;
+}
+
+/* measures.rs:24-47 (sharpness_planes): per column x of the pairs (x, x+1) with both above
+ * black, the squared difference, the left grey and the count, down the rows. */
+static void sharpness_planes(const uint8_t *image, int width, int height, float *energy,
+                             float *grey, float *pairs) {
+    const int n = width - 1;
+    uint64_t *e = (uint64_t *)calloc((size_t)(n > 0 ? n : 1), sizeof(uint64_t));
+    uint64_t *g = (uint64_t *)calloc((size_t)(n > 0 ? n : 1), sizeof(uint64_t));
+    uint64_t *c = (uint64_t *)calloc((size_t)(n > 0 ? n : 1), sizeof(uint64_t));
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < n; x++) {
+            const int a = image[(size_t)y * width + x], b = image[(size_t)y * width + x + 1];
+            if (a > 0 && b > 0) {
+                const int64_t d = (int64_t)b - a;
+                e[x] += (uint64_t)(d * d);
+                g[x] += (uint64_t)a;
+                c[x] += 1;
+            }
+        }
+    *energy = sum_columns(e, n);
+    *grey = sum_columns(g, n);
+    *pairs = sum_columns(c, n);
+    free(e);
+    free(g);
+    free(c);
+}
+
+/* measures.rs:11-22 */
+void orc_sharpness(const uint8_t *left, const uint8_t *right, int width, int height,
+                   float *sums6) {
+    sharpness_planes(left, width, height, &sums6[0], &sums6[2], &sums6[4]);
+    sharpness_planes(right, width, height, &sums6[1], &sums6[3], &sums6[5]);
+}
+
+/* measures.rs:71-108: per band of width / bands columns, over rows 0, row_step, ...: each
+ * matched left pixel's absolute difference with the right pixel at its rounded disparity
+ * (clamped to the image) on each of the rows reach above to reach below (clamped), then
+ * the matched count; each sum rounded to f32. */
+void orc_alignment(const uint8_t *left, const uint8_t *right, const float *disparity, int width,
+                   int height, int bands, int reach, int row_step, float *sums) {
+    const int planes = 2 * reach + 2, band_width = width / bands;
+    uint64_t *acc = (uint64_t *)calloc((size_t)bands * planes, sizeof(uint64_t));
+    for (int y = 0; y < height; y += row_step)
+        for (int band = 0; band < bands; band++)
+            for (int x = band * band_width; x < (band + 1) * band_width; x++) {
+                const float value = disparity[(size_t)y * width + x];
+                if (!(value > 0.0f)) continue;
+                float sx = (float)x - roundf(value);
+                sx = sx < 0.0f ? 0.0f : (sx > (float)(width - 1) ? (float)(width - 1) : sx);
+                const int source_x = (int)sx;
+                const int grey = left[(size_t)y * width + x];
+                for (int offset = 0; offset <= 2 * reach; offset++) {
+                    const int source_y = clampi(y + offset - reach, 0, height - 1);
+                    const int d = (int)right[(size_t)source_y * width + source_x] - grey;
+                    acc[(size_t)band * planes + offset] += (uint64_t)(d < 0 ? -d : d);
+                }
+                acc[(size_t)band * planes + 2 * reach + 1] += 1;
+            }
+    for (int i = 0; i < bands * planes; i++) sums[i] = (float)acc[i];
+    free(acc);
+}

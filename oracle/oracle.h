@@ -24,6 +24,10 @@
  *   orc_cloud        - depth_test.exs:30-98, depth.rs:109-154
  *   orc_temporal     - filter_test.exs:69-100, filters.rs:406-458
  *   orc_quality      - measures.rs:241-252
+ *   orc_median       - filters.rs:331-357 (the sorted middle, 3x3 and 5x5)
+ *   orc_sharpness    - measures.rs:168-184 and diagnostics_test.exs:17-53 closed forms
+ *   orc_alignment    - measures.rs:186-239 (the moved offset, disparity 0, configured
+ *                      bands/reach/rows) and diagnostics_test.exs:79-95
  */
 #ifndef NXS_ORACLE_H
 #define NXS_ORACLE_H
@@ -77,6 +81,19 @@ void orc_temporal(float *disparity, float *previous, float *missed, int pixels, 
                   float delta, float hold, float max_missed);
 /* measures.rs:125-154; previous carried */
 void orc_quality(const float *disparity, float *previous, int width, int height, float *sums5);
+
+/* filters.rs:134-216 (median, median_row): the middle of each pixel's kernel x kernel
+ * window (3 or 5), borders replicated, in f32's total order. */
+void orc_median(const float *image, int width, int height, int kernel, float *out);
+
+/* measures.rs:11-51 (sharpness_sums): [left energy, right energy, left grey, right grey,
+ * left pairs, right pairs]. */
+void orc_sharpness(const uint8_t *left, const uint8_t *right, int width, int height,
+                   float *sums6);
+
+/* measures.rs:53-108 (Alignment, alignment_sums): bands rows of 2 reach + 2 sums. */
+void orc_alignment(const uint8_t *left, const uint8_t *right, const float *disparity, int width,
+                   int height, int bands, int reach, int row_step, float *sums);
 
 #ifdef __cplusplus
 }

@@ -56,6 +56,9 @@ def lib() -> ctypes.CDLL:
         _lib.orc_cloud.restype = i
         _lib.orc_temporal.argtypes = [_P, _P, _P, i, f, f, f, f]
         _lib.orc_quality.argtypes = [_P, _P, i, i, _P]
+        _lib.orc_median.argtypes = [_P, i, i, i, _P]
+        _lib.orc_sharpness.argtypes = [_P, _P, i, i, _P]
+        _lib.orc_alignment.argtypes = [_P, _P, _P, i, i, i, i, i, _P]
     return _lib
 
 
@@ -133,6 +136,36 @@ def median3(image):
     h, w = image.shape
     out = np.empty((h, w), np.float32)
     lib().orc_median3(_ptr(image), w, h, _ptr(out))
+    return out
+
+
+def median(image, kernel_size):
+    """filters.rs:134-216: the middle of each pixel's kernel_size^2 window (3 or 5)."""
+    image = _c(image, np.float32)
+    h, w = image.shape
+    out = np.empty((h, w), np.float32)
+    lib().orc_median(_ptr(image), w, h, int(kernel_size), _ptr(out))
+    return out
+
+
+def sharpness_sums(left, right):
+    """measures.rs:11-51: [left energy, right energy, left grey, right grey, left pairs,
+    right pairs]."""
+    left, right = _c(left, np.uint8), _c(right, np.uint8)
+    h, w = left.shape
+    out = np.empty(6, np.float32)
+    lib().orc_sharpness(_ptr(left), _ptr(right), w, h, _ptr(out))
+    return out
+
+
+def alignment_sums(left, right, disparity, bands=4, reach=2, row_step=4):
+    """measures.rs:53-108: bands rows of 2 reach + 2 sums."""
+    left, right = _c(left, np.uint8), _c(right, np.uint8)
+    disparity = _c(disparity, np.float32)
+    h, w = left.shape
+    out = np.empty((bands, 2 * reach + 2), np.float32)
+    lib().orc_alignment(_ptr(left), _ptr(right), _ptr(disparity), w, h, bands, reach, row_step,
+                        _ptr(out))
     return out
 
 

@@ -126,6 +126,77 @@ def test_speckle_radius(context, radius):
     context.request(protocol.release(segment_id))
 
 
+@pytest.mark.parametrize("kernel_size", [3, 5])
+def test_median_filter(context, kernel_size):
+    """filters.rs:134-216 through stage/median_filter.ex's native kind, on a speckled map and
+    on filters.rs:331-357's image (negatives and zeros). Sizes off the 256-wide rows."""
+    h, w = 37, 301
+    rng = np.random.default_rng(kernel_size)
+    speckled = np.where(rng.random((h, w)) < 0.3, -1.0, rng.uniform(0.5, 60.0, (h, w)))
+    formula = ((np.arange(h * w) * 37 + 11) % 53 / 4.0 - 1.0).reshape(h, w)
+    geometry, _ = rig_geometry(w, h)
+    seg = build(context, [("median_filter", stages.MedianFilter(kernel_size=kernel_size))], geometry)
+    for entry in (speckled.astype(np.float32), formula.astype(np.float32)):
+        out = run_ok(seg, {"disparity": entry})
+        same_bits(out["disparity"], oracle.median(entry, kernel_size))
+
+
+def test_sharpness(context):
+    """measures.rs:11-51 through stage/sharpness.ex's native kind: a random pair with black
+    pixels, and diagnostics_test.exs:17-25's closed form."""
+    rng = np.random.default_rng(12)
+    h, w = 70, 333
+    left = np.where(rng.random((h, w)) < 0.1, 0, rng.integers(1, 256, (h, w))).astype(np.uint8)
+    right = np.where(rng.random((h, w)) < 0.1, 0, rng.integers(1, 256, (h, w))).astype(np.uint8)
+    geometry, _ = rig_geometry(w, h)
+    seg = build(context, [("sharpness", stages.Sharpness())], geometry)
+    for _ in range(2):  # the totals are set back to zero for the next pair
+        out = run_ok(seg, {"left": left, "right": right})
+        same_bits(out["sharpness_sums"], oracle.sharpness_sums(left, right))
+    small = {"width": 3, "height": 2, "frame_width": 3, "frame_height": 2}
+    assert context.request(protocol.configure(7300, small, [("sharpness", [])])) is True
+    reply = context.request(protocol.run(7300, [[]], [
+        ("left", np.array([[10, 20, 10], [10, 10, 10]], np.uint8)),
+        ("right", np.full((2, 3), 10, np.uint8))]))
+    assert reply.tensors["sharpness_sums"].tolist() == [200.0, 0.0, 50.0, 40.0, 4.0, 4.0]
+    context.request(protocol.release(7300))
+
+
+@pytest.mark.parametrize("bands, reach, row_step", [(4, 2, 4), (3, 9, 1), (7, 0, 5)])
+def test_row_alignment(context, bands, reach, row_step):
+    """measures.rs:53-108 through stage/row_alignment.ex's native kind (its constants 4, 2,
+    4) and other build settings; reach 9 sums its 19 row offsets in two passes."""
+    rng = np.random.default_rng(bands * 100 + reach)
+    h, w = 61, 290
+    left = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    right = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    disparity = np.where(rng.random((h, w)) < 0.3, -1.0, rng.uniform(0.0, 80.0, (h, w)))
+    disparity[::7, ::5] = np.floor(disparity[::7, ::5]) + 0.5  # halves round away from zero
+    disparity = disparity.astype(np.float32)
+    g = {"width": w, "height": h, "frame_width": w, "frame_height": h}
+    settings = [("bands", float(bands)), ("reach", float(reach)), ("row_step", float(row_step))]
+    segment_id = 7400 + bands
+    assert context.request(protocol.configure(segment_id, g, [("row_alignment", settings)])) is True
+    for _ in range(2):
+        reply = context.request(protocol.run(segment_id, [[]], [
+            ("left", left), ("right", right), ("disparity", disparity)]))
+        got = reply.tensors["alignment_sums"]
+        assert got.shape == (bands, 2 * reach + 2)
+        same_bits(got, oracle.alignment_sums(left, right, disparity, bands, reach, row_step))
+    context.request(protocol.release(segment_id))
+
+
+def test_new_kinds_refuse_settings_as_segment_rs_does(context):
+    g = {"width": 32, "height": 24, "frame_width": 32, "frame_height": 24}
+    reply = context.request(protocol.configure(7500, g, [("median_filter", [("kernel_size", 4.0)])]))
+    assert reply.reason == "median_filter kernel_size must be one of [3, 5], got 4"
+    reply = context.request(protocol.configure(7500, g, [("row_alignment", [
+        ("bands", 0.0), ("reach", 2.0), ("row_step", 4.0)])]))
+    assert reply.reason == "row_alignment bands must be an integer from 1 to 32, got 0"
+    reply = context.request(protocol.configure(7500, g, [("sharpness", [("radius", 3.0)])]))
+    assert reply.reason == "sharpness has no setting radius"
+
+
 def test_temporal_filter_carries_across_pairs(context):
     geometry, _ = rig_geometry(64, 32)
     seg = build(context, [("temporal_filter", stages.TemporalFilter(alpha=0.5, delta=2.0, hold=1))],

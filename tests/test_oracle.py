@@ -442,6 +442,111 @@ def test_quality_known_answers():
     assert q.sums(np.array([[3.0, 1.0], [1.0, 4.0]])).tolist() == [4.0, 9.0, 27.0, 2.0, 1.0]
 
 
+@pytest.mark.parametrize("kernel_size", [3, 5])
+def test_median_filter_is_the_sorted_middle_with_replicated_borders(kernel_size):
+    """filters.rs:331-357's image, which holds negatives and zeros, at both window sizes."""
+    image = ((np.arange(21 * 9) * 37 + 11) % 53 / 4.0 - 1.0).astype(np.float32).reshape(9, 21)
+    got = oracle.median(image, kernel_size)
+    r = kernel_size // 2
+    padded = np.pad(image, r, mode="edge")
+    for y in range(9):
+        for x in range(21):
+            window = np.sort(padded[y:y + kernel_size, x:x + kernel_size].reshape(-1))
+            assert got[y, x] == window[kernel_size * kernel_size // 2]
+    if kernel_size == 3:
+        assert np.array_equal(got, oracle.median3(image))
+
+
+# --------------------------------------------------------------------------------------
+# Sharpness and RowAlignment sums (measures.rs:168-239; diagnostics_test.exs:17-95)
+
+
+def test_sharpness_sums_neighbour_differences_and_grey():
+    image = np.array([[10, 13, 13], [1, 1, 5]], np.uint8)
+    assert oracle.sharpness_sums(image, image).tolist() == [25.0, 25.0, 25.0, 25.0, 4.0, 4.0]
+
+
+def test_sharpness_counts_only_neighbours_both_above_black():
+    image = np.array([[10, 13, 13], [0, 0, 4]], np.uint8)
+    got = oracle.sharpness_sums(image, np.zeros((2, 3), np.uint8))
+    assert got.tolist() == [9.0, 0.0, 23.0, 0.0, 2.0, 0.0]
+
+
+def test_sharpness_diagnostics_cases():
+    left = np.array([[10, 20, 10], [10, 10, 10]], np.uint8)
+    right = np.full((2, 3), 10, np.uint8)
+    # Left: differences 10, -10, 0, 0 over greys 10, 20, 10, 10; right: flat.
+    assert oracle.sharpness_sums(left, right).tolist() == [200.0, 0.0, 50.0, 40.0, 4.0, 4.0]
+    # Only the pair (10, 20) counts.
+    one_row = np.array([[0, 200, 0, 10, 20]], np.uint8)
+    got = oracle.sharpness_sums(one_row, np.zeros((1, 5), np.uint8))
+    assert got.tolist() == [100.0, 0.0, 10.0, 0.0, 1.0, 0.0]
+    # A single column has no pairs.
+    column = np.array([[10], [20], [30]], np.uint8)
+    assert oracle.sharpness_sums(column, column).tolist() == [0.0] * 6
+
+
+def test_alignment_is_lowest_at_the_offset_the_right_image_is_moved_by():
+    width, height, bands, reach, row_step = 32, 24, 4, 2, 4
+    def texture(x, y):
+        return (x * 31 + y * 17 + x * y * 7) % 251
+    ys, xs = np.mgrid[0:height, 0:width]
+    left = texture(xs, ys + 4).astype(np.uint8)
+    right = texture(xs + 3, ys + 3).astype(np.uint8)  # rows one lower, three columns left
+    sums = oracle.alignment_sums(left, right, np.full((height, width), 3.0, np.float32),
+                                 bands, reach, row_step)
+    assert sums.shape == (bands, 2 * reach + 2)
+    for band in sums[1:]:
+        assert int(np.argmin(band[:2 * reach + 1])) == reach + 1
+        assert band[2 * reach + 1] == width // bands * height // row_step
+
+
+def test_alignment_counts_no_pixel_at_disparity_0():
+    image = np.full((24, 32), 90, np.uint8)
+    sums = oracle.alignment_sums(image, image, np.zeros((24, 32), np.float32))
+    assert sums.tolist() == [[0.0] * 6] * 4
+
+
+def test_alignment_takes_its_bands_reach_and_rows_as_configured():
+    image = np.full((6, 8), 90, np.uint8)
+    sums = oracle.alignment_sums(image, image, np.ones((6, 8), np.float32), 2, 1, 2)
+    assert sums.reshape(-1).tolist() == [0.0, 0.0, 0.0, 12.0, 0.0, 0.0, 0.0, 12.0]
+
+
+@pytest.mark.parametrize("bands", [4, 8])
+def test_alignment_has_no_samples_when_narrower_than_its_bands(bands):
+    image = np.array([[10, 20, 30]], np.uint8)
+    sums = oracle.alignment_sums(image, image, np.ones((1, 3), np.float32), bands)
+    assert sums.shape == (bands, 6) and not sums.any()
+
+
+def test_alignment_by_brute_force():
+    """Every sum against a direct count on a random scene (rounding half away from zero,
+    clamped columns and rows)."""
+    rng = np.random.default_rng(5)
+    h, w = 13, 29
+    left = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    right = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    disparity = np.where(rng.random((h, w)) < 0.3, -1.0, rng.uniform(0.0, 40.0, (h, w)))
+    disparity[0, :4] = [0.5, 1.5, 2.5, 35.5]
+    disparity = disparity.astype(np.float32)
+    bands, reach, row_step = 3, 2, 3
+    want = np.zeros((bands, 2 * reach + 2))
+    bw = w // bands
+    for y in range(0, h, row_step):
+        for x in range(bands * bw):
+            v = float(disparity[y, x])
+            if not v > 0:
+                continue
+            sx = min(max(x - np.floor(v + 0.5), 0), w - 1)
+            for o in range(2 * reach + 1):
+                sy = min(max(y + o - reach, 0), h - 1)
+                want[x // bw, o] += abs(int(right[sy, int(sx)]) - int(left[y, x]))
+            want[x // bw, 2 * reach + 1] += 1
+    got = oracle.alignment_sums(left, right, disparity, bands, reach, row_step)
+    assert np.array_equal(got, want.astype(np.float32))
+
+
 # --------------------------------------------------------------------------------------
 # Rectification (rectify.rs:173-211; rectify_test.exs:39-162)
 
