@@ -109,6 +109,48 @@ int nxs_match_host(const uint8_t *left, const uint8_t *right, int width, int hei
                    uint16_t *sums, float *selected, uint32_t *keys, float *checked,
                    float *disparity);
 
+/* ---- Band search and coarse-to-fine ----
+ *
+ * NxStereo.sgm_band/4 (lib/nx_stereo.ex:262-322): each pixel searches only the `band`
+ * disparities [base, base + band) of its s32 base {height,width} (device memory; a negative
+ * base: no estimate, no match). u8 images as nxs_match_device; disparities 64 or 128; band
+ * 8, 16, 32 or 64; census 0 (sparse_9x7), 1 (dense_9x5) or 2 (dense_9x5_blend). The output
+ * is sgm_band/4's: the 3x3 median of the checked map (a match at 0 stays 0). Bases above
+ * 65536 act as 65536. `workspace`: nxs_band_workspace_size bytes of device memory. */
+size_t nxs_band_workspace_size(int width, int height, int band, int paths);
+int nxs_match_band_device(const uint8_t *left, const uint8_t *right, const int32_t *base,
+                          int width, int height, int disparities, int band, int paths,
+                          int census, int p1, int p2, float uniqueness, float *disparity,
+                          void *workspace, size_t workspace_bytes, void *stream);
+
+/* The same on host buffers with the intermediate products (any output may be NULL): census
+ * u64 {H,W} of each image, band costs u8 {H,W,band}, summed path costs u16 {H,W,band}, the
+ * selected map f32 and keys u32 before the check, the checked map, the output. */
+int nxs_match_band_host(const uint8_t *left, const uint8_t *right, const int32_t *base,
+                        int width, int height, int disparities, int band, int paths, int census,
+                        int p1, int p2, float uniqueness, uint64_t *census_left,
+                        uint64_t *census_right, uint8_t *costs, uint16_t *sums, float *selected,
+                        uint32_t *keys, float *checked, float *disparity);
+
+/* NxStereo.sgm_coarse_to_fine/3 (lib/nx_stereo.ex:164-259): both images shrunk to half size
+ * (rounded 2x2 average) and matched over 64 disparities with coarse_paths; each pixel's base
+ * is twice the lowest match of its coarse 3x3 neighbourhood less `margin` (0..127), clipped
+ * to [0, 128 - band]; then the band search at 128 disparities. Even width and height, and
+ * halves that nxs_match_device takes. The output is sgm_coarse_to_fine/3's (the Disparity
+ * stage then maps values <= 0 to -1). */
+size_t nxs_coarse_to_fine_workspace_size(int width, int height, int band, int paths,
+                                         int coarse_paths);
+int nxs_coarse_to_fine_device(const uint8_t *left, const uint8_t *right, int width, int height,
+                              int band, int margin, int paths, int coarse_paths, int census,
+                              int p1, int p2, float uniqueness, float *disparity, void *workspace,
+                              size_t workspace_bytes, void *stream);
+/* On host buffers, with the half images u8 {H/2,W/2}, the coarse map f32 {H/2,W/2} and the
+ * bases s32 {H,W} (any output may be NULL). */
+int nxs_coarse_to_fine_host(const uint8_t *left, const uint8_t *right, int width, int height,
+                            int band, int margin, int paths, int coarse_paths, int census, int p1,
+                            int p2, float uniqueness, uint8_t *half_left, uint8_t *half_right,
+                            float *coarse, int32_t *base, float *disparity);
+
 /* ---- The batched form ----
  *
  * Many pairs of uneven sizes and disparity ranges through the matcher
@@ -184,7 +226,9 @@ int nxs_run_device(nxs_ctx *ctx, const uint8_t *record, size_t len, const nxs_te
  * Between start and stop, every matcher launch is bracketed by HIP events on
  * its own stream (up to `capacity` launches). nxs_profile_stop waits for
  * them and adds up milliseconds and launch counts per kernel, in the order
- * census, cost, path (the aggregation), select, consistency, median. No
+ * census_cost, path (the aggregation), select, check_median (the left-right check and the
+ * median), band_cost, band_path, band_select, shrink (coarse-to-fine's shrink and band
+ * bases). No
  * counterpart in the reference (test/bench/pipeline.exs --kernels times its
  * Vulkan kernels one by one). */
 int nxs_profile_start(int capacity);

@@ -20,6 +20,8 @@ NXS_OK, NXS_TIMEOUT, NXS_EINVAL, NXS_EDEVICE, NXS_ENOMEM, NXS_EUNKNOWN = 0, 1, -
 SYMBOLS = (
     "nxs_open", "nxs_close", "nxs_submit", "nxs_wait", "nxs_reply", "nxs_release_reply",
     "nxs_poll", "nxs_set_hbm_budget", "nxs_stats", "nxs_match_workspace_size", "nxs_match_device", "nxs_match_host",
+    "nxs_band_workspace_size", "nxs_match_band_device", "nxs_match_band_host",
+    "nxs_coarse_to_fine_workspace_size", "nxs_coarse_to_fine_device", "nxs_coarse_to_fine_host",
     "nxs_run_device", "nxs_profile_start", "nxs_profile_stop", "nxs_schedule_set",
     "nxs_schedule_get", "nxs_batch_create",
     "nxs_batch_run", "nxs_batch_describe", "nxs_batch_destroy", "nxs_last_error",
@@ -90,6 +92,18 @@ def load() -> ctypes.CDLL:
     lib.nxs_match_device.restype = i
     lib.nxs_match_host.argtypes = [P, P, i, i, i, i, i, i, f] + [P] * 8
     lib.nxs_match_host.restype = i
+    lib.nxs_band_workspace_size.argtypes = [i, i, i, i]
+    lib.nxs_band_workspace_size.restype = sz
+    lib.nxs_match_band_device.argtypes = [P, P, P, i, i, i, i, i, i, i, i, f, P, P, sz, P]
+    lib.nxs_match_band_device.restype = i
+    lib.nxs_match_band_host.argtypes = [P, P, P, i, i, i, i, i, i, i, i, f] + [P] * 8
+    lib.nxs_match_band_host.restype = i
+    lib.nxs_coarse_to_fine_workspace_size.argtypes = [i, i, i, i, i]
+    lib.nxs_coarse_to_fine_workspace_size.restype = sz
+    lib.nxs_coarse_to_fine_device.argtypes = [P, P, i, i, i, i, i, i, i, i, i, f, P, P, sz, P]
+    lib.nxs_coarse_to_fine_device.restype = i
+    lib.nxs_coarse_to_fine_host.argtypes = [P, P, i, i, i, i, i, i, i, i, i, f] + [P] * 5
+    lib.nxs_coarse_to_fine_host.restype = i
     lib.nxs_run_device.argtypes = [P, P, sz, ctypes.POINTER(NxsTensor), i,
                                    ctypes.POINTER(NxsTensor), i, ctypes.POINTER(i), P]
     lib.nxs_run_device.restype = i
@@ -166,6 +180,105 @@ def match_device(left_ptr: int, right_ptr: int, width: int, height: int, dispari
         paths, p1, p2, float(np.float32(uniqueness)), ctypes.c_void_p(out_ptr),
         ctypes.c_void_p(workspace_ptr), workspace_bytes, ctypes.c_void_p(stream or None))
     _check(rc, "nxs_match_device")
+
+
+# sgm_band/4's census windows (nx_stereo.ex:28), by the index the C ABI takes.
+CENSUS_WINDOWS = ("sparse_9x7", "dense_9x5", "dense_9x5_blend")
+
+
+def _census_index(window: str, key: str) -> int:
+    if window not in CENSUS_WINDOWS:
+        raise ValueError(f":{key} must be one of {list(CENSUS_WINDOWS)}, got {window!r}")
+    return CENSUS_WINDOWS.index(window)
+
+
+def sgm_band(left: np.ndarray, right: np.ndarray, base: np.ndarray, disparities: int = 64,
+             band: int = 16, paths: int = 8, census: str = "sparse_9x7", p1: int = 8,
+             p2: int = 64, uniqueness: float = 1.0526315789473684,
+             intermediates: bool = False) -> dict:
+    """NxStereo.sgm_band/4 (nx_stereo.ex:262-322) on the GPU from host arrays: each pixel
+    searches [base, base + band); a negative base gives no match. With `intermediates`,
+    every product of nxs_match_band_host."""
+    left = np.ascontiguousarray(left, np.uint8)
+    right = np.ascontiguousarray(right, np.uint8)
+    h, w = left.shape
+    if band not in (8, 16, 32, 64):
+        raise ValueError(f":band must be 8, 16, 32 or 64, got {band!r}")
+    base = np.asarray(base)
+    if base.dtype != np.int32 or base.shape != (h, w):
+        raise ValueError(f"sgm_band expects base as an s32 tensor of shape {(h, w)}, "
+                         f"got {base.dtype} {base.shape}")
+    base = np.ascontiguousarray(base)
+    index = _census_index(census, "census")
+    out = {"disparity": np.empty((h, w), np.float32)}
+    if intermediates:
+        out.update(
+            census_left=np.empty((h, w), np.uint64), census_right=np.empty((h, w), np.uint64),
+            costs=np.empty((h, w, band), np.uint8), sums=np.empty((h, w, band), np.uint16),
+            selected=np.empty((h, w), np.float32), keys=np.empty((h, w), np.uint32),
+            checked=np.empty((h, w), np.float32))
+    g = out.get
+    rc = load().nxs_match_band_host(
+        _ptr(left), _ptr(right), _ptr(base), w, h, disparities, band, paths, index, p1, p2,
+        float(np.float32(uniqueness)), _ptr(g("census_left")), _ptr(g("census_right")),
+        _ptr(g("costs")), _ptr(g("sums")), _ptr(g("selected")), _ptr(g("keys")),
+        _ptr(g("checked")), _ptr(out["disparity"]))
+    _check(rc, "nxs_match_band_host")
+    return out
+
+
+def sgm_coarse_to_fine(left: np.ndarray, right: np.ndarray, disparities: int = 128,
+                       band: int = 16, margin: int = 4, paths: int = 8,
+                       coarse_paths: int | None = None, fine_census: str = "sparse_9x7",
+                       p1: int = 8, p2: int = 64, uniqueness: float = 1.0526315789473684,
+                       intermediates: bool = False) -> dict:
+    """NxStereo.sgm_coarse_to_fine/3 (nx_stereo.ex:164-259) on the GPU from host arrays.
+    With `intermediates`, the half images, the coarse map and the bases as well."""
+    left = np.ascontiguousarray(left, np.uint8)
+    right = np.ascontiguousarray(right, np.uint8)
+    h, w = left.shape
+    if h % 2 or w % 2:
+        raise ValueError("sgm_coarse_to_fine expects images of even height and width, "
+                         f"got {(h, w)}")
+    if disparities != 128:
+        raise ValueError(":disparities must be 128: the coarse matching takes 64, "
+                         f"got {disparities!r}")
+    index = _census_index(fine_census, "fine_census")
+    out = {"disparity": np.empty((h, w), np.float32)}
+    if intermediates:
+        out.update(half_left=np.empty((h // 2, w // 2), np.uint8),
+                   half_right=np.empty((h // 2, w // 2), np.uint8),
+                   coarse=np.empty((h // 2, w // 2), np.float32),
+                   base=np.empty((h, w), np.int32))
+    g = out.get
+    rc = load().nxs_coarse_to_fine_host(
+        _ptr(left), _ptr(right), w, h, band, margin, paths, coarse_paths or paths, index, p1, p2,
+        float(np.float32(uniqueness)), _ptr(g("half_left")), _ptr(g("half_right")),
+        _ptr(g("coarse")), _ptr(g("base")), _ptr(out["disparity"]))
+    _check(rc, "nxs_coarse_to_fine_host")
+    return out
+
+
+def band_workspace_size(width: int, height: int, band: int, paths: int) -> int:
+    return int(load().nxs_band_workspace_size(width, height, band, paths))
+
+
+def coarse_to_fine_workspace_size(width: int, height: int, band: int, paths: int,
+                                  coarse_paths: int) -> int:
+    return int(load().nxs_coarse_to_fine_workspace_size(width, height, band, paths, coarse_paths))
+
+
+def coarse_to_fine_device(left_ptr: int, right_ptr: int, width: int, height: int, band: int,
+                          margin: int, paths: int, coarse_paths: int, fine_census: int, p1: int,
+                          p2: int, uniqueness: float, out_ptr: int, workspace_ptr: int,
+                          workspace_bytes: int, stream: int = 0) -> None:
+    """sgm_coarse_to_fine/3 on device buffers (raw device addresses), enqueued on `stream`."""
+    rc = load().nxs_coarse_to_fine_device(
+        ctypes.c_void_p(left_ptr), ctypes.c_void_p(right_ptr), width, height, band, margin,
+        paths, coarse_paths, fine_census, p1, p2, float(np.float32(uniqueness)),
+        ctypes.c_void_p(out_ptr), ctypes.c_void_p(workspace_ptr), workspace_bytes,
+        ctypes.c_void_p(stream or None))
+    _check(rc, "nxs_coarse_to_fine_device")
 
 
 def set_schedule(width, height, disparities, paths, prefetch, split, lanes=16, waves=1) -> None:

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIBRARY = os.path.join(HERE, "libnxs_hip.so")
 PORT = os.path.join(HERE, "stereo_match_hip")  # the Port executable over the library
-SOURCES = ["sgm.hip", "stages.hip", "runtime.cpp"]
+SOURCES = ["sgm.hip", "band.hip", "stages.hip", "runtime.cpp"]
 ARCH = "gfx950"
 
 FLAGS = [

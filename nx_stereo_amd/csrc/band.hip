@@ -1,0 +1,624 @@
+// Band search and coarse-to-fine matching on gfx950: NxStereo.sgm_band/4 and
+// sgm_coarse_to_fine/3 (lib/nx_stereo.ex:164-471).
+//
+// Coarse-to-fine shrinks both images to half size (rounded 2x2 average), matches the
+// halves over 64 disparities with the full-search kernels, and derives each pixel's band
+// base from that match: twice the lowest match of its coarse 3x3 neighbourhood, less a
+// margin. The fine pass then searches only `band` disparities per pixel, [base, base+band).
+//
+// Layout in HBM: base s32 {H,W}; band volumes pixel-major, volume[(y*W + x)*band + k], so
+// one pixel's band is contiguous; one u8 path volume per aggregation direction.
+//
+// Aggregation: each line of a direction is owned by `band` consecutive lanes, lane k holding
+// candidate base + k as a u32. A path moving to a pixel with another base moves its values
+// by the difference (ds_bpermute within the line's lanes); a candidate the previous band
+// lacks is reached only by the p2 jump (or, one past its ends, from the end + p1). Path
+// values stay <= p2 + 48 <= 127, so every direction stores bytes and selection adds them.
+#include "device.hpp"
+#include "kernels.hpp"
+#include "profile.hpp"
+
+namespace nxs {
+
+// Bases above this act as this: every candidate lies past the image's left edge then (W is
+// at most 4096), and base + k stays far from overflow.
+constexpr int BASE_CEILING = 1 << 16;
+
+__device__ __forceinline__ int searched_of(int base) { return min(max(base, 0), BASE_CEILING); }
+
+// ------------------------------------------------------------------------------------------
+// nx_stereo.ex:236-247 (shrink): half[y][x] = (sum of the 2x2 block + 2) / 4. Grid y: the
+// image (0 left, 1 right).
+
+__global__ __launch_bounds__(256) void shrink_kernel(const uint8_t *left, const uint8_t *right,
+                                                     int W, int H, uint8_t *half_left,
+                                                     uint8_t *half_right) {
+    const int cw = W / 2, ch = H / 2;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= cw * ch) return;
+    const int y = i / cw, x = i - y * cw;
+    const uint8_t *top = (blockIdx.y ? right : left) + (size_t)(2 * y) * W + 2 * x;
+    const uint32_t sum = top[0] + top[1] + top[W] + top[W + 1];
+    (blockIdx.y ? half_right : half_left)[i] = (uint8_t)((sum + 2) >> 2);
+}
+
+// ------------------------------------------------------------------------------------------
+// nx_stereo.ex:249-259 (band_base): the lowest match (>= 0) of the coarse 3x3
+// neighbourhood, none outside the map; base = clip(floor(2 lowest) - margin, 0,
+// 128 - band), -1 without a match, over the coarse pixel's 2x2 block.
+
+__global__ __launch_bounds__(256) void band_base_kernel(const float *coarse, int cw, int ch,
+                                                        int band, int margin, int32_t *base) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= cw * ch) return;
+    const int y = i / cw, x = i - y * cw;
+    const float none = 1.0e9f;
+    float lowest = none;
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            const int sy = y + dy, sx = x + dx;
+            if (sy < 0 || sy >= ch || sx < 0 || sx >= cw) continue;
+            const float v = coarse[sy * cw + sx];
+            lowest = fminf(lowest, v >= 0.0f ? v : none);
+        }
+    int b = -1;
+    if (lowest < none) b = min(max((int)(floorf(lowest * 2.0f) - (float)margin), 0), 128 - band);
+    const int W = 2 * cw;
+    int2 *row0 = reinterpret_cast<int2 *>(base + (size_t)(2 * y) * W + 2 * x);
+    int2 *row1 = reinterpret_cast<int2 *>(base + (size_t)(2 * y + 1) * W + 2 * x);
+    *row0 = make_int2(b, b);
+    *row1 = make_int2(b, b);
+}
+
+// ------------------------------------------------------------------------------------------
+// Band census and costs (nx_stereo.ex:324-344, 490-525), one block per image row. The seven
+// rows around row y of both images come into LDS (clamped), the block makes row y's census
+// of both there, then C[y,x,k] = popcount(FL[y,x] ^ FR[y, x - base - k]), FR = 0 left of the
+// image. A thread makes 16 bytes of one pixel (8 with a band of 8): consecutive threads
+// store consecutive runs.
+
+constexpr int BAND_COST_BLOCK = 512;
+
+__host__ __device__ constexpr size_t band_cost_lds(int W) { return 16 * (size_t)W + 14 * (size_t)W; }
+
+// Census of pixel x of the middle row of `img` (7 rows of W). WINDOW 0: sparse_9x7 (dy in
+// -3..3, dx in -4..4, both non-zero, 48 bits); 1: dense_9x5 (dy in -2..2, dx in -4..4 but
+// the centre, 44 bits); 2: dense_9x5 against the blended anchor (1,2,1)x(1,2,1) over the
+// 3x3 plus 16 centre plus 24, over 32. Bit order row-major over the window.
+template <int WINDOW>
+__device__ __forceinline__ uint64_t census_at(const uint8_t *img, int x, int W) {
+    const uint8_t *mid = img + 3 * W;
+    int anchor = mid[x];
+    if constexpr (WINDOW == 2) {
+        int sum = 0;
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++)
+                sum += (dy ? 1 : 2) * (dx ? 1 : 2) * img[(3 + dy) * W + min(max(x + dx, 0), W - 1)];
+        anchor = (sum + 16 * anchor + 24) >> 5;
+    }
+    constexpr int REACH = WINDOW == 0 ? 3 : 2;
+    uint64_t f = 0;
+    int bit = 0;
+#pragma unroll
+    for (int dy = -REACH; dy <= REACH; dy++) {
+        if (WINDOW == 0 && dy == 0) continue;
+        const uint8_t *row = img + (3 + dy) * W;
+#pragma unroll
+        for (int dx = -4; dx <= 4; dx++) {
+            if (dx == 0 && (WINDOW == 0 || dy == 0)) continue;
+            f |= (uint64_t)((int)row[min(max(x + dx, 0), W - 1)] + 1 < anchor) << bit;
+            bit++;
+        }
+    }
+    return f;
+}
+
+// Grid x: the row.
+template <int WINDOW, int B>
+__global__ __launch_bounds__(BAND_COST_BLOCK) void band_cost_kernel(
+    const uint8_t *left, const uint8_t *right, const int32_t *base, int W, int H, uint8_t *costs,
+    uint64_t *census_left, uint64_t *census_right) {
+    extern __shared__ __align__(16) uint8_t band_lds[];
+    uint64_t *cl = reinterpret_cast<uint64_t *>(band_lds);  // [W]
+    uint64_t *cr = cl + W;                                    // [W]
+    uint8_t *rows = reinterpret_cast<uint8_t *>(cr + W);      // [2][7][W]
+    const int y = blockIdx.x;
+    for (int side = 0; side < 2; side++) {
+        const uint8_t *img = side ? right : left;
+        for (int r = 0; r < 7; r++) {
+            const uint8_t *src = img + (size_t)min(max(y + r - 3, 0), H - 1) * W;
+            uint8_t *dst = rows + (side * 7 + r) * W;
+            for (int x = threadIdx.x; x < W; x += BAND_COST_BLOCK) dst[x] = src[x];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2 * W; i += BAND_COST_BLOCK) {
+        const int side = i >= W ? 1 : 0, x = i - side * W;
+        const uint64_t f = census_at<WINDOW>(rows + side * 7 * W, x, W);
+        (side ? cr : cl)[x] = f;
+        if (census_left) (side ? census_right : census_left)[(size_t)y * W + x] = f;
+    }
+    __syncthreads();
+    constexpr int CHUNKS = B >= 16 ? B / 16 : 1;
+    constexpr int WORDS = B >= 16 ? 4 : 2;
+    for (int q = threadIdx.x; q < W * CHUNKS; q += BAND_COST_BLOCK) {
+        const int x = q / CHUNKS, chunk = q - x * CHUNKS;
+        const int first = x - searched_of(base[(size_t)y * W + x]) - chunk * 16;
+        const uint64_t l = cl[x];
+        uint32_t words[WORDS];
+#pragma unroll
+        for (int w = 0; w < WORDS; w++) {
+            uint32_t word = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int column = first - 4 * w - j;
+                const uint64_t r = column >= 0 ? cr[column] : 0;
+                word |= (uint32_t)__builtin_popcountll(l ^ r) << (8 * j);
+            }
+            words[w] = word;
+        }
+        uint8_t *out = costs + ((size_t)y * W + x) * B + chunk * 16;
+        if constexpr (WORDS == 4)
+            *reinterpret_cast<uint4 *>(out) = make_uint4(words[0], words[1], words[2], words[3]);
+        else
+            *reinterpret_cast<uint2 *>(out) = make_uint2(words[0], words[1]);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Band aggregation (nx_stereo.ex:346-419, scan_band), one direction per line. Lane k of a
+// line holds L(p, base(p) + k). Written unnormalised like the full search: with v the
+// previous pixel's values, m = min(v) and delta = base(p) - base(previous),
+//   part[j] = min(v[j], min(v[j-1], v[j+1]) + p1)     (v outside the band: a sentinel)
+//   L(p, k) = min(part[k + delta], m + p2) - m + C(p, k)   for 0 <= k + delta < band,
+//             min(v[0] + p1, m + p2) - m + C               for k + delta = -1,
+//             min(v[band-1] + p1, m + p2) - m + C          for k + delta = band,
+//             p2 + C                                       further out,
+// which is scan_band's min(window, p2, neighbours + p1) over the moved, normalised values.
+// A line entering the image starts from v = 0 in its own band (delta 0). The move is a
+// ds_bpermute, taken only when a line of the wave changes base.
+
+// Neighbouring lanes of one line and the minimum over its B lanes (B = 8, 16, 32 or 64).
+template <int B>
+struct BandLanes {
+    static __device__ __forceinline__ uint32_t from_left(uint32_t edge, uint32_t v, int lane) {
+        const uint32_t t = B <= 16 ? dpp_or<DPP_ROW_SHR1>(edge, v) : dpp_or<DPP_WAVE_SHR1>(edge, v);
+        return (B == 8 || B == 32) && lane == 0 ? edge : t;
+    }
+    static __device__ __forceinline__ uint32_t from_right(uint32_t edge, uint32_t v, int lane) {
+        const uint32_t t = B <= 16 ? dpp_or<DPP_ROW_SHL1>(edge, v) : dpp_or<DPP_WAVE_SHL1>(edge, v);
+        return (B == 8 || B == 32) && lane == B - 1 ? edge : t;
+    }
+    // quad_perm [1,0,3,2] and [2,3,0,1] (the quad), row_half_mirror (8 lanes), row_mirror
+    // (16 lanes), then rows 2r and 2r+1, then the wave's halves.
+    static __device__ __forceinline__ uint32_t min(uint32_t v) {
+        v = ::min(v, dpp<0xB1>(v));
+        v = ::min(v, dpp<0x4E>(v));
+        v = ::min(v, dpp<0x141>(v));
+        if constexpr (B >= 16) v = ::min(v, dpp<0x140>(v));
+        if constexpr (B >= 32) v = pair_rows_min(v);
+        if constexpr (B == 64) v = halves_min(v);
+        return v;
+    }
+};
+
+struct BandPathArgs {
+    const uint8_t *costs;
+    const int32_t *base;
+    uint8_t *paths[8];  // one volume per direction launched
+    int W, H, p1, p2;
+    int directions;     // 3 or 8
+    int block_start[9]; // direction r's first block; block_start[directions] = the grid
+};
+
+// Direction r: orientation r >> 1 (rows, columns, x-y diagonals, x+y diagonals), backward
+// when r is odd: (0,1) (0,-1) (1,0) (-1,0) (1,1) (-1,-1) (1,-1) (-1,1) in (dy, dx).
+__host__ __device__ __forceinline__ int band_orientation(int r) { return r >> 1; }
+
+constexpr int BAND_PATH_BLOCK = 64;
+
+template <int B, int PREFETCH>
+__global__ __launch_bounds__(BAND_PATH_BLOCK) void band_path_kernel(BandPathArgs a) {
+    constexpr int G = BAND_PATH_BLOCK / B;  // lines per wave
+    constexpr uint32_t SENT = 0xFFFFu;       // a value no candidate takes
+    const int lane = threadIdx.x % B, group = threadIdx.x / B;
+    // The direction, its first block and its volume, by constant indices (a dynamic index
+    // into the argument struct would copy it to scratch).
+    int r = 0, start = 0;
+    uint8_t *volume_of_r = a.paths[0];
+#pragma unroll
+    for (int i = 1; i < 8; i++)
+        if (i < a.directions && (int)blockIdx.x >= a.block_start[i]) {
+            r = i;
+            start = a.block_start[i];
+            volume_of_r = a.paths[i];
+        }
+    const int o = band_orientation(r);
+    const bool backward = r & 1;
+    const int W = a.W, H = a.H;
+    const int first_line = ((int)blockIdx.x - start) * G;
+    const LineGeometry g = line_geometry(o, first_line + group, W, H);
+    int longest = 0;
+#pragma unroll
+    for (int i = 0; i < G; i++) longest = max(longest, line_geometry(o, first_line + i, W, H).length);
+    if (longest == 0) return;  // wave-uniform
+
+    const int dy = o == 0 ? 0 : 1;
+    const int dx = o == 0 ? 1 : (o == 1 ? 0 : (o == 2 ? 1 : -1));
+    const int L = g.length;
+    int pixel = g.y0 * W + g.x0, step = dy * W + dx;
+    if (backward) {
+        pixel += max(L - 1, 0) * step;
+        step = -step;
+    }
+    // Byte offsets (all under 2^31): this lane's cost and path byte, the pixel's base.
+    const uint32_t volume = (uint32_t)W * H * B;
+    const Rsrc costs = buffer_of(a.costs, volume);
+    const Rsrc bases = buffer_of(a.base, (uint32_t)W * H * 4);
+    const Rsrc out = buffer_of(volume_of_r, volume);
+    const uint32_t p1 = (uint32_t)a.p1, p2 = (uint32_t)a.p2;
+    const int bstep = step * B;
+    uint32_t at = (uint32_t)(pixel * B + lane);
+    uint32_t ahead = at;
+    uint32_t at_base = (uint32_t)pixel * 4u;
+    uint32_t ahead_base = at_base;
+
+    uint32_t cost_ring[PREFETCH];
+    int base_ring[PREFETCH];
+#pragma unroll
+    for (int p = 0; p < PREFETCH; p++) {
+        const bool in = p < L;
+        cost_ring[p] = __builtin_amdgcn_raw_buffer_load_b8(costs, in ? ahead : MASKED, 0, 0);
+        base_ring[p] = (int)__builtin_amdgcn_raw_buffer_load_b32(bases, in ? ahead_base : MASKED, 0, 0);
+        ahead += bstep;
+        ahead_base += step * 4;
+    }
+    uint32_t v = 0;  // the previous pixel's values: 0 entering the image
+    int previous = searched_of(base_ring[0]);
+    for (int t0 = 0; t0 < longest; t0 += PREFETCH) {
+#pragma unroll
+        for (int p = 0; p < PREFETCH; p++) {
+            const int t = t0 + p;
+            __builtin_amdgcn_sched_barrier(0);
+            const bool active = t < L;
+            const int b = searched_of(base_ring[p]);
+            const uint32_t c = cost_ring[p];
+            const bool next = t + PREFETCH < L;
+            cost_ring[p] = __builtin_amdgcn_raw_buffer_load_b8(costs, next ? ahead : MASKED, 0, 0);
+            base_ring[p] = (int)__builtin_amdgcn_raw_buffer_load_b32(bases, next ? ahead_base : MASKED, 0, 0);
+            const uint32_t lo = BandLanes<B>::from_left(SENT, v, lane);
+            const uint32_t hi = BandLanes<B>::from_right(SENT, v, lane);
+            uint32_t sel = ::min(v, ::min(lo, hi) + p1);
+            const uint32_t m = BandLanes<B>::min(v);
+            const int delta = active ? b - previous : 0;
+            if (__builtin_amdgcn_ballot_w64(delta != 0)) {
+                const int moved = lane + delta;
+                const int from = min(max(moved, 0), B - 1);
+                const uint32_t packed = sel | ((v + p1) << 16);
+                const uint32_t f =
+                    (uint32_t)__builtin_amdgcn_ds_bpermute((group * B + from) * 4, (int)packed);
+                sel = (uint32_t)moved < (uint32_t)B ? (f & 0xFFFFu)
+                                                      : (moved == -1 || moved == B ? f >> 16 : SENT);
+            }
+            v = ::min(sel, m + p2) - m + c;
+            previous = b;
+            __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, out, active ? at : MASKED, 0, 0);
+            at += bstep;
+            ahead += bstep;
+            ahead_base += step * 4;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// nx_stereo.ex:431-471 (band_select), one pixel per thread: the sums of the directions'
+// bytes, winner (ties to the lower k), second best outside winner +-1, subpixel step in
+// f32 away from the band's ends, and the key (minimum << 16 | base + winner). A negative
+// base or a winner past x gives no match.
+
+struct BandSelectArgs {
+    const uint8_t *paths[8];
+    int directions;
+    const int32_t *base;
+    int W, H, D;
+    float uniqueness;
+    float *selected;
+    uint32_t *keys;
+    uint16_t *sums;  // optional: the sums, W*H*band u16
+};
+
+template <int B>
+__global__ __launch_bounds__(256) void band_select_kernel(BandSelectArgs a) {
+    const int W = a.W, D = a.D;
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= W * a.H) return;
+    const int x = p % W;
+    constexpr int WORDS = B / 4;
+    uint32_t s[B / 2];  // u16 pairs (2k, 2k+1)
+#pragma unroll
+    for (int k = 0; k < B / 2; k++) s[k] = 0;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        if (r >= a.directions) break;
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(a.paths[r] + (size_t)p * B);
+        uint32_t w[WORDS];
+        if constexpr (B == 8) {
+            const uint2 q = *reinterpret_cast<const uint2 *>(src);
+            w[0] = q.x;
+            w[1] = q.y;
+        } else {
+#pragma unroll
+            for (int c = 0; c < B / 16; c++) {
+                const uint4 q = reinterpret_cast<const uint4 *>(src)[c];
+                w[4 * c] = q.x;
+                w[4 * c + 1] = q.y;
+                w[4 * c + 2] = q.z;
+                w[4 * c + 3] = q.w;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < WORDS; k++) {
+            s[2 * k] = pk_add(s[2 * k], __builtin_amdgcn_perm(0u, w[k], 0x0C010C00u));
+            s[2 * k + 1] = pk_add(s[2 * k + 1], __builtin_amdgcn_perm(0u, w[k], 0x0C030C02u));
+        }
+    }
+    if (a.sums) {
+        uint32_t *dst = reinterpret_cast<uint32_t *>(a.sums + (size_t)p * B);
+#pragma unroll
+        for (int k = 0; k < B / 2; k++) dst[k] = s[k];
+    }
+    const int raw = a.base[p];
+    const int b = searched_of(raw);
+    int values[B];
+#pragma unroll
+    for (int k = 0; k < B; k++) {
+        const int sum = (int)((k & 1) ? s[k >> 1] >> 16 : s[k >> 1] & 0xFFFFu);
+        values[k] = sum + (b + k > x ? 0x8000 : 0);
+    }
+    int minimum = values[0], winner = 0;
+#pragma unroll
+    for (int k = 1; k < B; k++)
+        if (values[k] < minimum) {
+            minimum = values[k];
+            winner = k;
+        }
+    int second = 0x7FFFFFFF;
+    float lower = 0.0f, upper = 0.0f;
+#pragma unroll
+    for (int k = 0; k < B; k++) {
+        const int d = k - winner;
+        if (d > 1 || d < -1) second = min(second, values[k]);
+        if (k == max(winner - 1, 0)) lower = (float)values[k];
+        if (k == min(winner + 1, B - 1)) upper = (float)values[k];
+    }
+    const float minimum_f = (float)minimum;
+    const int winner_d = b + winner;
+    const bool accepted =
+        raw >= 0 && winner_d <= x && (float)second > minimum_f * a.uniqueness;
+    const float denominator = lower + upper - 2.0f * minimum_f;
+    float correction = (lower - upper) / (2.0f * fmaxf(denominator, 1.0f));
+    correction = fminf(fmaxf(correction, -0.5f), 0.5f);
+    const bool refine = denominator > 0.0f && winner > 0 && winner < B - 1 &&
+                        winner_d < min(D - 1, x);
+    const float value = (float)winner_d + (refine ? correction : 0.0f);
+    a.selected[p] = accepted ? value : -1.0f;
+    a.keys[p] = accepted ? ((uint32_t)minimum << 16) | (uint32_t)winner_d : INVALID_KEY;
+}
+
+// ------------------------------------------------------------------------------------------
+
+static size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static int band_directions(int paths) { return paths == 8 ? 8 : 3; }
+
+size_t band_bytes(int W, int H, int band, int paths) {
+    const size_t P = (size_t)W * H, N = P * band;
+    return up256(8 * P) * 2 + up256(N) * (1 + band_directions(paths)) + up256(4 * P) * 3;
+}
+
+void band_carve(BandBuffers &b, void *base, int W, int H, int band, int paths) {
+    const size_t P = (size_t)W * H, N = P * band;
+    uint8_t *p = static_cast<uint8_t *>(base);
+    b.census_left = reinterpret_cast<uint64_t *>(p);
+    p += up256(8 * P);
+    b.census_right = reinterpret_cast<uint64_t *>(p);
+    p += up256(8 * P);
+    b.costs = p;
+    p += up256(N);
+    for (int r = 0; r < 8; r++) {
+        if (r < band_directions(paths)) {
+            b.paths[r] = p;
+            p += up256(N);
+        } else {
+            b.paths[r] = nullptr;
+        }
+    }
+    b.selected = reinterpret_cast<float *>(p);
+    p += up256(4 * P);
+    b.keys = reinterpret_cast<uint32_t *>(p);
+    p += up256(4 * P);
+    b.checked = reinterpret_cast<float *>(p);
+}
+
+template <int WINDOW>
+static void launch_band_cost_window(const BandParams &p, const uint8_t *left,
+                                    const uint8_t *right, const int32_t *base, BandBuffers &b,
+                                    hipStream_t stream) {
+    uint64_t *cl = p.keep_stages ? b.census_left : nullptr;
+    uint64_t *cr = p.keep_stages ? b.census_right : nullptr;
+    const size_t lds = band_cost_lds(p.width);
+    const dim3 grid(p.height), block(BAND_COST_BLOCK);
+    switch (p.band) {
+        case 8: hipLaunchKernelGGL((band_cost_kernel<WINDOW, 8>), grid, block, lds, stream, left, right, base, p.width, p.height, b.costs, cl, cr); break;
+        case 16: hipLaunchKernelGGL((band_cost_kernel<WINDOW, 16>), grid, block, lds, stream, left, right, base, p.width, p.height, b.costs, cl, cr); break;
+        case 32: hipLaunchKernelGGL((band_cost_kernel<WINDOW, 32>), grid, block, lds, stream, left, right, base, p.width, p.height, b.costs, cl, cr); break;
+        default: hipLaunchKernelGGL((band_cost_kernel<WINDOW, 64>), grid, block, lds, stream, left, right, base, p.width, p.height, b.costs, cl, cr); break;
+    }
+}
+
+// Above 64 KB of LDS a kernel must say so first (band_cost_lds(4096) = 120 KB).
+template <int WINDOW, int B>
+static hipError_t allow_band_cost_lds() {
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(band_cost_kernel<WINDOW, B>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)band_cost_lds(MAX_IMAGE_WIDTH));
+}
+
+template <int WINDOW>
+static hipError_t allow_band_cost_window() {
+    hipError_t e = allow_band_cost_lds<WINDOW, 8>();
+    if (e == hipSuccess) e = allow_band_cost_lds<WINDOW, 16>();
+    if (e == hipSuccess) e = allow_band_cost_lds<WINDOW, 32>();
+    if (e == hipSuccess) e = allow_band_cost_lds<WINDOW, 64>();
+    return e;
+}
+
+template <int B>
+static void launch_band_paths(const BandPathArgs &a, int blocks, hipStream_t stream) {
+    hipLaunchKernelGGL((band_path_kernel<B, 8>), dim3(blocks), dim3(BAND_PATH_BLOCK), 0, stream, a);
+}
+
+template <int B>
+static void launch_band_select(const BandParams &p, const int32_t *base, BandBuffers &b,
+                               uint16_t *sums, hipStream_t stream) {
+    const int pixels = p.width * p.height;
+    BandSelectArgs a;
+    for (int r = 0; r < 8; r++) a.paths[r] = b.paths[r];
+    a.directions = band_directions(p.paths);
+    a.base = base;
+    a.W = p.width;
+    a.H = p.height;
+    a.D = p.disparities;
+    a.uniqueness = p.uniqueness;
+    a.selected = b.selected;
+    a.keys = b.keys;
+    a.sums = sums;
+    hipLaunchKernelGGL(band_select_kernel<B>, dim3((pixels + 255) / 256), dim3(256), 0, stream, a);
+}
+
+hipError_t launch_band(const BandParams &p, const uint8_t *left, const uint8_t *right,
+                       const int32_t *base, BandBuffers &b, float *disparity, uint16_t *sums,
+                       hipStream_t stream) {
+    const int W = p.width, H = p.height;
+    if (W > MAX_IMAGE_WIDTH || W < 1 || H < 1) return hipErrorInvalidValue;
+    if (p.band != 8 && p.band != 16 && p.band != 32 && p.band != 64) return hipErrorInvalidValue;
+    static const hipError_t lds_ok = [] {
+        hipError_t e = allow_band_cost_window<0>();
+        if (e == hipSuccess) e = allow_band_cost_window<1>();
+        if (e == hipSuccess) e = allow_band_cost_window<2>();
+        return e;
+    }();
+    if (lds_ok != hipSuccess) return lds_ok;
+
+    int slot = profile_begin(PK_BAND_COST, stream);
+    switch (p.census) {
+        case 1: launch_band_cost_window<1>(p, left, right, base, b, stream); break;
+        case 2: launch_band_cost_window<2>(p, left, right, base, b, stream); break;
+        default: launch_band_cost_window<0>(p, left, right, base, b, stream); break;
+    }
+    profile_end(slot, stream);
+
+    BandPathArgs a;
+    a.costs = b.costs;
+    a.base = base;
+    for (int r = 0; r < 8; r++) a.paths[r] = b.paths[r];
+    a.W = W;
+    a.H = H;
+    a.p1 = p.p1;
+    a.p2 = p.p2;
+    a.directions = band_directions(p.paths);
+    const int lines[4] = {H, W, W + H - 1, W + H - 1};
+    const int per_block = BAND_PATH_BLOCK / p.band;
+    int blocks = 0;
+    for (int r = 0; r < a.directions; r++) {
+        a.block_start[r] = blocks;
+        blocks += (lines[band_orientation(r)] + per_block - 1) / per_block;
+    }
+    for (int r = a.directions; r <= 8; r++) a.block_start[r] = blocks;
+    slot = profile_begin(PK_BAND_PATH, stream);
+    switch (p.band) {
+        case 8: launch_band_paths<8>(a, blocks, stream); break;
+        case 16: launch_band_paths<16>(a, blocks, stream); break;
+        case 32: launch_band_paths<32>(a, blocks, stream); break;
+        default: launch_band_paths<64>(a, blocks, stream); break;
+    }
+    profile_end(slot, stream);
+
+    slot = profile_begin(PK_BAND_SELECT, stream);
+    switch (p.band) {
+        case 8: launch_band_select<8>(p, base, b, sums, stream); break;
+        case 16: launch_band_select<16>(p, base, b, sums, stream); break;
+        case 32: launch_band_select<32>(p, base, b, sums, stream); break;
+        default: launch_band_select<64>(p, base, b, sums, stream); break;
+    }
+    profile_end(slot, stream);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_check_median(b.selected, b.keys, b.checked, disparity, W, H, p.keep_stages,
+                               p.positive, stream);
+}
+
+// ------------------------------------------------------------------------------------------
+// Coarse-to-fine (nx_stereo.ex:164-259).
+
+size_t coarse_to_fine_bytes(int W, int H, int band, int paths, int coarse_paths) {
+    const int cw = W / 2, ch = H / 2;
+    const size_t half = (size_t)cw * ch;
+    return up256(half) * 2 + match_bytes(cw, ch, 64, coarse_paths) + up256(4 * half) +
+           up256(4 * (size_t)W * H) + band_bytes(W, H, band, paths);
+}
+
+void coarse_to_fine_carve(CoarseBuffers &c, void *base, int W, int H, int band, int paths,
+                          int coarse_paths) {
+    const int cw = W / 2, ch = H / 2;
+    const size_t half = (size_t)cw * ch;
+    uint8_t *p = static_cast<uint8_t *>(base);
+    c.half_left = p;
+    p += up256(half);
+    c.half_right = p;
+    p += up256(half);
+    match_carve(c.match, p, cw, ch, 64, coarse_paths);
+    p += match_bytes(cw, ch, 64, coarse_paths);
+    c.coarse = reinterpret_cast<float *>(p);
+    p += up256(4 * half);
+    c.base = reinterpret_cast<int32_t *>(p);
+    p += up256(4 * (size_t)W * H);
+    band_carve(c.band, p, W, H, band, paths);
+}
+
+hipError_t launch_coarse_to_fine(const BandParams &fine, int margin, int coarse_paths,
+                                 const uint8_t *left, const uint8_t *right, CoarseBuffers &c,
+                                 float *disparity, hipStream_t stream) {
+    const int W = fine.width, H = fine.height, cw = W / 2, ch = H / 2;
+    if ((W | H) & 1) return hipErrorInvalidValue;
+    const int half = cw * ch;
+    int slot = profile_begin(PK_SHRINK, stream);
+    hipLaunchKernelGGL(shrink_kernel, dim3((half + 255) / 256, 2), dim3(256), 0, stream, left,
+                       right, W, H, c.half_left, c.half_right);
+    profile_end(slot, stream);
+    MatchParams coarse;
+    coarse.width = cw;
+    coarse.height = ch;
+    coarse.disparities = 64;
+    coarse.paths = coarse_paths;
+    coarse.p1 = fine.p1;
+    coarse.p2 = fine.p2;
+    coarse.uniqueness = fine.uniqueness;
+    coarse.keep_stages = false;
+    coarse.positive = false;  // sgm/3's own output: a match at 0 counts
+    hipError_t e = launch_match(coarse, c.half_left, c.half_right, c.match, c.coarse, nullptr, stream);
+    if (e != hipSuccess) return e;
+    slot = profile_begin(PK_SHRINK, stream);
+    hipLaunchKernelGGL(band_base_kernel, dim3((half + 255) / 256), dim3(256), 0, stream, c.coarse,
+                       cw, ch, fine.band, margin, c.base);
+    profile_end(slot, stream);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_band(fine, left, right, c.base, c.band, disparity, nullptr, stream);
+}
+
+}  // namespace nxs

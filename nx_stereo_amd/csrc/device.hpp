@@ -101,4 +101,35 @@ __device__ __forceinline__ uint32_t row_min_pk(uint32_t v) {
     return v;
 }
 
+// The lines of the aggregation: orientation o is 0 rows, 1 columns, 2 x-y diagonals, 3 x+y
+// diagonals. Line i's first pixel in the forward direction and its length (0 past the
+// last line).
+struct LineGeometry {
+    int y0, x0, length;
+};
+
+// Line i of orientation o, its first pixel in the forward direction and its length.
+__device__ __forceinline__ LineGeometry line_geometry(int o, int i, int W, int H) {
+    LineGeometry g;
+    if (o == 0) {  // row i, forward L->R
+        g.y0 = i;
+        g.x0 = 0;
+        g.length = i < H ? W : 0;
+    } else if (o == 1) {  // column i, forward down
+        g.y0 = 0;
+        g.x0 = i;
+        g.length = i < W ? H : 0;
+    } else if (o == 2) {  // x - y = i - (H-1), forward down-right
+        const int c = i - (H - 1);
+        g.y0 = max(0, -c);
+        g.x0 = g.y0 + c;
+        g.length = i < W + H - 1 ? min(H - g.y0, W - g.x0) : 0;
+    } else {  // x + y = i, forward down-left
+        g.y0 = max(0, i - (W - 1));
+        g.x0 = i - g.y0;
+        g.length = i < W + H - 1 ? min(H - g.y0, g.x0 + 1) : 0;
+    }
+    return g;
+}
+
 }  // namespace nxs

@@ -9,6 +9,7 @@
 namespace nxs {
 
 constexpr uint32_t INVALID_KEY = 0xFFFFFFFFu;
+constexpr int MAX_IMAGE_WIDTH = 4096;
 
 // Device buffers of one matching at W x H x D. `paired[o]` holds, per line
 // orientation o (rows, columns, x-y diagonals, x+y diagonals), the byte sum
@@ -30,6 +31,8 @@ struct MatchParams {
     int width, height, disparities, paths, p1, p2;
     float uniqueness;
     bool keep_stages = false;  // also store the census images and the checked map (stage checks)
+    bool positive = true;      // the Disparity stage's "> 0 else -1" after the median; false
+                               // leaves sgm/3's own output (a match at 0 stays 0)
 };
 
 // One pair of a matching launch: its images, output, workspace and live settings. A launch
@@ -74,6 +77,62 @@ hipError_t launch_match(const MatchParams &p, const uint8_t *left, const uint8_t
 // `p`) whose descriptors are `table` in device memory, each kernel one launch for all.
 hipError_t launch_match_batch(const MatchParams &p, const PairDesc *table, int count,
                               const Schedule &schedule, hipStream_t stream);
+
+// The left-right check and 3x3 median of a selected map and its keys (the last kernel of
+// every matching); `checked` receives the checked map when keep_checked.
+hipError_t launch_check_median(const float *selected, const uint32_t *keys, float *checked,
+                               float *out, int W, int H, bool keep_checked, bool positive,
+                               hipStream_t stream);
+
+// ---- Band search (sgm_band/4) and coarse-to-fine (sgm_coarse_to_fine/3) ----
+
+// Device buffers of one band search at W x H with `band` candidates per pixel: one byte
+// path volume per direction (3 or 8).
+struct BandBuffers {
+    uint64_t *census_left = nullptr, *census_right = nullptr;  // W*H, kept on request
+    uint8_t *costs = nullptr;                                  // W*H*band
+    uint8_t *paths[8] = {};                                    // W*H*band each
+    float *selected = nullptr;                                 // W*H
+    uint32_t *keys = nullptr;                                  // W*H
+    float *checked = nullptr;                                  // W*H
+};
+
+size_t band_bytes(int width, int height, int band, int paths);
+void band_carve(BandBuffers &b, void *base, int width, int height, int band, int paths);
+
+struct BandParams {
+    int width, height, disparities, band, paths;
+    int census;  // 0 sparse_9x7, 1 dense_9x5, 2 dense_9x5_blend
+    int p1, p2;
+    float uniqueness;
+    bool keep_stages = false;  // also store the census images and the checked map
+    bool positive = false;     // the Disparity stage's "> 0 else -1" after the median
+};
+
+// left/right u8 and base s32 {H,W} (a negative base: no estimate) -> disparity f32 {H,W}.
+// `sums` (optional, W*H*band u16) receives the aggregated path costs.
+hipError_t launch_band(const BandParams &p, const uint8_t *left, const uint8_t *right,
+                       const int32_t *base, BandBuffers &b, float *disparity, uint16_t *sums,
+                       hipStream_t stream);
+
+// Coarse-to-fine's own buffers: the half-size images, their match, the coarse map, the
+// bases and the fine band search.
+struct CoarseBuffers {
+    uint8_t *half_left = nullptr, *half_right = nullptr;  // (W/2)*(H/2)
+    MatchBuffers match;                                   // at (W/2, H/2, 64)
+    float *coarse = nullptr;                              // (W/2)*(H/2)
+    int32_t *base = nullptr;                              // W*H
+    BandBuffers band;
+};
+
+size_t coarse_to_fine_bytes(int width, int height, int band, int paths, int coarse_paths);
+void coarse_to_fine_carve(CoarseBuffers &c, void *base, int width, int height, int band,
+                          int paths, int coarse_paths);
+
+// `fine` at 128 disparities (even width and height); the coarse pass takes 64.
+hipError_t launch_coarse_to_fine(const BandParams &fine, int margin, int coarse_paths,
+                                 const uint8_t *left, const uint8_t *right, CoarseBuffers &c,
+                                 float *disparity, hipStream_t stream);
 
 // Camera model of the rectification stage: k[9], plumb_bob d[5], inverse_pr[9], all f32.
 struct Camera {

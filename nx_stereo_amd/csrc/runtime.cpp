@@ -1379,6 +1379,200 @@ int nxs_match_host(const uint8_t *left, const uint8_t *right, int width, int hei
     return rc;
 }
 
+namespace nxs {
+
+// sgm_band/4's checks (nx_stereo.ex:88-131, 286-308): the matching's, with 64 or 128
+// disparities, a band of 8, 16, 32 or 64 and a census window index.
+static void check_band(int W, int H, int D, int band, int paths, int census, int p1, int p2,
+                       float uniqueness) {
+    check_matching(W, H, D, paths, p1, p2, uniqueness);
+    if (D != 64 && D != 128) throw Error{"disparities must be 64 or 128, got " + std::to_string(D)};
+    if (band != 8 && band != 16 && band != 32 && band != 64)
+        throw Error{"band must be 8, 16, 32 or 64, got " + std::to_string(band)};
+    if (census < 0 || census > 2)
+        throw Error{"census must be 0 (sparse_9x7), 1 (dense_9x5) or 2 (dense_9x5_blend), got " +
+                    std::to_string(census)};
+    if (p1 < 0) throw Error{"p1 must be at least 0"};
+}
+
+// sgm_coarse_to_fine/3's (nx_stereo.ex:164-185): even sizes whose halves match at 64.
+static void check_coarse_to_fine(int W, int H, int band, int margin, int paths, int coarse_paths,
+                                 int census, int p1, int p2, float uniqueness) {
+    check_band(W, H, 128, band, paths, census, p1, p2, uniqueness);
+    if ((W | H) & 1)
+        throw Error{"coarse_to_fine expects an even height and width, got " + std::to_string(W) +
+                    "x" + std::to_string(H)};
+    check_matching(W / 2, H / 2, 64, coarse_paths, p1, p2, uniqueness);
+    if (margin < 0 || margin > 127)
+        throw Error{"margin must be an integer from 0 to 127, got " + std::to_string(margin)};
+}
+
+static BandParams band_params(int W, int H, int D, int band, int paths, int census, int p1,
+                              int p2, float uniqueness) {
+    BandParams bp;
+    bp.width = W;
+    bp.height = H;
+    bp.disparities = D;
+    bp.band = band;
+    bp.paths = paths;
+    bp.census = census;
+    bp.p1 = p1;
+    bp.p2 = p2;
+    bp.uniqueness = uniqueness;
+    return bp;
+}
+
+}  // namespace nxs
+
+size_t nxs_band_workspace_size(int width, int height, int band, int paths) {
+    return nxs::band_bytes(width, height, band, paths);
+}
+
+int nxs_match_band_device(const uint8_t *left, const uint8_t *right, const int32_t *base,
+                          int width, int height, int disparities, int band, int paths,
+                          int census, int p1, int p2, float uniqueness, float *disparity,
+                          void *workspace, size_t workspace_bytes, void *stream) {
+    try {
+        nxs::check_band(width, height, disparities, band, paths, census, p1, p2, uniqueness);
+    } catch (const nxs::Error &e) {
+        return nxs::fail(NXS_EINVAL, e.reason);
+    }
+    if (!left || !right || !base || !disparity || !workspace)
+        return nxs::fail(NXS_EINVAL, "a device pointer is null");
+    if (workspace_bytes < nxs::band_bytes(width, height, band, paths))
+        return nxs::fail(NXS_EINVAL, "the workspace is smaller than nxs_band_workspace_size");
+    nxs::BandBuffers b;
+    nxs::band_carve(b, workspace, width, height, band, paths);
+    const nxs::BandParams bp =
+        nxs::band_params(width, height, disparities, band, paths, census, p1, p2, uniqueness);
+    hipError_t e = nxs::launch_band(bp, left, right, base, b, disparity, nullptr, (hipStream_t)stream);
+    if (e != hipSuccess) return nxs::fail(NXS_EDEVICE, hipGetErrorString(e));
+    return NXS_OK;
+}
+
+int nxs_match_band_host(const uint8_t *left, const uint8_t *right, const int32_t *base,
+                        int width, int height, int disparities, int band, int paths, int census,
+                        int p1, int p2, float uniqueness, uint64_t *census_left,
+                        uint64_t *census_right, uint8_t *costs, uint16_t *sums, float *selected,
+                        uint32_t *keys, float *checked, float *disparity) {
+    try {
+        nxs::check_band(width, height, disparities, band, paths, census, p1, p2, uniqueness);
+    } catch (const nxs::Error &e) {
+        return nxs::fail(NXS_EINVAL, e.reason);
+    }
+    if (!left || !right || !base) return nxs::fail(NXS_EINVAL, "no input image or base");
+    const size_t P = (size_t)width * height, N = P * band;
+    const size_t ws = nxs::band_bytes(width, height, band, paths);
+    uint8_t *dev = nullptr;
+    const size_t total = ws + 2 * P + 4 * P + 4 * P + 2 * N + 1024;
+    hipError_t e = hipMalloc(&dev, total);
+    if (e != hipSuccess) return nxs::fail(NXS_ENOMEM, hipGetErrorString(e));
+    uint8_t *d_left = dev + ws, *d_right = d_left + P;
+    int32_t *d_base = reinterpret_cast<int32_t *>(nxs::up256_ptr(d_right + P));
+    float *d_out = reinterpret_cast<float *>(nxs::up256_ptr((uint8_t *)(d_base + P)));
+    uint16_t *d_sums = reinterpret_cast<uint16_t *>(nxs::up256_ptr((uint8_t *)(d_out + P)));
+    nxs::BandBuffers b;
+    nxs::band_carve(b, dev, width, height, band, paths);
+    nxs::BandParams bp =
+        nxs::band_params(width, height, disparities, band, paths, census, p1, p2, uniqueness);
+    bp.keep_stages = true;
+    int rc = NXS_OK;
+    auto check = [&](hipError_t err) {
+        if (err != hipSuccess && rc == NXS_OK) rc = nxs::fail(NXS_EDEVICE, hipGetErrorString(err));
+    };
+    check(hipMemcpy(d_left, left, P, hipMemcpyHostToDevice));
+    check(hipMemcpy(d_right, right, P, hipMemcpyHostToDevice));
+    check(hipMemcpy(d_base, base, 4 * P, hipMemcpyHostToDevice));
+    if (rc == NXS_OK)
+        check(nxs::launch_band(bp, d_left, d_right, d_base, b, d_out, sums ? d_sums : nullptr, nullptr));
+    check(hipDeviceSynchronize());
+    auto down = [&](void *host, const void *devp, size_t n) {
+        if (host && rc == NXS_OK) check(hipMemcpy(host, devp, n, hipMemcpyDeviceToHost));
+    };
+    down(census_left, b.census_left, 8 * P);
+    down(census_right, b.census_right, 8 * P);
+    down(costs, b.costs, N);
+    down(sums, d_sums, 2 * N);
+    down(selected, b.selected, 4 * P);
+    down(keys, b.keys, 4 * P);
+    down(checked, b.checked, 4 * P);
+    down(disparity, d_out, 4 * P);
+    (void)hipFree(dev);
+    return rc;
+}
+
+size_t nxs_coarse_to_fine_workspace_size(int width, int height, int band, int paths,
+                                         int coarse_paths) {
+    return nxs::coarse_to_fine_bytes(width, height, band, paths, coarse_paths);
+}
+
+int nxs_coarse_to_fine_device(const uint8_t *left, const uint8_t *right, int width, int height,
+                              int band, int margin, int paths, int coarse_paths, int census,
+                              int p1, int p2, float uniqueness, float *disparity, void *workspace,
+                              size_t workspace_bytes, void *stream) {
+    try {
+        nxs::check_coarse_to_fine(width, height, band, margin, paths, coarse_paths, census, p1,
+                                  p2, uniqueness);
+    } catch (const nxs::Error &e) {
+        return nxs::fail(NXS_EINVAL, e.reason);
+    }
+    if (!left || !right || !disparity || !workspace)
+        return nxs::fail(NXS_EINVAL, "a device pointer is null");
+    if (workspace_bytes < nxs::coarse_to_fine_bytes(width, height, band, paths, coarse_paths))
+        return nxs::fail(NXS_EINVAL, "the workspace is smaller than nxs_coarse_to_fine_workspace_size");
+    nxs::CoarseBuffers c;
+    nxs::coarse_to_fine_carve(c, workspace, width, height, band, paths, coarse_paths);
+    const nxs::BandParams bp =
+        nxs::band_params(width, height, 128, band, paths, census, p1, p2, uniqueness);
+    hipError_t e = nxs::launch_coarse_to_fine(bp, margin, coarse_paths, left, right, c, disparity,
+                                              (hipStream_t)stream);
+    if (e != hipSuccess) return nxs::fail(NXS_EDEVICE, hipGetErrorString(e));
+    return NXS_OK;
+}
+
+int nxs_coarse_to_fine_host(const uint8_t *left, const uint8_t *right, int width, int height,
+                            int band, int margin, int paths, int coarse_paths, int census, int p1,
+                            int p2, float uniqueness, uint8_t *half_left, uint8_t *half_right,
+                            float *coarse, int32_t *base, float *disparity) {
+    try {
+        nxs::check_coarse_to_fine(width, height, band, margin, paths, coarse_paths, census, p1,
+                                  p2, uniqueness);
+    } catch (const nxs::Error &e) {
+        return nxs::fail(NXS_EINVAL, e.reason);
+    }
+    if (!left || !right) return nxs::fail(NXS_EINVAL, "no input image");
+    const size_t P = (size_t)width * height, half = P / 4;
+    const size_t ws = nxs::coarse_to_fine_bytes(width, height, band, paths, coarse_paths);
+    uint8_t *dev = nullptr;
+    hipError_t e = hipMalloc(&dev, ws + 2 * P + 4 * P + 1024);
+    if (e != hipSuccess) return nxs::fail(NXS_ENOMEM, hipGetErrorString(e));
+    uint8_t *d_left = dev + ws, *d_right = d_left + P;
+    float *d_out = reinterpret_cast<float *>(nxs::up256_ptr(d_right + P));
+    nxs::CoarseBuffers c;
+    nxs::coarse_to_fine_carve(c, dev, width, height, band, paths, coarse_paths);
+    const nxs::BandParams bp =
+        nxs::band_params(width, height, 128, band, paths, census, p1, p2, uniqueness);
+    int rc = NXS_OK;
+    auto check = [&](hipError_t err) {
+        if (err != hipSuccess && rc == NXS_OK) rc = nxs::fail(NXS_EDEVICE, hipGetErrorString(err));
+    };
+    check(hipMemcpy(d_left, left, P, hipMemcpyHostToDevice));
+    check(hipMemcpy(d_right, right, P, hipMemcpyHostToDevice));
+    if (rc == NXS_OK)
+        check(nxs::launch_coarse_to_fine(bp, margin, coarse_paths, d_left, d_right, c, d_out, nullptr));
+    check(hipDeviceSynchronize());
+    auto down = [&](void *host, const void *devp, size_t n) {
+        if (host && rc == NXS_OK) check(hipMemcpy(host, devp, n, hipMemcpyDeviceToHost));
+    };
+    down(half_left, c.half_left, half);
+    down(half_right, c.half_right, half);
+    down(coarse, c.coarse, 4 * half);
+    down(base, c.base, 4 * P);
+    down(disparity, d_out, 4 * P);
+    (void)hipFree(dev);
+    return rc;
+}
+
 int nxs_run_device(nxs_ctx *ctx, const uint8_t *record, size_t len, const nxs_tensor *entries,
                    int n_entries, nxs_tensor *outputs, int max_outputs, int *n_outputs,
                    void *stream) {

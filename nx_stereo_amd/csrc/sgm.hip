@@ -386,34 +386,6 @@ struct PathState {
     }
 };
 
-struct LineGeometry {
-    int y0, x0, length;
-};
-
-// Line i of orientation o, its first pixel in the forward direction and its length.
-__device__ __forceinline__ LineGeometry line_geometry(int o, int i, int W, int H) {
-    LineGeometry g;
-    if (o == 0) {  // row i, forward L->R
-        g.y0 = i;
-        g.x0 = 0;
-        g.length = i < H ? W : 0;
-    } else if (o == 1) {  // column i, forward down
-        g.y0 = 0;
-        g.x0 = i;
-        g.length = i < W ? H : 0;
-    } else if (o == 2) {  // x - y = i - (H-1), forward down-right
-        const int c = i - (H - 1);
-        g.y0 = max(0, -c);
-        g.x0 = g.y0 + c;
-        g.length = i < W + H - 1 ? min(H - g.y0, W - g.x0) : 0;
-    } else {  // x + y = i, forward down-left
-        g.y0 = max(0, i - (W - 1));
-        g.x0 = i - g.y0;
-        g.length = i < W + H - 1 ? min(H - g.y0, g.x0 + 1) : 0;
-    }
-    return g;
-}
-
 struct PathArgs {
     PairDesc single;
     const PairDesc *table;
@@ -1090,7 +1062,7 @@ __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const Pair
 // the cheapest left match projected onto it, ties to the larger disparity, via an LDS
 // atomic minimum of (cost << 16) | (65535 - d) into the row's map.
 
-constexpr int MAX_WIDTH = 4096;
+constexpr int MAX_WIDTH = MAX_IMAGE_WIDTH;
 
 // The checked value of pixel x of a row: `sel` is the row's selected disparities and
 // `right` its map, already made.
@@ -1130,7 +1102,8 @@ constexpr int CHECK_BLOCK = 1024;
 
 __global__ __launch_bounds__(CHECK_BLOCK) void check_median_kernel(PairDesc single,
                                                                    const PairDesc *table, int W,
-                                                                   int H, int keep_checked) {
+                                                                   int H, int keep_checked,
+                                                                   int positive) {
     extern __shared__ __align__(16) uint32_t check_lds[];
     uint32_t *right = check_lds;                                    // [3][W]
     float *checked = reinterpret_cast<float *>(check_lds + 3 * W);  // [3][W]
@@ -1184,7 +1157,7 @@ __global__ __launch_bounds__(CHECK_BLOCK) void check_median_kernel(PairDesc sing
         CX(5, 8) CX(4, 7) CX(3, 6) CX(1, 4) CX(2, 5) CX(4, 7) CX(4, 2) CX(6, 4) CX(4, 2)
 #undef CX
         const float m = __int_as_float(v[4]);
-        out[x] = m > 0.0f ? m : -1.0f;
+        out[x] = !positive || m > 0.0f ? m : -1.0f;
     }
 }
 
@@ -1371,7 +1344,27 @@ static hipError_t launch_all(const MatchParams &p, const PairDesc &single, const
     if (check_lds_ok != hipSuccess) return check_lds_ok;
     slot = profile_begin(PK_CHECK_MEDIAN, stream);
     hipLaunchKernelGGL(check_median_kernel, dim3(H, count), dim3(CHECK_BLOCK), check_median_lds(W), stream,
-                       single, table, W, H, p.keep_stages ? 1 : 0);
+                       single, table, W, H, p.keep_stages ? 1 : 0, p.positive ? 1 : 0);
+    profile_end(slot, stream);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_median(const float *selected, const uint32_t *keys, float *checked,
+                               float *out, int W, int H, bool keep_checked, bool positive,
+                               hipStream_t stream) {
+    if (W > MAX_WIDTH || W < 1 || H < 1) return hipErrorInvalidValue;
+    static const hipError_t check_lds_ok = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(check_median_kernel),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)check_median_lds(MAX_WIDTH));
+    if (check_lds_ok != hipSuccess) return check_lds_ok;
+    PairDesc single{};
+    single.disparity = out;
+    single.b.selected = const_cast<float *>(selected);
+    single.b.keys = const_cast<uint32_t *>(keys);
+    single.b.checked = checked;
+    const int slot = profile_begin(PK_CHECK_MEDIAN, stream);
+    hipLaunchKernelGGL(check_median_kernel, dim3(H, 1), dim3(CHECK_BLOCK), check_median_lds(W),
+                       stream, single, nullptr, W, H, keep_checked ? 1 : 0, positive ? 1 : 0);
     profile_end(slot, stream);
     return hipGetLastError();
 }

@@ -112,7 +112,8 @@ class Context:
 
 NxsTensor = _native.NxsTensor
 
-PROFILED_KERNELS = ("census_cost", "path", "select", "check_median")
+PROFILED_KERNELS = ("census_cost", "path", "select", "check_median", "band_cost", "band_path",
+                    "band_select", "shrink")
 
 
 def profile_start(capacity: int = 4096) -> None:

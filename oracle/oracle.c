@@ -206,9 +206,10 @@ void orc_median3(const float *image, int width, int height, float *out) {
         }
 }
 
-/* reference.rs:256-277; then segment.rs:407-418 (a match at 0 is unmatched). */
-int orc_sgm(const uint8_t *left, const uint8_t *right, int width, int height, int D, int paths,
-            int p1, int p2, float uniqueness, float *disparity) {
+/* reference.rs:256-277; nx_stereo.ex:475-480: sgm/3's output, the median of the
+ * checked map (a match at 0 stays 0). */
+int orc_sgm_raw(const uint8_t *left, const uint8_t *right, int width, int height, int D,
+                int paths, int p1, int p2, float uniqueness, float *disparity) {
     size_t pixels = (size_t)width * height, volume = pixels * D;
     uint64_t *fl = (uint64_t *)malloc(8 * pixels), *fr = (uint64_t *)malloc(8 * pixels);
     uint8_t *costs = (uint8_t *)malloc(volume);
@@ -224,8 +225,6 @@ int orc_sgm(const uint8_t *left, const uint8_t *right, int width, int height, in
         orc_select(sums, width, height, D, uniqueness, selected, keys);
         orc_consistency(selected, keys, width, height, D, checked);
         orc_median3(checked, width, height, disparity);
-        for (size_t i = 0; i < pixels; i++)
-            if (!(disparity[i] > 0.0f)) disparity[i] = -1.0f;
     }
     free(fl);
     free(fr);
@@ -235,6 +234,276 @@ int orc_sgm(const uint8_t *left, const uint8_t *right, int width, int height, in
     free(checked);
     free(keys);
     return ok ? 0 : -1;
+}
+
+/* stage/disparity.ex:162; segment.rs:407-418: a match at 0 is unmatched. */
+static void positive_only(float *disparity, size_t pixels) {
+    for (size_t i = 0; i < pixels; i++)
+        if (!(disparity[i] > 0.0f)) disparity[i] = -1.0f;
+}
+
+int orc_sgm(const uint8_t *left, const uint8_t *right, int width, int height, int D, int paths,
+            int p1, int p2, float uniqueness, float *disparity) {
+    int rc = orc_sgm_raw(left, right, width, height, D, paths, p1, p2, uniqueness, disparity);
+    if (rc == 0) positive_only(disparity, (size_t)width * height);
+    return rc;
+}
+
+/* ---- Band search and coarse-to-fine (nx_stereo.ex:164-471) ---- */
+
+/* nx_stereo.ex:236-247: the rounded 2x2 average, (sum + 2) / 4. */
+void orc_shrink(const uint8_t *image, int width, int height, uint8_t *out) {
+    int hw = width / 2, hh = height / 2;
+    for (int y = 0; y < hh; y++)
+        for (int x = 0; x < hw; x++) {
+            const uint8_t *a = image + (size_t)(2 * y) * width + 2 * x;
+            int sum = a[0] + a[1] + a[width] + a[width + 1];
+            out[(size_t)y * hw + x] = (uint8_t)((sum + 2) / 4);
+        }
+}
+
+/* nx_stereo.ex:490-525: bit k, row-major over the window's neighbours, is set when the
+ * clamped neighbour + 1 < the anchor. sparse_9x7 (window 0): dy in -3..3, dx in -4..4,
+ * both non-zero, anchor the centre (orc_census). dense_9x5 (1): dy in -2..2, dx in
+ * -4..4 but the centre. dense_9x5_blend (2): dense_9x5 with the anchor
+ * (sum of w(dy)w(dx) * neighbour over the 3x3, w = 1,2,1, + 16 * centre + 24) / 32. */
+void orc_census_window(const uint8_t *image, int width, int height, int window,
+                       uint64_t *features) {
+    if (window == 0) {
+        orc_census(image, width, height, features);
+        return;
+    }
+    static const int weight[3] = {1, 2, 1};
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < width; x++) {
+            int anchor = image[(size_t)y * width + x];
+            if (window == 2) {
+                int sum = 0;
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++)
+                        sum += weight[dy + 1] * weight[dx + 1] *
+                               image[(size_t)clampi(y + dy, 0, height - 1) * width +
+                                     clampi(x + dx, 0, width - 1)];
+                anchor = (sum + 16 * anchor + 24) / 32;
+            }
+            uint64_t f = 0;
+            int bit = 0;
+            for (int dy = -2; dy <= 2; dy++)
+                for (int dx = -4; dx <= 4; dx++) {
+                    if (dy == 0 && dx == 0) continue;
+                    int v = image[(size_t)clampi(y + dy, 0, height - 1) * width +
+                                  clampi(x + dx, 0, width - 1)];
+                    if (v + 1 < anchor) f |= (uint64_t)1 << bit;
+                    bit++;
+                }
+            features[(size_t)y * width + x] = f;
+        }
+}
+
+/* nx_stereo.ex:249-259: for each coarse pixel, the lowest match (>= 0) of its 3x3
+ * neighbourhood (outside the image: none); base = clip(floor(2 * lowest) - margin,
+ * 0, 128 - band), -1 without a match; each coarse pixel covers a 2x2 block. */
+void orc_band_base(const float *coarse, int cw, int ch, int band, int margin, int32_t *base) {
+    const float none = 1.0e9f;
+    int width = 2 * cw;
+    for (int y = 0; y < ch; y++)
+        for (int x = 0; x < cw; x++) {
+            float lowest = none;
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    int sy = y + dy, sx = x + dx;
+                    if (sy < 0 || sy >= ch || sx < 0 || sx >= cw) continue;
+                    float v = coarse[(size_t)sy * cw + sx];
+                    float m = v >= 0.0f ? v : none;
+                    if (m < lowest) lowest = m;
+                }
+            int32_t b = -1;
+            if (lowest < none) {
+                float start = floorf(lowest * 2.0f) - (float)margin;
+                b = clampi((int32_t)start, 0, 128 - band);
+            }
+            for (int j = 0; j < 2; j++)
+                for (int i = 0; i < 2; i++) base[(size_t)(2 * y + j) * width + 2 * x + i] = b;
+        }
+}
+
+/* nx_stereo.ex:324-344: costs[y,x,k] = popcount(FL[y,x] ^ FR[y, x - base - k]), FR = 0
+ * left of the image; `searched` is the non-negative base. */
+void orc_band_costs(const uint64_t *left, const uint64_t *right, const int32_t *searched,
+                    int width, int height, int band, uint8_t *costs) {
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < width; x++) {
+            size_t at = (size_t)y * width + x;
+            for (int k = 0; k < band; k++) {
+                int column = x - searched[at] - k;
+                uint64_t r = column >= 0 ? right[(size_t)y * width + column] : 0;
+                costs[at * band + k] = (uint8_t)__builtin_popcountll(left[at] ^ r);
+            }
+        }
+}
+
+/* nx_stereo.ex:379-419 (scan_band): orc_path where the previous pixel's normalized costs
+ * are moved to this pixel's band by disparity; a candidate the previous band lacks is p2.
+ * A path entering the image starts from zero in its own band. */
+void orc_band_path(const uint8_t *costs, const int32_t *searched, int width, int height,
+                   int band, int dy, int dx, int p1, int p2, int32_t *out) {
+    int32_t *normalized = (int32_t *)malloc(sizeof(int32_t) * band);
+    for (int yi = 0; yi < height; yi++) {
+        int y = dy < 0 ? height - 1 - yi : yi;
+        for (int xi = 0; xi < width; xi++) {
+            int x = dx < 0 ? width - 1 - xi : xi;
+            int py = y - dy, px = x - dx;
+            int inside = py >= 0 && py < height && px >= 0 && px < width;
+            size_t at = (size_t)y * width + x;
+            const int32_t *previous = inside ? out + ((size_t)py * width + px) * band : NULL;
+            int delta = previous ? searched[at] - searched[(size_t)py * width + px] : 0;
+            int32_t minimum = 0;
+            if (previous) {
+                minimum = previous[0];
+                for (int k = 1; k < band; k++)
+                    if (previous[k] < minimum) minimum = previous[k];
+            }
+            for (int k = 0; k < band; k++) normalized[k] = previous ? previous[k] - minimum : 0;
+            for (int k = 0; k < band; k++) {
+                int32_t window[3];
+                for (int j = 0; j < 3; j++) {
+                    int index = k - 1 + j + delta;
+                    window[j] = index >= 0 && index < band ? normalized[index] : p2;
+                }
+                int32_t v = window[1] < p2 ? window[1] : p2;
+                int32_t n = (window[0] < window[2] ? window[0] : window[2]) + p1;
+                if (n < v) v = n;
+                out[at * band + k] = v + costs[at * band + k];
+            }
+        }
+    }
+    free(normalized);
+}
+
+/* nx_stereo.ex:346-377: the 3 or 8 directions of orc_aggregate, summed. */
+void orc_band_aggregate(const uint8_t *costs, const int32_t *searched, int width, int height,
+                        int band, int paths, int p1, int p2, uint16_t *sums) {
+    static const int directions[8][2] = {{0, 1}, {0, -1}, {1, 0},   {-1, 0},
+                                         {1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
+    size_t volume = (size_t)width * height * band;
+    int32_t *path = (int32_t *)malloc(sizeof(int32_t) * volume);
+    memset(sums, 0, sizeof(uint16_t) * volume);
+    int count = paths == 8 ? 8 : 3;
+    for (int r = 0; r < count; r++) {
+        orc_band_path(costs, searched, width, height, band, directions[r][0], directions[r][1],
+                      p1, p2, path);
+        for (size_t i = 0; i < volume; i++) sums[i] = (uint16_t)(sums[i] + path[i]);
+    }
+    free(path);
+}
+
+/* nx_stereo.ex:431-471 (band_select): orc_select within each pixel's band; the winner's
+ * disparity is base + k; a negative base, or a winner past x, gives no match; no
+ * refinement at the band's ends. */
+void orc_band_select(const uint16_t *sums, const int32_t *base, int width, int height, int band,
+                     int D, float uniqueness, float *disparity, uint32_t *keys) {
+    int32_t *values = (int32_t *)malloc(sizeof(int32_t) * band);
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < width; x++) {
+            size_t p = (size_t)y * width + x, at = p * band;
+            int estimated = base[p] >= 0;
+            int b = base[p] > 0 ? base[p] : 0;
+            for (int k = 0; k < band; k++) values[k] = sums[at + k] + (b + k > x ? 0x8000 : 0);
+            int32_t minimum = values[0];
+            int winner = 0;
+            for (int k = 1; k < band; k++)
+                if (values[k] < minimum) {
+                    minimum = values[k];
+                    winner = k;
+                }
+            int32_t second = INT32_MAX;
+            for (int k = 0; k < band; k++)
+                if (abs(k - winner) > 1 && values[k] < second) second = values[k];
+            float lower = (float)values[winner - 1 < 0 ? 0 : winner - 1];
+            float upper = (float)values[winner + 1 > band - 1 ? band - 1 : winner + 1];
+            float minimum_f32 = (float)minimum;
+            int winner_d = b + winner;
+            int accepted =
+                estimated && winner_d <= x && (float)second > minimum_f32 * uniqueness;
+            float denominator = lower + upper - 2.0f * minimum_f32;
+            float den1 = denominator > 1.0f ? denominator : 1.0f;
+            float correction = (lower - upper) / (2.0f * den1);
+            if (correction < -0.5f) correction = -0.5f;
+            if (correction > 0.5f) correction = 0.5f;
+            int limit = D - 1 < x ? D - 1 : x;
+            int refine = denominator > 0.0f && winner > 0 && winner < band - 1 && winner_d < limit;
+            float value = (float)winner_d + (refine ? correction : 0.0f);
+            uint32_t key = ((uint32_t)minimum << 16) | (uint32_t)winner_d;
+            disparity[p] = accepted ? value : -1.0f;
+            keys[p] = accepted ? key : ORC_INVALID_KEY;
+        }
+    free(values);
+}
+
+/* nx_stereo.ex:314-322 (band_n): band census, costs, aggregation and selection, then the
+ * left-right check over D and the 3x3 median. */
+int orc_sgm_band(const uint8_t *left, const uint8_t *right, const int32_t *base, int width,
+                 int height, int D, int band, int paths, int window, int p1, int p2,
+                 float uniqueness, float *disparity) {
+    size_t pixels = (size_t)width * height, volume = pixels * band;
+    uint64_t *fl = (uint64_t *)malloc(8 * pixels), *fr = (uint64_t *)malloc(8 * pixels);
+    int32_t *searched = (int32_t *)malloc(4 * pixels);
+    uint8_t *costs = (uint8_t *)malloc(volume);
+    uint16_t *sums = (uint16_t *)malloc(2 * volume);
+    float *selected = (float *)malloc(4 * pixels), *checked = (float *)malloc(4 * pixels);
+    uint32_t *keys = (uint32_t *)malloc(4 * pixels);
+    int ok = fl && fr && searched && costs && sums && selected && checked && keys;
+    if (ok) {
+        for (size_t i = 0; i < pixels; i++) searched[i] = base[i] > 0 ? base[i] : 0;
+        orc_census_window(left, width, height, window, fl);
+        orc_census_window(right, width, height, window, fr);
+        orc_band_costs(fl, fr, searched, width, height, band, costs);
+        orc_band_aggregate(costs, searched, width, height, band, paths, p1, p2, sums);
+        orc_band_select(sums, base, width, height, band, D, uniqueness, selected, keys);
+        orc_consistency(selected, keys, width, height, D, checked);
+        orc_median3(checked, width, height, disparity);
+    }
+    free(fl);
+    free(fr);
+    free(searched);
+    free(costs);
+    free(sums);
+    free(selected);
+    free(checked);
+    free(keys);
+    return ok ? 0 : -1;
+}
+
+/* nx_stereo.ex:164-214 (sgm_coarse_to_fine): shrink both images, match them at 64
+ * disparities with coarse_paths, take each pixel's band base from that match, then
+ * sgm_band at 128 disparities; with positive, the Disparity stage's rule after it
+ * (stage/disparity.ex:162). Even width and height. Returns 0, or -1 when memory runs
+ * out. */
+int orc_coarse_to_fine(const uint8_t *left, const uint8_t *right, int width, int height,
+                       int band, int margin, int paths, int coarse_paths, int window, int p1,
+                       int p2, float uniqueness, int positive, float *disparity) {
+    int cw = width / 2, ch = height / 2;
+    size_t pixels = (size_t)width * height, half = (size_t)cw * ch;
+    uint8_t *hl = (uint8_t *)malloc(half), *hr = (uint8_t *)malloc(half);
+    float *coarse = (float *)malloc(4 * half);
+    int32_t *base = (int32_t *)malloc(4 * pixels);
+    int rc = hl && hr && coarse && base ? 0 : -1;
+    if (rc == 0) {
+        orc_shrink(left, width, height, hl);
+        orc_shrink(right, width, height, hr);
+        rc = orc_sgm_raw(hl, hr, cw, ch, 64, coarse_paths, p1, p2, uniqueness, coarse);
+    }
+    if (rc == 0) {
+        orc_band_base(coarse, cw, ch, band, margin, base);
+        rc = orc_sgm_band(left, right, base, width, height, 128, band, paths, window, p1, p2,
+                          uniqueness, disparity);
+    }
+    if (rc == 0 && positive) positive_only(disparity, pixels);
+    free(hl);
+    free(hr);
+    free(coarse);
+    free(base);
+    return rc;
 }
 
 /* rectify.rs:107-128: the projection, operation for operation. */

@@ -28,6 +28,18 @@
  *   orc_sharpness    - measures.rs:168-184 and diagnostics_test.exs:17-53 closed forms
  *   orc_alignment    - measures.rs:186-239 (the moved offset, disparity 0, configured
  *                      bands/reach/rows) and diagnostics_test.exs:79-95
+ *   orc_shrink       - closed forms (constant, rounding of +2) and a reshape-sum route
+ *   orc_census_window- sparse = orc_census; dense/blend brute force against an
+ *                      ndimage (scipy) anchor and window, constant images give 0
+ *   orc_band_base    - ndimage minimum_filter route; no-match and clip cases
+ *   orc_band_costs   - equals orc_costs shifted by a constant base; column < 0 reads 0
+ *   orc_band_path    - orc_path (pinned above) on a full-range volume whose costs are 255
+ *                      outside each pixel's band, with random varying bases, 8 directions
+ *   orc_band_select  - constant base b: orc_select on the band volume at x - b, plus b;
+ *                      a full-range volume with 0xFFFF outside the band (edges unrefined)
+ *   orc_sgm_band     - nx_stereo_test.exs:162-225 (band 64 at base 0 equals sgm/3 with
+ *                      3 and 8 paths; bands around the full match agree; base -1 gives -1)
+ *   orc_coarse_to_fine - nx_stereo_test.exs:115-147 (shifted texture, depth edges)
  */
 #ifndef NXS_ORACLE_H
 #define NXS_ORACLE_H
@@ -59,8 +71,11 @@ void orc_consistency(const float *disparity, const uint32_t *keys, int width, in
                      int disparities, float *out);
 /* reference.rs:237-254 */
 void orc_median3(const float *image, int width, int height, float *out);
-/* reference.rs:256-277 then the Disparity stage's "> 0 else -1" (segment.rs:407-418).
- * Returns 0, or -1 when memory runs out. */
+/* reference.rs:256-277 (sgm/3's output, median of the checked map). Returns 0, or -1
+ * when memory runs out. */
+int orc_sgm_raw(const uint8_t *left, const uint8_t *right, int width, int height,
+                int disparities, int paths, int p1, int p2, float uniqueness, float *disparity);
+/* orc_sgm_raw then the Disparity stage's "> 0 else -1" (segment.rs:407-418). */
 int orc_sgm(const uint8_t *left, const uint8_t *right, int width, int height, int disparities,
             int paths, int p1, int p2, float uniqueness, float *disparity);
 
@@ -94,6 +109,36 @@ void orc_sharpness(const uint8_t *left, const uint8_t *right, int width, int hei
 /* measures.rs:53-108 (Alignment, alignment_sums): bands rows of 2 reach + 2 sums. */
 void orc_alignment(const uint8_t *left, const uint8_t *right, const float *disparity, int width,
                    int height, int bands, int reach, int row_step, float *sums);
+
+/* ---- Band search and coarse-to-fine (nx_stereo.ex:164-471) ---- */
+
+/* nx_stereo.ex:236-247: {height/2, width/2}, the rounded 2x2 average. */
+void orc_shrink(const uint8_t *image, int width, int height, uint8_t *out);
+/* nx_stereo.ex:490-525: window 0 sparse_9x7, 1 dense_9x5, 2 dense_9x5_blend. */
+void orc_census_window(const uint8_t *image, int width, int height, int window,
+                       uint64_t *features);
+/* nx_stereo.ex:249-259: coarse f32 {ch, cw} -> base s32 {2ch, 2cw}, -1 without a match. */
+void orc_band_base(const float *coarse, int cw, int ch, int band, int margin, int32_t *base);
+/* nx_stereo.ex:324-344; `searched` = max(base, 0); costs u8 {H,W,band}. */
+void orc_band_costs(const uint64_t *left, const uint64_t *right, const int32_t *searched,
+                    int width, int height, int band, uint8_t *costs);
+/* nx_stereo.ex:379-419: one direction (dy, dx), values as i32 {H,W,band}. */
+void orc_band_path(const uint8_t *costs, const int32_t *searched, int width, int height,
+                   int band, int dy, int dx, int p1, int p2, int32_t *out);
+/* nx_stereo.ex:346-377 */
+void orc_band_aggregate(const uint8_t *costs, const int32_t *searched, int width, int height,
+                        int band, int paths, int p1, int p2, uint16_t *sums);
+/* nx_stereo.ex:431-471; `base` may be negative (no estimate). */
+void orc_band_select(const uint16_t *sums, const int32_t *base, int width, int height, int band,
+                     int disparities, float uniqueness, float *disparity, uint32_t *keys);
+/* nx_stereo.ex:314-322 (band_n), sgm_band/4's output. */
+int orc_sgm_band(const uint8_t *left, const uint8_t *right, const int32_t *base, int width,
+                 int height, int disparities, int band, int paths, int window, int p1, int p2,
+                 float uniqueness, float *disparity);
+/* nx_stereo.ex:164-214; positive: also the stage's "> 0 else -1". */
+int orc_coarse_to_fine(const uint8_t *left, const uint8_t *right, int width, int height,
+                       int band, int margin, int paths, int coarse_paths, int window, int p1,
+                       int p2, float uniqueness, int positive, float *disparity);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,19 @@ def lib() -> ctypes.CDLL:
         _lib.orc_median.argtypes = [_P, i, i, i, _P]
         _lib.orc_sharpness.argtypes = [_P, _P, i, i, _P]
         _lib.orc_alignment.argtypes = [_P, _P, _P, i, i, i, i, i, _P]
+        _lib.orc_sgm_raw.argtypes = [_P, _P, i, i, i, i, i, i, f, _P]
+        _lib.orc_sgm_raw.restype = i
+        _lib.orc_shrink.argtypes = [_P, i, i, _P]
+        _lib.orc_census_window.argtypes = [_P, i, i, i, _P]
+        _lib.orc_band_base.argtypes = [_P, i, i, i, i, _P]
+        _lib.orc_band_costs.argtypes = [_P, _P, _P, i, i, i, _P]
+        _lib.orc_band_path.argtypes = [_P, _P, i, i, i, i, i, i, i, _P]
+        _lib.orc_band_aggregate.argtypes = [_P, _P, i, i, i, i, i, i, _P]
+        _lib.orc_band_select.argtypes = [_P, _P, i, i, i, i, f, _P, _P]
+        _lib.orc_sgm_band.argtypes = [_P, _P, _P, i, i, i, i, i, i, i, i, f, _P]
+        _lib.orc_sgm_band.restype = i
+        _lib.orc_coarse_to_fine.argtypes = [_P, _P, i, i, i, i, i, i, i, i, i, f, i, _P]
+        _lib.orc_coarse_to_fine.restype = i
     return _lib
 
 
@@ -177,6 +190,113 @@ def sgm(left, right, disparities=64, paths=8, p1=8, p2=64, uniqueness=1.05263157
                        float(np.float32(uniqueness)), _ptr(out))
     if rc:
         raise MemoryError("oracle sgm ran out of memory")
+    return out
+
+
+def sgm_raw(left, right, disparities=64, paths=8, p1=8, p2=64, uniqueness=1.0526315789473684):
+    """sgm/3's own output (nx_stereo.ex:475-480): a match at 0 stays 0."""
+    left, right = _c(left, np.uint8), _c(right, np.uint8)
+    h, w = left.shape
+    out = np.empty((h, w), np.float32)
+    rc = lib().orc_sgm_raw(_ptr(left), _ptr(right), w, h, disparities, paths, p1, p2,
+                           float(np.float32(uniqueness)), _ptr(out))
+    if rc:
+        raise MemoryError("oracle sgm ran out of memory")
+    return out
+
+
+# Census windows of sgm_band/4 (nx_stereo.ex:28), by the index the C code takes.
+CENSUS_WINDOWS = ("sparse_9x7", "dense_9x5", "dense_9x5_blend")
+
+
+def shrink(image):
+    """nx_stereo.ex:236-247: the rounded 2x2 average."""
+    image = _c(image, np.uint8)
+    h, w = image.shape
+    out = np.empty((h // 2, w // 2), np.uint8)
+    lib().orc_shrink(_ptr(image), w, h, _ptr(out))
+    return out
+
+
+def census_window(image, window="sparse_9x7"):
+    image = _c(image, np.uint8)
+    h, w = image.shape
+    out = np.empty((h, w), np.uint64)
+    lib().orc_census_window(_ptr(image), w, h, CENSUS_WINDOWS.index(window), _ptr(out))
+    return out
+
+
+def band_base(coarse, band=16, margin=4):
+    """nx_stereo.ex:249-259: coarse f32 {h, w} -> base s32 {2h, 2w}."""
+    coarse = _c(coarse, np.float32)
+    ch, cw = coarse.shape
+    out = np.empty((2 * ch, 2 * cw), np.int32)
+    lib().orc_band_base(_ptr(coarse), cw, ch, band, margin, _ptr(out))
+    return out
+
+
+def band_costs(fl, fr, searched, band):
+    fl, fr, searched = _c(fl, np.uint64), _c(fr, np.uint64), _c(searched, np.int32)
+    h, w = fl.shape
+    out = np.empty((h, w, band), np.uint8)
+    lib().orc_band_costs(_ptr(fl), _ptr(fr), _ptr(searched), w, h, band, _ptr(out))
+    return out
+
+
+def band_path(costs_volume, searched, dy, dx, p1, p2):
+    c, searched = _c(costs_volume, np.uint8), _c(searched, np.int32)
+    h, w, band = c.shape
+    out = np.empty((h, w, band), np.int32)
+    lib().orc_band_path(_ptr(c), _ptr(searched), w, h, band, dy, dx, p1, p2, _ptr(out))
+    return out
+
+
+def band_aggregate(costs_volume, searched, paths, p1, p2):
+    c, searched = _c(costs_volume, np.uint8), _c(searched, np.int32)
+    h, w, band = c.shape
+    out = np.empty((h, w, band), np.uint16)
+    lib().orc_band_aggregate(_ptr(c), _ptr(searched), w, h, band, paths, p1, p2, _ptr(out))
+    return out
+
+
+def band_select(sums, base, disparities, uniqueness):
+    s, base = _c(sums, np.uint16), _c(base, np.int32)
+    h, w, band = s.shape
+    disp = np.empty((h, w), np.float32)
+    keys = np.empty((h, w), np.uint32)
+    lib().orc_band_select(_ptr(s), _ptr(base), w, h, band, disparities,
+                          float(np.float32(uniqueness)), _ptr(disp), _ptr(keys))
+    return disp, keys
+
+
+def sgm_band(left, right, base, disparities=64, band=16, paths=8, census="sparse_9x7", p1=8,
+             p2=64, uniqueness=1.0526315789473684):
+    """sgm_band/4 (nx_stereo.ex:286-322)."""
+    left, right, base = _c(left, np.uint8), _c(right, np.uint8), _c(base, np.int32)
+    h, w = left.shape
+    out = np.empty((h, w), np.float32)
+    rc = lib().orc_sgm_band(_ptr(left), _ptr(right), _ptr(base), w, h, disparities, band, paths,
+                            CENSUS_WINDOWS.index(census), p1, p2,
+                            float(np.float32(uniqueness)), _ptr(out))
+    if rc:
+        raise MemoryError("oracle sgm_band ran out of memory")
+    return out
+
+
+def coarse_to_fine(left, right, band=16, margin=4, paths=8, coarse_paths=None,
+                   fine_census="sparse_9x7", p1=8, p2=64, uniqueness=1.0526315789473684,
+                   positive=False):
+    """sgm_coarse_to_fine/3 (nx_stereo.ex:164-214); positive: the Disparity stage's output
+    (stage/disparity.ex:162), where a match at 0 is unmatched."""
+    left, right = _c(left, np.uint8), _c(right, np.uint8)
+    h, w = left.shape
+    out = np.empty((h, w), np.float32)
+    rc = lib().orc_coarse_to_fine(_ptr(left), _ptr(right), w, h, band, margin, paths,
+                                  coarse_paths or paths, CENSUS_WINDOWS.index(fine_census), p1,
+                                  p2, float(np.float32(uniqueness)), 1 if positive else 0,
+                                  _ptr(out))
+    if rc:
+        raise MemoryError("oracle coarse_to_fine ran out of memory")
     return out
 
 
