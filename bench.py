@@ -139,7 +139,7 @@ def large_pair_leg(runs: int = 10) -> dict:
         match()
     stream.synchronize()
     kernels = segment.profile_stop()
-    us = {k: v[0] / max(v[1], 1) * 1e3 for k, v in kernels.items()}
+    us = {k: v[0] / v[1] * 1e3 for k, v in kernels.items() if v[1]}
     pixels, n = w * h, w * h * d
     aggregation_us = us["path"] + us["select"]
     agg = (6 * n + 8 * pixels) / (aggregation_us * 1e-6) / 1e9
@@ -165,6 +165,103 @@ def large_pair_leg(runs: int = 10) -> dict:
         t = traffic["aggregation_traffic_bytes"]
         out["roofline"]["aggregation"]["traffic"] = t
         out["roofline"]["aggregation"]["traffic_gbs"] = round(t / (aggregation_us * 1e-6) / 1e9, 1)
+    return out
+
+
+def _segment_rate(members, frames, runs):
+    """Pairs/s of a plain pipeline segment at 728x544 on device-resident frames."""
+    from nx_stereo_amd import hip, segment
+
+    geometry = workload()[0]
+    context = segment.Context(0)
+    ok, seg = segment.HipSegment.build(members, geometry, context)
+    reply = context.request(seg.configure_record())
+    if reply is not True:
+        raise RuntimeError(f"configure refused: {reply}")
+    seg.configured_version = seg.version
+    record = seg.run_record({})
+    stream = hip.Stream()
+    entries = [("left_frame", 0, (HEIGHT, WIDTH), frames[0].ptr),
+               ("right_frame", 0, (HEIGHT, WIDTH), frames[1].ptr)]
+    for _ in range(5):
+        context.run_device(record, entries, stream=stream.handle)
+    hip.synchronize()
+    start, end = hip.Event(), hip.Event()
+    start.record(stream)
+    for _ in range(runs):
+        context.run_device(record, entries, stream=stream.handle)
+    end.record(stream)
+    end.synchronize()
+    ms = start.elapsed_ms(end) / runs
+    context.close()
+    return ms
+
+
+def coarse_to_fine_leg(runs: int = 20) -> dict:
+    """sgm_coarse_to_fine/3 (band 16, margin 4, 8 paths for both passes, sparse census)
+    through the matcher alone at 728x544 and 1920x1080, and the plain pipeline with the
+    Disparity stage's search coarse_to_fine at 728x544."""
+    from nx_stereo_amd import _native, hip, segment, stages
+
+    band, margin, paths = 16, 4, 8
+    out = {"settings": {"band": band, "margin": margin, "paths": paths, "coarse_paths": paths,
+                        "fine_census": "sparse_9x7", "disparities": 128}}
+    for w, h in ((WIDTH, HEIGHT), (1920, 1080)):
+        left, right = match_pair(w, h, 128, SEED + 5)
+        l, r, o = hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h)
+        ws = hip.Buffer(_native.coarse_to_fine_workspace_size(w, h, band, paths, paths))
+        stream = hip.Stream()
+
+        def match():
+            _native.coarse_to_fine_device(l.ptr, r.ptr, w, h, band, margin, paths, paths, 0, 8, 64,
+                                          1.0526315789473684, o.ptr, ws.ptr, ws.nbytes,
+                                          stream.handle)
+
+        for _ in range(3):
+            match()
+        stream.synchronize()
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        for _ in range(runs):
+            match()
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / runs
+        segment.profile_start(16 * runs + 16)
+        for _ in range(runs):
+            match()
+        stream.synchronize()
+        kernels = segment.profile_stop()
+        us = {k: v[0] / max(v[1], 1) * 1e3 for k, v in kernels.items() if v[1]}
+        per_pair = {k: v[0] / runs * 1e3 for k, v in kernels.items() if v[1]}
+        pixels, nb = w * h, w * h * band
+        # The fine aggregation: band paths + band selection, the band volume's 6 Nb + 8P.
+        agg_us = us["band_path"] + us["band_select"]
+        agg = (6 * nb + 8 * pixels) / (agg_us * 1e-6) / 1e9
+        out[f"{w}x{h}"] = {
+            "workload": f"one rectified planar pair {w}x{h}, disparities 4..120, "
+                        "sgm_coarse_to_fine/3 alone",
+            "ms_per_pair": round(ms, 4), "pairs_per_s": round(1e3 / ms, 2),
+            "kernel_us_per_launch": {k: round(v, 2) for k, v in us.items()},
+            "kernel_us_per_pair": {k: round(v, 2) for k, v in per_pair.items()},
+            "roofline": {"kernel": "band aggregation: band_path + band_select",
+                         "achieved": round(agg, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(agg / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes": 6 * nb + 8 * pixels},
+        }
+        for b in (l, r, o, ws):
+            b.free()
+        stream.destroy()
+    geometry, members, left_frame, right_frame = workload()
+    members = [(name, stages.Disparity(num_disparities=128, paths=paths, search="coarse_to_fine")
+                if name == "disparity" else stage) for name, stage in members]
+    frames = [hip.Buffer.of(f) for f in (left_frame, right_frame)]
+    ms = _segment_rate(members, frames, runs)
+    out["pipeline_728x544"] = {
+        "workload": "the plain pipeline of the headline with Disparity search coarse_to_fine",
+        "ms_per_pair": round(ms, 4), "pairs_per_s": round(1e3 / ms, 2)}
+    for f in frames:
+        f.free()
     return out
 
 
@@ -365,8 +462,9 @@ def main() -> int:
     parser.add_argument("--steps", type=int, default=200)
     parser.add_argument("--warmup", type=int, default=20)
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    parser.add_argument("--legs", default="large,batched,served",
-                        help="comma-separated extra legs: large, batched, served (or none)")
+    parser.add_argument("--legs", default="large,batched,served,coarse_to_fine",
+                        help="comma-separated extra legs: large, batched, served, "
+                             "coarse_to_fine (or none)")
     parser.add_argument("--kernels-only", choices=sorted(TRAFFIC_FILES),
                         help="run three matchings and the calibration copy, for rocprofv3 --pmc")
     parser.add_argument("--traffic", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT"),
@@ -480,7 +578,7 @@ def main() -> int:
             "traffic": None,
             "kernel": "aggregation: path_kernel + select_kernel",
             "algorithmic_bytes": algorithmic,
-            "kernel_us": {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in kernels.items()},
+            "kernel_us": {k: round(v[0] / v[1] * 1e3, 2) for k, v in kernels.items() if v[1]},
         },
     }
     traffic = load_traffic("728")
@@ -501,6 +599,8 @@ def main() -> int:
         result["batched"] = batched_leg()
     if "served" in legs:
         result["served"] = served_leg()
+    if "coarse_to_fine" in legs:
+        result["coarse_to_fine"] = coarse_to_fine_leg()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(geometry, members, left_frame, right_frame)
     print(json.dumps(result), flush=True)

@@ -14,6 +14,8 @@
 // by the difference (ds_bpermute within the line's lanes); a candidate the previous band
 // lacks is reached only by the p2 jump (or, one past its ends, from the end + p1). Path
 // values stay <= p2 + 48 <= 127, so every direction stores bytes and selection adds them.
+#include <type_traits>
+
 #include "device.hpp"
 #include "kernels.hpp"
 #include "profile.hpp"
@@ -193,15 +195,16 @@ struct BandLanes {
         const uint32_t t = B <= 16 ? dpp_or<DPP_ROW_SHL1>(edge, v) : dpp_or<DPP_WAVE_SHL1>(edge, v);
         return (B == 8 || B == 32) && lane == B - 1 ? edge : t;
     }
-    // quad_perm [1,0,3,2] and [2,3,0,1] (the quad), row_half_mirror (8 lanes), row_mirror
-    // (16 lanes), then rows 2r and 2r+1, then the wave's halves.
-    static __device__ __forceinline__ uint32_t min(uint32_t v) {
-        v = ::min(v, dpp<0xB1>(v));
-        v = ::min(v, dpp<0x4E>(v));
-        v = ::min(v, dpp<0x141>(v));
-        if constexpr (B >= 16) v = ::min(v, dpp<0x140>(v));
-        if constexpr (B >= 32) v = pair_rows_min(v);
-        if constexpr (B == 64) v = halves_min(v);
+    // The maximum over the line's lanes: quad_perm [1,0,3,2] and [2,3,0,1] (the quad),
+    // row_half_mirror (8 lanes), row_mirror (16 lanes), then rows 2r and 2r+1, then the
+    // wave's halves.
+    static __device__ __forceinline__ uint32_t max(uint32_t v) {
+        v = ::max(v, dpp<0xB1>(v));
+        v = ::max(v, dpp<0x4E>(v));
+        v = ::max(v, dpp<0x141>(v));
+        if constexpr (B >= 16) v = ::max(v, dpp<0x140>(v));
+        if constexpr (B >= 32) v = pair_rows_max(v);
+        if constexpr (B == 64) v = halves_max(v);
         return v;
     }
 };
@@ -221,10 +224,66 @@ __host__ __device__ __forceinline__ int band_orientation(int r) { return r >> 1;
 
 constexpr int BAND_PATH_BLOCK = 64;
 
+// One line's recurrence in B lanes, on u = K - L (K = 1024 above every path value): the
+// minima become maxima, so a DPP move that reads past the band's end and takes 0 is
+// neutral, and the neighbour terms fold into DPP-sourced max instructions.
+template <int B>
+struct BandLine {
+    static constexpr uint32_t K = 0x400;
+    uint32_t u = K;  // K - the previous pixel's values (0 entering the image)
+    int delta = 0;   // the generic walk's base change at the coming step
+    bool moving = false;
+
+    // part[k] = min(v[k], min(v[k-1], v[k+1]) + p1) and the line's minimum m come from v
+    // alone; the candidate k + delta of the previous band is part moved by delta (a DPP
+    // move for one, where the band's end lane takes its own v + p1; ds_bpermute beyond;
+    // further out, nothing: p2). Every candidate is >= m, so
+    // min(sel, m + p2) - m = min(sel - m, p2). In u: sel - m = u(m) - u(sel).
+    // `moving`: a line of the wave changes base here; `delta_of()` its change (called only
+    // then).
+    template <class DeltaOf>
+    __device__ __forceinline__ uint32_t step(uint32_t c, int lane, int group, uint32_t p1,
+                                             uint32_t p2, bool moving_here, DeltaOf delta_of) {
+        const uint32_t hi = BandLanes<B>::from_right(0u, u, lane);
+        const uint32_t near = ::max(BandLanes<B>::from_left(0u, u, lane), hi);
+        const uint32_t part = ::max(u, near - p1);  // near >= K - 127 > p1
+        const uint32_t m = BandLanes<B>::max(u);
+        uint32_t sel = part;
+        // Rare (a base changes on under 1% of steps in smooth scenes): laid out away from the
+        // common step, which then falls through without a taken branch.
+        if (__builtin_expect(moving_here, false)) {
+            const int d = delta_of();
+            const uint32_t vp = u - p1;
+            const uint32_t up = BandLanes<B>::from_right(vp, part, lane);
+            const uint32_t down = BandLanes<B>::from_left(vp, part, lane);
+            sel = d == 1 ? up : (d == -1 ? down : part);
+            if (__builtin_amdgcn_ballot_w64(d > 1 || d < -1)) {
+                const int moved = lane + d;
+                const int from = min(max(moved, 0), B - 1);
+                const uint32_t f = (uint32_t)__builtin_amdgcn_ds_bpermute(
+                    (group * B + from) * 4, (int)(part | (vp << 16)));
+                const uint32_t moved_sel =
+                    (uint32_t)moved < (uint32_t)B ? (f & 0xFFFFu)
+                                                  : (moved == -1 || moved == B ? f >> 16 : 0u);
+                sel = d > 1 || d < -1 ? moved_sel : sel;
+            }
+        }
+        const uint32_t e = ::min(m - sel, p2);
+        u = (K - c) - e;
+        return e + c;  // this pixel's path values
+    }
+
+    // The generic walk's change from this step's base b to the next step's (none past the
+    // line's end).
+    __device__ __forceinline__ void next(int b, int following, bool live) {
+        delta = live ? following - b : 0;
+        moving = __builtin_amdgcn_ballot_w64(delta != 0) != 0;
+    }
+};
+
 template <int B, int PREFETCH>
 __global__ __launch_bounds__(BAND_PATH_BLOCK) void band_path_kernel(BandPathArgs a) {
     constexpr int G = BAND_PATH_BLOCK / B;  // lines per wave
-    constexpr uint32_t SENT = 0xFFFFu;       // a value no candidate takes
     const int lane = threadIdx.x % B, group = threadIdx.x / B;
     // The direction, its first block and its volume, by constant indices (a dynamic index
     // into the argument struct would copy it to scratch).
@@ -242,9 +301,13 @@ __global__ __launch_bounds__(BAND_PATH_BLOCK) void band_path_kernel(BandPathArgs
     const int W = a.W, H = a.H;
     const int first_line = ((int)blockIdx.x - start) * G;
     const LineGeometry g = line_geometry(o, first_line + group, W, H);
-    int longest = 0;
+    int longest = 0, shortest = 1 << 30;
 #pragma unroll
-    for (int i = 0; i < G; i++) longest = max(longest, line_geometry(o, first_line + i, W, H).length);
+    for (int i = 0; i < G; i++) {
+        const int length = line_geometry(o, first_line + i, W, H).length;
+        longest = max(longest, length);
+        shortest = min(shortest, length);
+    }
     if (longest == 0) return;  // wave-uniform
 
     const int dy = o == 0 ? 0 : 1;
@@ -261,24 +324,124 @@ __global__ __launch_bounds__(BAND_PATH_BLOCK) void band_path_kernel(BandPathArgs
     const Rsrc bases = buffer_of(a.base, (uint32_t)W * H * 4);
     const Rsrc out = buffer_of(volume_of_r, volume);
     const uint32_t p1 = (uint32_t)a.p1, p2 = (uint32_t)a.p2;
-    const int bstep = step * B;
-    uint32_t at = (uint32_t)(pixel * B + lane);
-    uint32_t ahead = at;
-    uint32_t at_base = (uint32_t)pixel * 4u;
-    uint32_t ahead_base = at_base;
 
     uint32_t cost_ring[PREFETCH];
+    {
+        uint32_t ahead = (uint32_t)(pixel * B + lane);
+#pragma unroll
+        for (int p = 0; p < PREFETCH; p++) {
+            cost_ring[p] = __builtin_amdgcn_raw_buffer_load_b8(costs, p < L ? ahead : MASKED, 0, 0);
+            ahead += step * B;
+        }
+    }
+    BandLine<B> line;
+
+    if constexpr (B >= 16 && PREFETCH == 16) {
+        // Blocks of 16 steps. A block loads its bases at once, lane j of a line the base of
+        // step t0 + j (its last pixel's past the line's end), and turns their changes into
+        // a lane mask: a step tests scalar bits. The bits of step j: lane j of every line.
+        constexpr uint64_t LINE_BITS = B == 16 ? 0x0001000100010001ull
+                                               : (B == 32 ? 0x0000000100000001ull : 1ull);
+        const int j = lane < 16 ? lane : 15;
+        int bases_now = 0, previous = 0;
+        // Lane j: the change into step t0 + j (none into the first step).
+        auto changes_of = [&](int t0) {
+            const int before = t0 == 0 ? dpp_or<DPP_ROW_SHR1>(bases_now, bases_now)
+                                       : dpp_or<DPP_ROW_SHR1>(dpp<dpp_row_ror(1)>(previous), bases_now);
+            return bases_now - before;
+        };
+        auto delta_at = [&](int changes, int p) {
+            return [=] { return __builtin_amdgcn_ds_bpermute((group * B + p) * 4, changes); };
+        };
+        if (o < 2 && shortest == longest) {
+            // Rows and columns where every line of the wave has the same length n: offsets
+            // are the lane's at the line's lowest address plus a wave-uniform position.
+            // Past the line's end, loads read its last pixel (used by nothing) and stores
+            // are skipped.
+            const int n = longest;
+            const uint32_t low = (uint32_t)(o == 0 ? g.y0 * W : g.x0);
+            const uint32_t lane_cost = low * B + lane, lane_base = low * 4;
+            const int stride = o == 0 ? 1 : W;
+            const int move = backward ? -stride : stride;
+            auto position = [&](int t) { return (backward ? n - 1 - t : t) * stride; };
+            auto block_bases = [&](int t0) {
+                return (int)__builtin_amdgcn_raw_buffer_load_b32(
+                    bases, lane_base + (uint32_t)position(min(t0 + j, n - 1)) * 4u, 0, 0);
+            };
+            bases_now = searched_of(block_bases(0));
+            // WHOLE: every load and store of the block lies inside the line.
+            auto block = [&](int t0, auto whole) {
+                constexpr bool WHOLE = decltype(whole)::value;
+                const int bases_next = block_bases(t0 + 16);
+                const int changes = changes_of(t0);
+                uint64_t moves = __builtin_amdgcn_ballot_w64(changes != 0 && lane < 16);
+                int here = position(t0), there = position(min(t0 + PREFETCH, n - 1));
+#pragma unroll
+                for (int p = 0; p < 16; p++) {
+                    const int t = t0 + p;
+                    __builtin_amdgcn_sched_barrier(0);
+                    const uint32_t c = cost_ring[p];
+                    cost_ring[p] = __builtin_amdgcn_raw_buffer_load_b8(costs, lane_cost, (uint32_t)there * B, 0);
+                    const bool moving_here = (moves & LINE_BITS) != 0;
+                    moves >>= 1;
+                    const uint32_t v = line.step(c, lane, group, p1, p2, moving_here, delta_at(changes, p));
+                    if (WHOLE || t < n)
+                        __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, out, lane_cost, (uint32_t)here * B, 0);
+                    here += move;
+                    there = WHOLE ? there + move : position(min(t + 1 + PREFETCH, n - 1));
+                }
+                previous = bases_now;
+                bases_now = searched_of(bases_next);
+            };
+            int t0 = 0;
+            for (; t0 + 2 * PREFETCH <= n; t0 += 16) block(t0, std::true_type{});
+            for (; t0 < n; t0 += 16) block(t0, std::false_type{});
+            return;
+        }
+        // Lines of unequal length (diagonals, an orientation's last wave): per-lane offsets,
+        // masked past each line's end.
+        auto block_bases = [&](int t0) {
+            const int t = min(t0 + j, L - 1);
+            return (int)__builtin_amdgcn_raw_buffer_load_b32(
+                bases, L > 0 ? (uint32_t)(pixel + t * step) * 4u : MASKED, 0, 0);
+        };
+        bases_now = searched_of(block_bases(0));
+        uint32_t at = (uint32_t)(pixel * B + lane);
+        uint32_t ahead = at + (uint32_t)(PREFETCH * step * B);
+        for (int t0 = 0; t0 < longest; t0 += 16) {
+            const int bases_next = block_bases(t0 + 16);
+            const int changes = changes_of(t0);
+            uint64_t moves = __builtin_amdgcn_ballot_w64(changes != 0 && lane < 16);
+#pragma unroll
+            for (int p = 0; p < 16; p++) {
+                const int t = t0 + p;
+                __builtin_amdgcn_sched_barrier(0);
+                const uint32_t c = cost_ring[p];
+                cost_ring[p] = __builtin_amdgcn_raw_buffer_load_b8(costs, t + PREFETCH < L ? ahead : MASKED, 0, 0);
+                const bool moving_here = (moves & LINE_BITS) != 0;
+                moves >>= 1;
+                const uint32_t v = line.step(c, lane, group, p1, p2, moving_here, delta_at(changes, p));
+                __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, out, t < L ? at : MASKED, 0, 0);
+                at += step * B;
+                ahead += step * B;
+            }
+            previous = bases_now;
+            bases_now = searched_of(bases_next);
+        }
+        return;
+    }
+
+    // A band of 8: every step loads its line's base, per-lane offsets, masked past each
+    // line's end.
     int base_ring[PREFETCH];
+    uint32_t ahead_base = (uint32_t)pixel * 4u;
 #pragma unroll
     for (int p = 0; p < PREFETCH; p++) {
-        const bool in = p < L;
-        cost_ring[p] = __builtin_amdgcn_raw_buffer_load_b8(costs, in ? ahead : MASKED, 0, 0);
-        base_ring[p] = (int)__builtin_amdgcn_raw_buffer_load_b32(bases, in ? ahead_base : MASKED, 0, 0);
-        ahead += bstep;
+        base_ring[p] = (int)__builtin_amdgcn_raw_buffer_load_b32(bases, p < L ? ahead_base : MASKED, 0, 0);
         ahead_base += step * 4;
     }
-    uint32_t v = 0;  // the previous pixel's values: 0 entering the image
-    int previous = searched_of(base_ring[0]);
+    uint32_t at = (uint32_t)(pixel * B + lane);
+    uint32_t ahead = at + (uint32_t)(PREFETCH * step * B);
     for (int t0 = 0; t0 < longest; t0 += PREFETCH) {
 #pragma unroll
         for (int p = 0; p < PREFETCH; p++) {
@@ -290,25 +453,14 @@ __global__ __launch_bounds__(BAND_PATH_BLOCK) void band_path_kernel(BandPathArgs
             const bool next = t + PREFETCH < L;
             cost_ring[p] = __builtin_amdgcn_raw_buffer_load_b8(costs, next ? ahead : MASKED, 0, 0);
             base_ring[p] = (int)__builtin_amdgcn_raw_buffer_load_b32(bases, next ? ahead_base : MASKED, 0, 0);
-            const uint32_t lo = BandLanes<B>::from_left(SENT, v, lane);
-            const uint32_t hi = BandLanes<B>::from_right(SENT, v, lane);
-            uint32_t sel = ::min(v, ::min(lo, hi) + p1);
-            const uint32_t m = BandLanes<B>::min(v);
-            const int delta = active ? b - previous : 0;
-            if (__builtin_amdgcn_ballot_w64(delta != 0)) {
-                const int moved = lane + delta;
-                const int from = min(max(moved, 0), B - 1);
-                const uint32_t packed = sel | ((v + p1) << 16);
-                const uint32_t f =
-                    (uint32_t)__builtin_amdgcn_ds_bpermute((group * B + from) * 4, (int)packed);
-                sel = (uint32_t)moved < (uint32_t)B ? (f & 0xFFFFu)
-                                                      : (moved == -1 || moved == B ? f >> 16 : SENT);
-            }
-            v = ::min(sel, m + p2) - m + c;
-            previous = b;
+            const uint32_t v = line.step(c, lane, group, p1, p2, line.moving,
+                                         [&] { return line.delta; });
             __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, out, active ? at : MASKED, 0, 0);
-            at += bstep;
-            ahead += bstep;
+            // The next step's base sits in the next slot (slot 0 after the last one,
+            // reloaded at this group's first step).
+            line.next(b, searched_of(base_ring[(p + 1) % PREFETCH]), t + 1 < L);
+            at += step * B;
+            ahead += step * B;
             ahead_base += step * 4;
         }
     }
@@ -479,7 +631,7 @@ static hipError_t allow_band_cost_window() {
 
 template <int B>
 static void launch_band_paths(const BandPathArgs &a, int blocks, hipStream_t stream) {
-    hipLaunchKernelGGL((band_path_kernel<B, 8>), dim3(blocks), dim3(BAND_PATH_BLOCK), 0, stream, a);
+    hipLaunchKernelGGL((band_path_kernel<B, 16>), dim3(blocks), dim3(BAND_PATH_BLOCK), 0, stream, a);
 }
 
 template <int B>

@@ -92,6 +92,15 @@ __device__ __forceinline__ uint32_t halves_min(uint32_t v) {
     return min((uint32_t)s[0], (uint32_t)s[1]);
 }
 
+__device__ __forceinline__ uint32_t pair_rows_max(uint32_t v) {
+    const auto s = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return max((uint32_t)s[0], (uint32_t)s[1]);
+}
+__device__ __forceinline__ uint32_t halves_max(uint32_t v) {
+    const auto s = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return max((uint32_t)s[0], (uint32_t)s[1]);
+}
+
 // Lane-wise u16 minimum of two halves over the 16 lanes of a DPP row.
 __device__ __forceinline__ uint32_t row_min_pk(uint32_t v) {
     v = pk_min(v, dpp<dpp_row_ror(8)>(v));

@@ -276,6 +276,11 @@ struct Member {
     Kind kind;
     std::string name;
     int disparities = 0, paths = 0;
+    // Disparity with search coarse_to_fine (sgm_coarse_to_fine/3): the half-size match's
+    // paths, the fine band, its margin and census window (0 sparse_9x7, 1 dense_9x5,
+    // 2 dense_9x5_blend).
+    bool coarse_to_fine = false;
+    int coarse_paths = 0, band = 0, margin = 0, fine_census = 0;
     int radius = 0;
     int kernel_size = 0;
     int bands = 0, reach = 0, row_step = 0;
@@ -346,6 +351,48 @@ static void check_matching(int W, int H, int D, int paths, int p1, int p2, float
     }
 }
 
+// sgm_band/4's checks (nx_stereo.ex:88-131, 286-308): the matching's, with 64 or 128
+// disparities, a band of 8, 16, 32 or 64 and a census window index.
+static void check_band(int W, int H, int D, int band, int paths, int census, int p1, int p2,
+                       float uniqueness) {
+    check_matching(W, H, D, paths, p1, p2, uniqueness);
+    if (D != 64 && D != 128) throw Error{"disparities must be 64 or 128, got " + std::to_string(D)};
+    if (band != 8 && band != 16 && band != 32 && band != 64)
+        throw Error{"band must be 8, 16, 32 or 64, got " + std::to_string(band)};
+    if (census < 0 || census > 2)
+        throw Error{"census must be 0 (sparse_9x7), 1 (dense_9x5) or 2 (dense_9x5_blend), got " +
+                    std::to_string(census)};
+    if (p1 < 0) throw Error{"p1 must be at least 0"};
+}
+
+// sgm_coarse_to_fine/3's (nx_stereo.ex:164-185): even sizes whose halves match at 64.
+static void check_coarse_to_fine(int W, int H, int band, int margin, int paths, int coarse_paths,
+                                 int census, int p1, int p2, float uniqueness) {
+    check_band(W, H, 128, band, paths, census, p1, p2, uniqueness);
+    if ((W | H) & 1)
+        throw Error{"coarse_to_fine expects an even height and width, got " + std::to_string(W) +
+                    "x" + std::to_string(H)};
+    check_matching(W / 2, H / 2, 64, coarse_paths, p1, p2, uniqueness);
+    if (margin < 0 || margin > 127)
+        throw Error{"margin must be an integer from 0 to 127, got " + std::to_string(margin)};
+}
+
+static BandParams band_params(int W, int H, int D, int band, int paths, int census, int p1,
+                              int p2, float uniqueness) {
+    BandParams bp;
+    bp.width = W;
+    bp.height = H;
+    bp.disparities = D;
+    bp.band = band;
+    bp.paths = paths;
+    bp.census = census;
+    bp.p1 = p1;
+    bp.p2 = p2;
+    bp.uniqueness = uniqueness;
+    return bp;
+}
+
+
 static Member make_member(const Geometry &g, const std::string &kind, const Options &o) {
     Member m;
     m.name = kind;
@@ -358,10 +405,38 @@ static Member make_member(const Geometry &g, const std::string &kind, const Opti
                         std::to_string(W) + "x" + std::to_string(H) + " working size"};
         m.kind = Kind::Rectification;
     } else if (kind == "disparity") {
-        takes_only(o, kind, {"num_disparities", "paths"});
+        takes_only(o, kind, {"num_disparities", "paths", "search", "coarse_paths", "band", "margin",
+                             "fine_census"});
         m.disparities = integer_option(o, kind, "num_disparities", {64, 128, 192, 256});
         m.paths = integer_option(o, kind, "paths", {3, 8});
-        check_matching(W, H, m.disparities, m.paths, 8, 64, 1.0f);
+        // search 0: full; 1: coarse_to_fine, with the settings stage/disparity.ex:71-106
+        // declares for it and the checks of its init (stage/disparity.ex:121-195).
+        m.coarse_to_fine = o.count("search") && integer_option(o, kind, "search", {0, 1}) == 1;
+        if (m.coarse_to_fine) {
+            if (m.disparities != 128)
+                throw Error{"coarse_to_fine searches 128 disparities, got " +
+                            std::to_string(m.disparities)};
+            if ((W | H) & 1)
+                throw Error{"coarse_to_fine needs an even width and height, got " +
+                            std::to_string(W) + "x" + std::to_string(H)};
+            m.coarse_paths =
+                o.count("coarse_paths") ? integer_option(o, kind, "coarse_paths", {3, 8}) : m.paths;
+            m.band = o.count("band") ? integer_option(o, kind, "band", {8, 16, 32, 64}) : 16;
+            m.margin = o.count("margin") ? count_option(o, kind, "margin", 0, 63) : 4;
+            if (m.margin >= m.band)
+                throw Error{"margin must be below band (" + std::to_string(m.band) + "), got " +
+                            std::to_string(m.margin)};
+            m.fine_census = o.count("fine_census") ? integer_option(o, kind, "fine_census", {0, 1, 2}) : 0;
+            check_coarse_to_fine(W, H, m.band, m.margin, m.paths, m.coarse_paths, m.fine_census, 8,
+                                 64, 1.0f);
+        } else {
+            if (o.count("fine_census") && option(o, kind, "fine_census") != 0.0)
+                throw Error{"fine_census requires coarse_to_fine search"};
+            for (const char *only : {"coarse_paths", "band", "margin"})
+                if (o.count(only))
+                    throw Error{std::string(only) + " requires coarse_to_fine search"};
+            check_matching(W, H, m.disparities, m.paths, 8, 64, 1.0f);
+        }
         m.kind = Kind::Disparity;
     } else if (kind == "sharpness") {
         takes_only(o, kind, {});
@@ -507,7 +582,10 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
     size_t match = 0, cloud = 0, partials = 0;
     for (auto &m : s->members) {
         if (m.kind == Kind::Disparity)
-            match = std::max(match, match_bytes(g.width, g.height, m.disparities, m.paths));
+            match = std::max(match, m.coarse_to_fine
+                                        ? coarse_to_fine_bytes(g.width, g.height, m.band, m.paths,
+                                                               m.coarse_paths)
+                                        : match_bytes(g.width, g.height, m.disparities, m.paths));
         if (m.kind == Kind::PointCloud) {
             max_dec = m.decimation;
             cloud = std::max(cloud, (size_t)(g.width / m.decimation) * (g.height / m.decimation) * 3);
@@ -741,15 +819,27 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
             case Kind::Disparity: {
                 const int p1 = penalty(in, "p1"), p2 = penalty(in, "p2");
                 const float u = scalar(in, "uniqueness");
-                check_matching(W, H, m.disparities, m.paths, p1, p2, u);
+                if (m.coarse_to_fine)
+                    check_coarse_to_fine(W, H, m.band, m.margin, m.paths, m.coarse_paths,
+                                         m.fine_census, p1, p2, u);
+                else
+                    check_matching(W, H, m.disparities, m.paths, p1, p2, u);
                 auto *l = (const uint8_t *)shaped(tensors, "left", TYPE_U8, image, "u8 image");
                 auto *r = (const uint8_t *)shaped(tensors, "right", TYPE_U8, image, "u8 image");
                 float *out = s.disparity[free_disparity];
-                MatchBuffers b;
                 if (!workspace) throw Error{"no matcher workspace"};
-                match_carve(b, workspace, W, H, m.disparities, m.paths);
-                MatchParams mp{(int)W, (int)H, m.disparities, m.paths, p1, p2, u};
-                HIP_TRY(launch_match(mp, l, r, b, out, nullptr, stream));
+                if (m.coarse_to_fine) {
+                    CoarseBuffers c;
+                    coarse_to_fine_carve(c, workspace, W, H, m.band, m.paths, m.coarse_paths);
+                    BandParams bp = band_params(W, H, 128, m.band, m.paths, m.fine_census, p1, p2, u);
+                    bp.positive = true;  // the stage's "> 0 else -1" (stage/disparity.ex:162)
+                    HIP_TRY(launch_coarse_to_fine(bp, m.margin, m.coarse_paths, l, r, c, out, stream));
+                } else {
+                    MatchBuffers b;
+                    match_carve(b, workspace, W, H, m.disparities, m.paths);
+                    MatchParams mp{(int)W, (int)H, m.disparities, m.paths, p1, p2, u};
+                    HIP_TRY(launch_match(mp, l, r, b, out, nullptr, stream));
+                }
                 tensors["disparity"] = Slot{TYPE_F32, image, out};
                 free_disparity ^= 1;
                 break;
@@ -1378,51 +1468,6 @@ int nxs_match_host(const uint8_t *left, const uint8_t *right, int width, int hei
     (void)hipFree(dev);
     return rc;
 }
-
-namespace nxs {
-
-// sgm_band/4's checks (nx_stereo.ex:88-131, 286-308): the matching's, with 64 or 128
-// disparities, a band of 8, 16, 32 or 64 and a census window index.
-static void check_band(int W, int H, int D, int band, int paths, int census, int p1, int p2,
-                       float uniqueness) {
-    check_matching(W, H, D, paths, p1, p2, uniqueness);
-    if (D != 64 && D != 128) throw Error{"disparities must be 64 or 128, got " + std::to_string(D)};
-    if (band != 8 && band != 16 && band != 32 && band != 64)
-        throw Error{"band must be 8, 16, 32 or 64, got " + std::to_string(band)};
-    if (census < 0 || census > 2)
-        throw Error{"census must be 0 (sparse_9x7), 1 (dense_9x5) or 2 (dense_9x5_blend), got " +
-                    std::to_string(census)};
-    if (p1 < 0) throw Error{"p1 must be at least 0"};
-}
-
-// sgm_coarse_to_fine/3's (nx_stereo.ex:164-185): even sizes whose halves match at 64.
-static void check_coarse_to_fine(int W, int H, int band, int margin, int paths, int coarse_paths,
-                                 int census, int p1, int p2, float uniqueness) {
-    check_band(W, H, 128, band, paths, census, p1, p2, uniqueness);
-    if ((W | H) & 1)
-        throw Error{"coarse_to_fine expects an even height and width, got " + std::to_string(W) +
-                    "x" + std::to_string(H)};
-    check_matching(W / 2, H / 2, 64, coarse_paths, p1, p2, uniqueness);
-    if (margin < 0 || margin > 127)
-        throw Error{"margin must be an integer from 0 to 127, got " + std::to_string(margin)};
-}
-
-static BandParams band_params(int W, int H, int D, int band, int paths, int census, int p1,
-                              int p2, float uniqueness) {
-    BandParams bp;
-    bp.width = W;
-    bp.height = H;
-    bp.disparities = D;
-    bp.band = band;
-    bp.paths = paths;
-    bp.census = census;
-    bp.p1 = p1;
-    bp.p2 = p2;
-    bp.uniqueness = uniqueness;
-    return bp;
-}
-
-}  // namespace nxs
 
 size_t nxs_band_workspace_size(int width, int height, int band, int paths) {
     return nxs::band_bytes(width, height, band, paths);

@@ -132,10 +132,16 @@ def move_rows(inverse_pr, rows):
 
 
 class Disparity(Stage):
-    """stage/disparity.ex: full-search SGM. 192 and 256 disparities extend the reference."""
+    """stage/disparity.ex: SGM, full search or coarse-to-fine (sgm_coarse_to_fine/3). 192
+    and 256 disparities extend the reference's full search. HipSegment computes
+    coarse_to_fine itself; the reference's native matcher refuses it
+    (stage/disparity.ex:166, 184-185), so its settings travel only to this backend:
+    search (0 full, 1 coarse_to_fine), coarse_paths, band, margin and fine_census (0
+    sparse_9x7, 1 dense_9x5, 2 dense_9x5_blend)."""
 
     KIND = "disparity"
     OUTPUTS = ("disparity",)
+    CENSUS_WINDOWS = ("sparse_9x7", "dense_9x5", "dense_9x5_blend")
     SETTINGS = {
         "p1": {"default": 8, "type": "integer", "range": (0, 78), "live": True},
         "p2": {"default": 64, "type": "integer", "range": (1, 79), "greater_than": "p1", "live": True},
@@ -143,18 +149,41 @@ class Disparity(Stage):
         "num_disparities": {"default": 64, "type": "integer", "values": [64, 128, 192, 256]},
         "paths": {"default": 8, "type": "integer", "values": [3, 8]},
         "search": {"default": "full", "type": "string", "values": ["full", "coarse_to_fine"]},
+        "coarse_paths": {"default": None, "type": "integer", "values": [3, 8]},
+        "band": {"default": 16, "type": "integer", "values": [8, 16, 32, 64]},
+        "margin": {"default": 4, "type": "integer", "range": (0, 63)},
+        "fine_census": {"default": "sparse_9x7", "type": "atom", "values": list(CENSUS_WINDOWS)},
         "matcher": {"default": None, "type": "atom", "values": [None, "nx", "native"]},
     }
+
+    def init(self, settings):
+        # stage/disparity.ex:121-195 (the geometry's even sizes are checked at configure)
+        values = self.init_settings(settings)
+        values["coarse_paths"] = values["coarse_paths"] or values["paths"]
+        if values["search"] == "coarse_to_fine":
+            if values["num_disparities"] != 128:
+                raise StageError("coarse_to_fine searches 128 disparities, got "
+                                 f"{values['num_disparities']}")
+            if values["margin"] >= values["band"]:
+                raise StageError(f"margin must be below band ({values['band']}), "
+                                 f"got {values['margin']}")
+        elif values["fine_census"] != "sparse_9x7":
+            raise StageError("fine_census requires coarse_to_fine search")
+        return values
 
     def params(self, geometry):
         return {k: self.state[k] for k in ("p1", "p2", "uniqueness")}
 
     def native_settings(self):
-        return [("num_disparities", self.state["num_disparities"]), ("paths", self.state["paths"])]
+        settings = [("num_disparities", self.state["num_disparities"]),
+                    ("paths", self.state["paths"])]
+        if self.state["search"] == "coarse_to_fine":
+            settings += [("search", 1), ("coarse_paths", self.state["coarse_paths"]),
+                         ("band", self.state["band"]), ("margin", self.state["margin"]),
+                         ("fine_census", self.CENSUS_WINDOWS.index(self.state["fine_census"]))]
+        return settings
 
     def compute(self):
-        if self.state["search"] == "coarse_to_fine":
-            return self.state["matcher"] or "nx"
         return self.state["matcher"]
 
 

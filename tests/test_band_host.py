@@ -31,3 +31,25 @@ def test_sgm_coarse_to_fine_validates_its_inputs():
         _native.sgm_coarse_to_fine(left, right, disparities=64)
     with pytest.raises(ValueError, match=":fine_census must be one of"):
         _native.sgm_coarse_to_fine(left, right, fine_census="sparse")
+
+
+def test_the_disparity_stage_checks_coarse_to_fine_as_its_init_does():
+    from nx_stereo_amd.stages import Disparity, StageError
+
+    with pytest.raises(StageError, match="coarse_to_fine searches 128 disparities, got 64"):
+        Disparity(search="coarse_to_fine")
+    with pytest.raises(StageError, match=r"margin must be below band \(8\), got 8"):
+        Disparity(search="coarse_to_fine", num_disparities=128, band=8, margin=8)
+    with pytest.raises(StageError, match="fine_census requires coarse_to_fine search"):
+        Disparity(fine_census="dense_9x5")
+    with pytest.raises(StageError, match="band must be one of"):
+        Disparity(search="coarse_to_fine", num_disparities=128, band=12)
+    stage = Disparity(search="coarse_to_fine", num_disparities=128, paths=3,
+                      fine_census="dense_9x5_blend")
+    assert stage.state["coarse_paths"] == 3  # paths by default
+    assert stage.native_settings() == [
+        ("num_disparities", 128), ("paths", 3), ("search", 1), ("coarse_paths", 3),
+        ("band", 16), ("margin", 4), ("fine_census", 2)]
+    assert Disparity(num_disparities=128).native_settings() == [
+        ("num_disparities", 128), ("paths", 8)]
+    assert stage.compute() is None

@@ -197,6 +197,57 @@ def test_new_kinds_refuse_settings_as_segment_rs_does(context):
     assert reply.reason == "sharpness has no setting radius"
 
 
+C2F_STAGE_CASES = [
+    {},
+    dict(coarse_paths=3, band=32, margin=8, fine_census="dense_9x5_blend"),
+    dict(paths=3, band=8, margin=0, fine_census="dense_9x5"),
+]
+
+
+@pytest.mark.parametrize("opts", C2F_STAGE_CASES)
+def test_disparity_stage_coarse_to_fine(context, opts):
+    # stage/disparity.ex:149-162: sgm_coarse_to_fine/3, then "> 0 else -1".
+    left, right, _ = scenes.planar_pair(320, 96, seed=5, low=4, high=110)
+    geometry, _ = rig_geometry(320, 96)
+    stage = stages.Disparity(num_disparities=128, search="coarse_to_fine", p1=10, p2=70,
+                             uniqueness=1.1, **opts)
+    seg = build(context, [("disparity", stage)], geometry)
+    out = run_ok(seg, {"left": left, "right": right})
+    want = oracle.coarse_to_fine(left, right, band=stage.state["band"],
+                                 margin=stage.state["margin"], paths=stage.state["paths"],
+                                 coarse_paths=stage.state["coarse_paths"],
+                                 fine_census=stage.state["fine_census"], p1=10, p2=70,
+                                 uniqueness=1.1, positive=True)
+    same_bits(out["disparity"], want)
+    assert (want > 0).mean() > 0.5
+
+
+def test_coarse_to_fine_settings_refused_as_the_stage_does(context):
+    g = {"width": 322, "height": 96, "frame_width": 322, "frame_height": 96}
+    base = [("num_disparities", 128.0), ("paths", 8.0)]
+    cases = [
+        (base + [("search", 1.0)], None),
+        ([("num_disparities", 64.0), ("paths", 8.0), ("search", 1.0)],
+         "coarse_to_fine searches 128 disparities, got 64"),
+        (base + [("search", 1.0), ("band", 16.0), ("margin", 16.0)],
+         "margin must be below band (16), got 16"),
+        (base + [("search", 1.0), ("band", 12.0)],
+         "disparity band must be one of [8, 16, 32, 64], got 12"),
+        (base + [("fine_census", 1.0)], "fine_census requires coarse_to_fine search"),
+        (base + [("band", 16.0)], "band requires coarse_to_fine search"),
+    ]
+    for settings, reason in cases:
+        reply = context.request(protocol.configure(7600, g, [("disparity", settings)]))
+        if reason is None:
+            assert reply is True
+            context.request(protocol.release(7600))
+        else:
+            assert reply.reason == reason, settings
+    odd = {"width": 321, "height": 96, "frame_width": 321, "frame_height": 96}
+    reply = context.request(protocol.configure(7601, odd, [("disparity", base + [("search", 1.0)])]))
+    assert reply.reason == "coarse_to_fine needs an even width and height, got 321x96"
+
+
 def test_temporal_filter_carries_across_pairs(context):
     geometry, _ = rig_geometry(64, 32)
     seg = build(context, [("temporal_filter", stages.TemporalFilter(alpha=0.5, delta=2.0, hold=1))],
