@@ -375,13 +375,27 @@ def served_leg(duration_s: float = 1.5) -> dict:
 # OUT (profiles/traffic_*.json), which the default run reads into roofline.traffic.
 
 TRAFFIC_FILES = {"728": os.path.join(ROOT, "profiles", "traffic_728x544x128.json"),
-                 "1080": os.path.join(ROOT, "profiles", "traffic_1920x1080x256.json")}
+                 "1080": os.path.join(ROOT, "profiles", "traffic_1920x1080x256.json"),
+                 "c2f728": os.path.join(ROOT, "profiles", "traffic_c2f_728x544.json"),
+                 "c2f1080": os.path.join(ROOT, "profiles", "traffic_c2f_1920x1080.json")}
 CALIBRATION_BYTES = 256 << 20
 
 
 def kernels_only(size: str) -> int:
     from nx_stereo_amd import _native, hip
 
+    if size.startswith("c2f"):
+        # sgm_coarse_to_fine/3 as the coarse_to_fine leg runs it (band 16, margin 4, 8 paths).
+        w, h = (WIDTH, HEIGHT) if size == "c2f728" else (1920, 1080)
+        left, right = match_pair(w, h, 128, SEED + 5)
+        l, r, o = hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h)
+        ws = hip.Buffer(_native.coarse_to_fine_workspace_size(w, h, 16, 8, 8))
+        for _ in range(5):
+            _native.coarse_to_fine_device(l.ptr, r.ptr, w, h, 16, 4, 8, 8, 0, 8, 64,
+                                          1.0526315789473684, o.ptr, ws.ptr, ws.nbytes)
+        hip.synchronize()
+        print(f"kernels-only coarse-to-fine {w}x{h} done", flush=True)
+        return 0
     # The kernels' accesses do not depend on the images' content: a planar pair of the size.
     if size == "728":
         w, h, d = WIDTH, HEIGHT, DISPARITIES
@@ -466,7 +480,8 @@ def main() -> int:
                         help="comma-separated extra legs: large, batched, served, "
                              "coarse_to_fine (or none)")
     parser.add_argument("--kernels-only", choices=sorted(TRAFFIC_FILES),
-                        help="run three matchings and the calibration copy, for rocprofv3 --pmc")
+                        help="run three matchings and the calibration copy (c2f*: five "
+                             "coarse-to-fine matchings), for rocprofv3")
     parser.add_argument("--traffic", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT"),
                         help="HBM bytes per launch from two rocprofv3 --pmc runs into OUT")
     args = parser.parse_args()
