@@ -12,7 +12,11 @@
  *   request  3 release    -> no reply record (the ticket completes with an empty reply)
  *
  * Tensors: name, type (0 u8, 1 u16, 2 f32, 3 s32), rank, u32 dimensions,
- * row-major little-endian elements (tensor.rs:25-40).
+ * row-major little-endian elements (tensor.rs:25-40). A frame or image entry may also come
+ * as type 4, u32 {height, width / 4}: four pixels a word, pixel 0 in the low byte, as
+ * NxSegment's pipelines without byte storage pack them (packing.ex; segment.ex:207-241).
+ * It is read as the u8 image it holds. Besides segment.rs's kinds, "decode"
+ * (stage/decode.ex: both frames area-resized to the working size) is built.
  *
  * Threading: every function may be called from any thread. nxs_submit copies
  * the record and returns at once; a context-owned thread computes requests
@@ -212,7 +216,7 @@ void nxs_batch_destroy(nxs_batch *batch);
  * are already on the GPU (the benchmark, the batched and served forms). */
 typedef struct nxs_tensor {
     const char *name;
-    uint8_t type; /* 0 u8, 1 u16, 2 f32, 3 s32 */
+    uint8_t type; /* 0 u8, 1 u16, 2 f32, 3 s32, 4 u32 (a packed frame or image) */
     uint8_t rank;
     uint32_t shape[4];
     void *data;

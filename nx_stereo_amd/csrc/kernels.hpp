@@ -181,6 +181,18 @@ hipError_t launch_alignment(const uint8_t *left, const uint8_t *right, const flo
                             int width, int height, int bands, int reach, int row_step,
                             uint64_t *totals, float *sums, hipStream_t stream);
 
+// Decode's area resize: per output row (or column) o, count[o] taps at index[o * most + k]
+// with f32 weight[o * most + k] (image.ex:458-476), in device memory.
+struct AreaTaps {
+    const int *count;
+    const int *index;
+    const float *weight;
+    int most;
+};
+hipError_t launch_resize_area(const uint8_t *left, const uint8_t *right, int frame_width,
+                              int width, int height, const AreaTaps &rows, const AreaTaps &cols,
+                              uint8_t *left_out, uint8_t *right_out, hipStream_t stream);
+
 hipError_t launch_fill_f32(float *dst, float value, size_t count, hipStream_t stream);
 
 }  // namespace nxs

@@ -93,14 +93,18 @@ void remember_schedule(int width, int height, int disparities, int paths, const 
 
 enum : uint8_t { CONFIGURE = 1, RUN = 2, RELEASE = 3 };
 enum : uint8_t { TAG_CONFIGURED = 7, TAG_OUTPUTS = 8, TAG_REFUSED = 9 };
-enum : uint8_t { TYPE_U8 = 0, TYPE_U16 = 1, TYPE_F32 = 2, TYPE_S32 = 3 };
+// tensor.rs:25-40's element types, and u32 (4): frames and images packed as words of four
+// pixels, pixel 0 in the low byte (packing.ex), as NxSegment's pipelines without byte
+// storage hold them.
+enum : uint8_t { TYPE_U8 = 0, TYPE_U16 = 1, TYPE_F32 = 2, TYPE_S32 = 3, TYPE_U32 = 4 };
 
 static int element_bytes(uint8_t type) {
     switch (type) {
         case TYPE_U8: return 1;
         case TYPE_U16: return 2;
         case TYPE_F32:
-        case TYPE_S32: return 4;
+        case TYPE_S32:
+        case TYPE_U32: return 4;
         default: return -1;
     }
 }
@@ -260,6 +264,7 @@ static std::string number_text(double v) {
 // Stages (segment.rs:244-333): kinds, build settings and the constants they share with Nx.
 
 enum class Kind {
+    Decode,
     Rectification,
     Sharpness,
     Disparity,
@@ -397,7 +402,15 @@ static Member make_member(const Geometry &g, const std::string &kind, const Opti
     Member m;
     m.name = kind;
     const int W = g.width, H = g.height;
-    if (kind == "rectification") {
+    if (kind == "decode") {
+        // stage/decode.ex:18-21 takes no settings; image.ex:410-412 only shrinks.
+        takes_only(o, kind, {});
+        if (g.frame_width < W || g.frame_height < H)
+            throw Error{"resize_area only shrinks, got " + std::to_string(g.frame_height) + "x" +
+                        std::to_string(g.frame_width) + " to " + std::to_string(H) + "x" +
+                        std::to_string(W)};
+        m.kind = Kind::Decode;
+    } else if (kind == "rectification") {
         takes_only(o, kind, {});
         if (g.frame_width < W || g.frame_height < H)
             throw Error{"rectification only shrinks: " + std::to_string(g.frame_width) + "x" +
@@ -484,6 +497,7 @@ static const std::vector<const char *> &member_outputs(Kind k) {
                                            cloud = {"cloud_grid", "cloud_points"},
                                            quality = {"quality_sums"};
     switch (k) {
+        case Kind::Decode:
         case Kind::Rectification: return rect;
         case Kind::Sharpness: return sharp;
         case Kind::RowAlignment: return align;
@@ -548,6 +562,7 @@ struct Segment {
     float *sharpness_sums = nullptr, *alignment_sums = nullptr;
     double *quality_partials = nullptr;
     int32_t *cloud_partials = nullptr, *cloud_kept = nullptr;
+    AreaTaps decode_rows{}, decode_cols{};  // Decode's area-resize taps (on the device)
     std::vector<float *> carry_a, carry_b;  // per member: temporal previous/missed, quality previous
 
     ~Segment() {
@@ -565,6 +580,60 @@ struct Segment {
 static size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
 static uint8_t *up256_ptr(uint8_t *p) {
     return reinterpret_cast<uint8_t *>((reinterpret_cast<uintptr_t>(p) + 255) & ~(uintptr_t)255);
+}
+
+// image.ex:458-476 (area_weights): output o of `size` covers the input span
+// [o scale, (o + 1) scale), scale = in_size / size; each input it overlaps is a tap with
+// weight share / scale, computed in f64 and rounded to f32 (the padding taps of zero weight
+// the reference adds change no sum and are left out).
+struct HostTaps {
+    int size = 0, most = 0;
+    std::vector<int> count, index;
+    std::vector<float> weight;
+    size_t bytes() const { return (size_t)size * (1 + 2 * (size_t)most) * 4; }
+    // Copies the taps to `dst` (bytes() bytes of device memory) and points at them.
+    AreaTaps upload(uint8_t *dst) const {
+        std::vector<uint8_t> host(bytes());
+        uint8_t *p = host.data();
+        std::memcpy(p, count.data(), 4 * (size_t)size);
+        std::memcpy(p + 4 * (size_t)size, index.data(), 4 * index.size());
+        std::memcpy(p + 4 * ((size_t)size + index.size()), weight.data(), 4 * weight.size());
+        if (hipMemcpy(dst, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess)
+            throw Error{"copying the decode taps failed"};
+        AreaTaps t;
+        t.count = reinterpret_cast<const int *>(dst);
+        t.index = t.count + size;
+        t.weight = reinterpret_cast<const float *>(t.index + index.size());
+        t.most = most;
+        return t;
+    }
+};
+
+static HostTaps area_taps(int size, int in_size) {
+    HostTaps t;
+    t.size = size;
+    const double scale = (double)in_size / (double)size;
+    t.most = (int)std::ceil(scale) + 1;
+    t.count.assign(size, 0);
+    t.index.assign((size_t)size * t.most, 0);
+    t.weight.assign((size_t)size * t.most, 0.0f);
+    for (int o = 0; o < size; o++) {
+        const double first = o * scale, last = (o + 1) * scale;
+        int n = 0;
+        for (int input = (int)std::floor(first); input <= (int)std::ceil(last) - 1; input++) {
+            const double share = std::min((double)input + 1, last) - std::max((double)input, first);
+            if (share > 0 && n < t.most) {
+                // (o + 1) scale can round past in_size in f64: a last tap one past the input
+                // with a weight near 1e-15, which changes no rounded output; it reads the
+                // last input instead.
+                t.index[(size_t)o * t.most + n] = std::min(input, in_size - 1);
+                t.weight[(size_t)o * t.most + n] = (float)(share / scale);
+                n++;
+            }
+        }
+        t.count[o] = n;
+    }
+    return t;
 }
 
 static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
@@ -603,11 +672,21 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
     }
     int carries = 0;
     for (auto &m : s->members) carries += m.kind == Kind::TemporalFilter ? 2 : (m.kind == Kind::Quality);
+    // Decode's taps: rows (H outputs of the frame's height), then columns (W outputs of its
+    // width), each as counts, indices and weights.
+    bool decode = false;
+    for (auto &m : s->members) decode = decode || m.kind == Kind::Decode;
+    HostTaps rows_taps, cols_taps;
+    if (decode) {
+        rows_taps = area_taps(g.height, g.frame_height);
+        cols_taps = area_taps(g.width, g.frame_width);
+    }
+    const size_t taps_bytes = rows_taps.bytes() + cols_taps.bytes();
     size_t bytes = up256(F) * 2 + up256(P) * 2 + up256(4 * P) * 2 + up256(2 * P) + up256(4 * P) +
                    up256(4 * cloud) + up256(4 * (partials + 1)) +
                    up256(quality_scratch_bytes(g.width, g.height)) + up256(32) +
                    up256(sharp) + up256(32) + up256(align) + up256(4 * align_sums) +
-                   up256(4 * P) * carries + 256;
+                   up256(4 * P) * carries + up256(taps_bytes) + 256;
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&s->arena, bytes));
     uint8_t *p = static_cast<uint8_t *>(s->arena);
@@ -646,6 +725,11 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
     if (align)
         HIP_TRY(launch_fill_f32(reinterpret_cast<float *>(s->alignment_totals), 0.0f,
                                 align / sizeof(float), s->stream));
+    if (decode) {
+        uint8_t *taps = carve(taps_bytes);
+        s->decode_rows = rows_taps.upload(taps);
+        s->decode_cols = cols_taps.upload(taps + rows_taps.bytes());
+    }
     s->workspace_bytes = match;
     for (auto &m : s->members) {
         float *a = nullptr, *b = nullptr;
@@ -756,6 +840,20 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
     std::map<std::string, Slot> &tensors = q.tensors;
     int free_disparity = 0;
 
+    // A frame or image packed as u32 words (segment.ex:207-241) holds the u8 image's bytes:
+    // it is read as that image.
+    auto packed = [](uint8_t type, const std::vector<uint32_t> &shape, const std::vector<uint32_t> &as) {
+        return type == TYPE_U32 && shape.size() == 2 && as[1] % 4 == 0 && shape[0] == as[0] &&
+               shape[1] * 4 == as[1];
+    };
+    auto unpacked = [&](const std::string &name, uint8_t &type, std::vector<uint32_t> &shape) {
+        const bool is_frame = name == "left_frame" || name == "right_frame";
+        const bool is_image = name == "left" || name == "right";
+        if ((is_frame && packed(type, shape, frame)) || (is_image && packed(type, shape, image))) {
+            type = TYPE_U8;
+            shape = is_frame ? frame : image;
+        }
+    };
     // Where an entry a stage can read goes, or nullptr.
     auto target_of = [&](const std::string &name, uint8_t type, const std::vector<uint32_t> &shape) -> void * {
         if ((name == "left_frame" || name == "right_frame") && type == TYPE_U8 && shape == frame)
@@ -770,7 +868,8 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
     for (auto &e : entries) {
         const HostTensor &t = e.second;
         Slot slot{t.type, t.shape, nullptr};
-        if (void *target = target_of(e.first, t.type, t.shape)) {
+        unpacked(e.first, slot.type, slot.shape);
+        if (void *target = target_of(e.first, slot.type, slot.shape)) {
             if (run_slot && staged + t.size <= run_slot->upload_bytes) {
                 std::memcpy(run_slot->upload + staged, t.bytes, t.size);
                 HIP_TRY(hipMemcpyAsync(target, run_slot->upload + staged, t.size,
@@ -787,7 +886,8 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
     }
     for (auto &e : device_entries) {
         Slot slot{e.type, e.shape, nullptr};
-        if (void *target = target_of(e.name, e.type, e.shape)) {
+        unpacked(e.name, slot.type, slot.shape);
+        if (void *target = target_of(e.name, slot.type, slot.shape)) {
             if (e.name == "disparity") {
                 // Filters may change the disparity in place: work on the segment's copy.
                 HIP_TRY(hipMemcpyAsync(target, e.data, 4 * P, hipMemcpyDeviceToDevice, stream));
@@ -806,6 +906,15 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
         const Member &m = s.members[i];
         const Inputs &in = inputs[i];
         switch (m.kind) {
+            case Kind::Decode: {
+                auto *lf = (const uint8_t *)shaped(tensors, "left_frame", TYPE_U8, frame, "u8 image");
+                auto *rf = (const uint8_t *)shaped(tensors, "right_frame", TYPE_U8, frame, "u8 image");
+                HIP_TRY(launch_resize_area(lf, rf, g.frame_width, W, H, s.decode_rows,
+                                           s.decode_cols, s.images[0], s.images[1], stream));
+                tensors["left"] = Slot{TYPE_U8, image, s.images[0]};
+                tensors["right"] = Slot{TYPE_U8, image, s.images[1]};
+                break;
+            }
             case Kind::Rectification: {
                 Camera left = camera(in, "left"), right = camera(in, "right");
                 auto *lf = (const uint8_t *)shaped(tensors, "left_frame", TYPE_U8, frame, "u8 image");

@@ -686,6 +686,48 @@ hipError_t launch_alignment(const uint8_t *left, const uint8_t *right, const flo
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------------------
+// Decode (stage/decode.ex:24-57; image.ex:402-476, resize_area): the f32 area resize of both
+// frames to the working size. The reference resizes rows first, into an f32 image of the
+// frame's width, then columns; a thread makes one output pixel, recomputing the row pass at
+// its column taps with the same operations in the same order (taps in order, the first
+// product starting each sum), so the f32 values are the reference's. Then rounded half away
+// from zero and clipped to 0..255. Grid y: the image (0 left, 1 right).
+
+__global__ __launch_bounds__(256) void resize_area_kernel(const uint8_t *left,
+                                                          const uint8_t *right, int fw, int W,
+                                                          int H, AreaTaps rows, AreaTaps cols,
+                                                          uint8_t *left_out, uint8_t *right_out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= W * H) return;
+    const int y = i / W, x = i - y * W;
+    const uint8_t *frame = blockIdx.y ? right : left;
+    const int rn = rows.count[y], cn = cols.count[x];
+    const int *ri = rows.index + y * rows.most, *ci = cols.index + x * cols.most;
+    const float *rw = rows.weight + y * rows.most, *cw = cols.weight + x * cols.most;
+    float total = 0.0f;
+    for (int k = 0; k < cn; k++) {
+        const uint8_t *column = frame + ci[k];
+        float t = 0.0f;
+        for (int j = 0; j < rn; j++) {
+            const float product = rw[j] * (float)column[(size_t)ri[j] * fw];
+            t = j == 0 ? product : t + product;
+        }
+        const float product = cw[k] * t;
+        total = k == 0 ? product : total + product;
+    }
+    const float r = fminf(fmaxf(roundf(total), 0.0f), 255.0f);
+    (blockIdx.y ? right_out : left_out)[i] = (uint8_t)r;
+}
+
+hipError_t launch_resize_area(const uint8_t *left, const uint8_t *right, int fw, int W, int H,
+                              const AreaTaps &rows, const AreaTaps &cols, uint8_t *left_out,
+                              uint8_t *right_out, hipStream_t stream) {
+    hipLaunchKernelGGL(resize_area_kernel, dim3((W * H + 255) / 256, 2), dim3(256), 0, stream,
+                       left, right, fw, W, H, rows, cols, left_out, right_out);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_f32(float *dst, float value, size_t count, hipStream_t stream) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream,

@@ -15,7 +15,7 @@ Byte for byte the records NxStereo.NativeSegment writes and reads
 and the replies: tag 7 (configured), tag 8 (member count u8, u32 us each,
 output count u8, tensors) and tag 9 (refused, UTF-8 reason). A name is its
 length (u8) and bytes; a tensor is its name, type u8 (0 u8, 1 u16, 2 f32,
-3 s32), rank u8, u32 dimensions and row-major elements. Little-endian.
+3 s32, 4 u32: packed frames and images), rank u8, u32 dimensions and row-major elements. Little-endian.
 """
 from __future__ import annotations
 
@@ -27,8 +27,10 @@ import numpy as np
 CONFIGURE, RUN, RELEASE = 1, 2, 3
 TAG_LOG, TAG_CONFIGURED, TAG_OUTPUTS, TAG_REFUSED = 2, 7, 8, 9
 
+# tensor.rs:25-40's types, and 4: u32, frames and images packed four pixels a word
+# (packing.ex), which the segment reads as the u8 image they hold.
 TYPE_CODES = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.float32): 2,
-              np.dtype(np.int32): 3}
+              np.dtype(np.int32): 3, np.dtype(np.uint32): 4}
 CODE_TYPES = {v: k for k, v in TYPE_CODES.items()}
 
 

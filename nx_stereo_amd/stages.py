@@ -111,6 +111,14 @@ class Stage:
         return None
 
 
+class Decode(Stage):
+    """stage/decode.ex: both frames area-resized (image.ex:402-476) to the working size, for
+    frames that are already rectified; u8 frames, or frames packed as u32 words."""
+
+    KIND = "decode"
+    OUTPUTS = ("left", "right")
+
+
 class Rectification(Stage):
     """stage/rectification.ex: both frames undistorted, rectified and shrunk."""
 
@@ -292,8 +300,9 @@ class Quality(Stage):
     OUTPUTS = ("quality_sums",)
 
 
-KINDS = {cls.KIND: cls for cls in (Rectification, Sharpness, Disparity, MedianFilter, SpeckleFilter,
-                                   RowAlignment, TemporalFilter, Depth, PointCloud, Quality)}
+KINDS = {cls.KIND: cls for cls in (Decode, Rectification, Sharpness, Disparity, MedianFilter,
+                                   SpeckleFilter, RowAlignment, TemporalFilter, Depth, PointCloud,
+                                   Quality)}
 
 
 def check_geometry(stage: Stage, geometry: dict) -> None:
@@ -303,6 +312,10 @@ def check_geometry(stage: Stage, geometry: dict) -> None:
         raise StageError(
             f"rectification only shrinks: {geometry['frame_width']}x{geometry['frame_height']} "
             f"frames are smaller than the {w}x{h} working size")
+    if isinstance(stage, Decode) and (geometry["frame_width"] < w or geometry["frame_height"] < h):
+        raise StageError(
+            f"resize_area only shrinks, got {geometry['frame_height']}x{geometry['frame_width']} "
+            f"to {h}x{w}")
     if isinstance(stage, PointCloud):
         d = stage.state["decimation"]
         if d > h or d > w:

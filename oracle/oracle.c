@@ -249,6 +249,81 @@ int orc_sgm(const uint8_t *left, const uint8_t *right, int width, int height, in
     return rc;
 }
 
+/* image.ex:458-476 (area_weights): each output's taps, input index and f32 weight share /
+ * scale of the input span [o scale, (o + 1) scale), computed in f64 and rounded to f32
+ * (parity unpinned for the sub-1e-6 contributions of those taps, which round away).
+ * Returns the taps per output (padding with zero weights is left out: adding 0 changes no
+ * sum). index/weight hold size * (ceil(scale) + 1) entries. */
+static int area_taps(int size, int in_size, int *index, float *weight, int *count) {
+    double scale = (double)in_size / (double)size;
+    int most = (int)ceil(scale) + 1;
+    for (int o = 0; o < size; o++) {
+        double first = o * scale, last = (o + 1) * scale;
+        int n = 0;
+        for (int input = (int)floor(first); input <= (int)ceil(last) - 1; input++) {
+            double hi = input + 1 < last ? input + 1 : last;
+            double lo = input > first ? input : first;
+            double share = hi - lo;
+            if (share > 0) {
+                /* (o + 1) scale can round past in_size in f64, giving a last tap one past
+                 * the input with a weight near 1e-15; it changes no rounded output, so it
+                 * reads the last input instead of past it. */
+                index[o * most + n] = input < in_size ? input : in_size - 1;
+                weight[o * most + n] = (float)(share / scale);
+                n++;
+            }
+        }
+        count[o] = n;
+    }
+    return most;
+}
+
+/* image.ex:402-456 (resize_area): the f32 area resize of a u8 frame to {height, width}:
+ * rows first (each output row the weighted sum of its input rows, taps in order), then
+ * columns the same way on that f32 image, then rounded half away from zero, clipped to
+ * 0..255. The frame's own size copies it. */
+void orc_resize_area(const uint8_t *frame, int fw, int fh, int width, int height,
+                     uint8_t *out) {
+    if (fw == width && fh == height) {
+        memcpy(out, frame, (size_t)fw * fh);
+        return;
+    }
+    int ry = (int)ceil((double)fh / height) + 1, rx = (int)ceil((double)fw / width) + 1;
+    int *yi = (int *)malloc(sizeof(int) * height * ry), *yc = (int *)malloc(sizeof(int) * height);
+    float *yw = (float *)malloc(sizeof(float) * height * ry);
+    int *xi = (int *)malloc(sizeof(int) * width * rx), *xc = (int *)malloc(sizeof(int) * width);
+    float *xw = (float *)malloc(sizeof(float) * width * rx);
+    float *rows = (float *)malloc(sizeof(float) * (size_t)height * fw);
+    area_taps(height, fh, yi, yw, yc);
+    area_taps(width, fw, xi, xw, xc);
+    for (int y = 0; y < height; y++)
+        for (int c = 0; c < fw; c++) {
+            float t = 0.0f;
+            for (int k = 0; k < yc[y]; k++) {
+                float product = yw[y * ry + k] * (float)frame[(size_t)yi[y * ry + k] * fw + c];
+                t = k == 0 ? product : t + product;
+            }
+            rows[(size_t)y * fw + c] = t;
+        }
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < width; x++) {
+            float t = 0.0f;
+            for (int k = 0; k < xc[x]; k++) {
+                float product = xw[x * rx + k] * rows[(size_t)y * fw + xi[x * rx + k]];
+                t = k == 0 ? product : t + product;
+            }
+            float r = roundf(t);
+            out[(size_t)y * width + x] = (uint8_t)(r < 0.0f ? 0.0f : (r > 255.0f ? 255.0f : r));
+        }
+    free(yi);
+    free(yc);
+    free(yw);
+    free(xi);
+    free(xc);
+    free(xw);
+    free(rows);
+}
+
 /* ---- Band search and coarse-to-fine (nx_stereo.ex:164-471) ---- */
 
 /* nx_stereo.ex:236-247: the rounded 2x2 average, (sum + 2) / 4. */

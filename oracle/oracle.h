@@ -28,6 +28,8 @@
  *   orc_sharpness    - measures.rs:168-184 and diagnostics_test.exs:17-53 closed forms
  *   orc_alignment    - measures.rs:186-239 (the moved offset, disparity 0, configured
  *                      bands/reach/rows) and diagnostics_test.exs:79-95
+ *   orc_resize_area  - exact twofold = the rounded block mean (orc_shrink); the frame's own
+ *                      size copies; hand-worked 3 -> 2 and 5 -> 2 taps; constant frames
  *   orc_shrink       - closed forms (constant, rounding of +2) and a reshape-sum route
  *   orc_census_window- sparse = orc_census; dense/blend brute force against an
  *                      ndimage (scipy) anchor and window, constant images give 0
@@ -109,6 +111,11 @@ void orc_sharpness(const uint8_t *left, const uint8_t *right, int width, int hei
 /* measures.rs:53-108 (Alignment, alignment_sums): bands rows of 2 reach + 2 sums. */
 void orc_alignment(const uint8_t *left, const uint8_t *right, const float *disparity, int width,
                    int height, int bands, int reach, int row_step, float *sums);
+
+/* image.ex:402-476 (resize_area): u8 frame {fh, fw} -> u8 {height, width}, only shrinking;
+ * stage/decode.ex:24-57 applies it to both frames. */
+void orc_resize_area(const uint8_t *frame, int fw, int fh, int width, int height,
+                     uint8_t *out);
 
 /* ---- Band search and coarse-to-fine (nx_stereo.ex:164-471) ---- */
 

@@ -59,6 +59,7 @@ def lib() -> ctypes.CDLL:
         _lib.orc_median.argtypes = [_P, i, i, i, _P]
         _lib.orc_sharpness.argtypes = [_P, _P, i, i, _P]
         _lib.orc_alignment.argtypes = [_P, _P, _P, i, i, i, i, i, _P]
+        _lib.orc_resize_area.argtypes = [_P, i, i, i, i, _P]
         _lib.orc_sgm_raw.argtypes = [_P, _P, i, i, i, i, i, i, f, _P]
         _lib.orc_sgm_raw.restype = i
         _lib.orc_shrink.argtypes = [_P, i, i, _P]
@@ -202,6 +203,16 @@ def sgm_raw(left, right, disparities=64, paths=8, p1=8, p2=64, uniqueness=1.0526
                            float(np.float32(uniqueness)), _ptr(out))
     if rc:
         raise MemoryError("oracle sgm ran out of memory")
+    return out
+
+
+def resize_area(frame, size):
+    """image.ex:402-476: the area resize of a u8 frame to size = (height, width)."""
+    frame = _c(frame, np.uint8)
+    fh, fw = frame.shape
+    h, w = size
+    out = np.empty((h, w), np.uint8)
+    lib().orc_resize_area(_ptr(frame), fw, fh, w, h, _ptr(out))
     return out
 
 

@@ -248,6 +248,60 @@ def test_coarse_to_fine_settings_refused_as_the_stage_does(context):
     assert reply.reason == "coarse_to_fine needs an even width and height, got 321x96"
 
 
+def pack(image):
+    """packing.ex:16-21: a u8 image's bytes as u32 words, four pixels a word."""
+    image = np.ascontiguousarray(image, np.uint8)
+    return image.view("<u4").reshape(image.shape[0], image.shape[1] // 4)
+
+
+@pytest.mark.parametrize("frame_size,size", [((1456, 1088), (728, 544)), ((1000, 700), (728, 544)),
+                                             ((333, 217), (100, 64)), ((320, 96), (320, 96))])
+def test_decode_stage(context, frame_size, size):
+    # stage/decode.ex: both frames area-resized to the working size (image.ex:402-476).
+    (fw, fh), (w, h) = frame_size, size
+    g = {"width": w, "height": h, "frame_width": fw, "frame_height": fh}
+    frames = [scenes.noise(fw, fh, 3), scenes.noise(fw, fh, 4)]
+    seg = build(context, [("decode", stages.Decode())], g)
+    out = run_ok(seg, {"left_frame": frames[0], "right_frame": frames[1]})
+    for side, frame in zip(("left", "right"), frames):
+        same_bits(out[side], oracle.resize_area(frame, (h, w)))
+    if fw % 4 == 0:
+        # The same frames packed as u32 words (a pipeline without byte storage).
+        packed = run_ok(seg, {"left_frame": pack(frames[0]), "right_frame": pack(frames[1])})
+        for side in ("left", "right"):
+            same_bits(packed[side], out[side])
+
+
+def test_packed_images_enter_as_the_images_they_hold(context):
+    left, right, _ = scenes.planar_pair(320, 96, seed=9, low=4, high=56)
+    geometry, _ = rig_geometry(320, 96)
+    seg = build(context, [("disparity", stages.Disparity(num_disparities=64, paths=8))], geometry)
+    plain = run_ok(seg, {"left": left, "right": right})
+    packed = run_ok(seg, {"left": pack(left), "right": pack(right)})
+    same_bits(packed["disparity"], plain["disparity"])
+    same_bits(plain["disparity"], oracle.sgm(left, right, 64, 8))
+
+
+def test_decode_then_coarse_to_fine_from_packed_frames(context):
+    # A byte-storage-free pipeline's shape: packed camera frames, Decode, then the matcher.
+    w, h = 320, 96
+    frames = [scenes.noise(2 * w, 2 * h, 5), scenes.noise(2 * w, 2 * h, 6)]
+    g = {"width": w, "height": h, "frame_width": 2 * w, "frame_height": 2 * h}
+    disparity = stages.Disparity(num_disparities=128, search="coarse_to_fine")
+    seg = build(context, [("decode", stages.Decode()), ("disparity", disparity)], g)
+    out = run_ok(seg, {"left_frame": pack(frames[0]), "right_frame": pack(frames[1])})
+    left, right = (oracle.resize_area(f, (h, w)) for f in frames)
+    same_bits(out["disparity"], oracle.coarse_to_fine(left, right, positive=True))
+
+
+def test_decode_refuses_to_grow(context):
+    g = {"width": 64, "height": 48, "frame_width": 64, "frame_height": 40}
+    reply = context.request(protocol.configure(7700, g, [("decode", [])]))
+    assert reply.reason == "resize_area only shrinks, got 40x64 to 48x64"
+    reply = context.request(protocol.configure(7700, g, [("decode", [("kernel_size", 3.0)])]))
+    assert reply.reason == "decode has no setting kernel_size"
+
+
 def test_temporal_filter_carries_across_pairs(context):
     geometry, _ = rig_geometry(64, 32)
     seg = build(context, [("temporal_filter", stages.TemporalFilter(alpha=0.5, delta=2.0, hold=1))],

@@ -298,3 +298,27 @@ def test_coarse_to_fine_is_the_band_search_from_the_half_size_match():
     assert base.min() >= -1 and base.max() <= 64
     fine = oracle.coarse_to_fine(left, right, band=64, margin=63)
     assert (fine == oracle.sgm_band(left, right, base, 128, 64)).all()
+
+
+# ---- resize_area (Decode, image.ex:402-476) -------------------------------------------------
+
+def test_resize_area_twofold_is_the_rounded_block_mean():
+    # Twofold: weights 0.5 and 0.5, every sum a quarter of an integer, exact in f32; round
+    # half away from zero is then (sum + 2) // 4.
+    frame = np.random.default_rng(11).integers(0, 256, (18, 26), dtype=np.uint8)
+    assert (oracle.resize_area(frame, (9, 13)) == oracle.shrink(frame)).all()
+
+
+def test_resize_area_closed_forms():
+    frame = np.random.default_rng(12).integers(0, 256, (7, 9), dtype=np.uint8)
+    assert (oracle.resize_area(frame, (7, 9)) == frame).all()
+    assert (oracle.resize_area(np.full((21, 33), 201, np.uint8), (8, 10)) == 201).all()
+    # 3 -> 2 columns: taps (0, 2/3), (1, 1/3) and (1, 1/3), (2, 2/3).
+    row = np.array([[30, 60, 90]], np.uint8)
+    assert oracle.resize_area(row, (1, 2)).tolist() == [[40, 80]]
+    # 5 -> 2 rows: taps (0, .4) (1, .4) (2, .2) and (2, .2) (3, .4) (4, .4).
+    column = np.array([[10], [20], [100], [40], [50]], np.uint8)
+    assert oracle.resize_area(column, (2, 1)).tolist() == [[32], [56]]
+    # 4 -> 3 of a ramp: taps (0, .75) (1, .25); (1, .5) (2, .5); (2, .25) (3, .75).
+    ramp = np.array([[0, 40, 80, 120]], np.uint8)
+    assert oracle.resize_area(ramp, (1, 3)).tolist() == [[10, 60, 110]]
