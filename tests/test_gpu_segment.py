@@ -382,8 +382,8 @@ def test_full_pipeline_728x544x128(context):
 def test_refusals_map_like_the_port(context):
     geometry, _ = rig_geometry(32, 24)
     g = {k: geometry[k] for k in ("width", "height", "frame_width", "frame_height")}
-    reply = context.request(protocol.configure(9001, g, [("decode", [])]))
-    assert reply.reason == "no HIP implementation of decode"
+    reply = context.request(protocol.configure(9001, g, [("colorize", [])]))
+    assert reply.reason == "no HIP implementation of colorize"
     reply = context.request(protocol.run(9001, [], []))
     assert reply.reason == "segment 9001 is not configured"
     reply = context.request(protocol.configure(9001, g, [("temporal_filter", [("max_missed", 255.0),
@@ -412,7 +412,7 @@ def test_refusals_map_like_the_port(context):
 def test_a_refused_reconfigure_leaves_the_segment_unconfigured(context):
     g = {"width": 32, "height": 24, "frame_width": 32, "frame_height": 24}
     assert context.request(protocol.configure(77, g, [("depth", [])])) is True
-    assert isinstance(context.request(protocol.configure(77, g, [("decode", [])])), protocol.Refused)
+    assert isinstance(context.request(protocol.configure(77, g, [("colorize", [])])), protocol.Refused)
     assert context.request(protocol.run(77, [[]], [])).reason == "segment 77 is not configured"
 
 
