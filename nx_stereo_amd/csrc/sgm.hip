@@ -50,22 +50,28 @@ __device__ __forceinline__ PairDesc pair_of(const PairDesc &single, const PairDe
 // wave's LDS reads are of consecutive features (no bank conflicts); the bytes go through
 // an LDS tile and leave as one contiguous run of tile*D bytes, 16 bytes per thread.
 
-constexpr int COST_BLOCK = 1024;
 constexpr int MAX_D = 256;
 constexpr int COST_PITCH_PAD = 16;  // LDS bytes between pixels of the staged tile, spreading banks
-// The staged tile: 16 cost bytes per thread, and a pad per pixel (at most 256 pixels, D = 64).
-constexpr int COST_STAGED_BYTES = COST_BLOCK * 16 + COST_BLOCK / 4 * COST_PITCH_PAD;
+// The staged tile of a block of `block` threads: 16 cost bytes per thread, and a pad per
+// pixel (at most block / 4 pixels, D = 64).
+__host__ __device__ constexpr int cost_staged_bytes(int block) {
+    return block * 16 + block / 4 * COST_PITCH_PAD;
+}
 
 // LDS bytes of one block: the staged tile, the two census rows (the right one with D zero
 // features in front), then 7 rows of each image.
-__host__ __device__ constexpr size_t census_cost_lds(int W, int D) {
-    return (size_t)COST_STAGED_BYTES + 8 * (size_t)(2 * W + D) + 14 * (size_t)W;
+__host__ __device__ constexpr size_t census_cost_lds(int block, int W, int D) {
+    return (size_t)cost_staged_bytes(block) + 8 * (size_t)(2 * W + D) + 14 * (size_t)W;
 }
 
-// Grid y: the row; z: the pair. keep_census: also store both census images.
+// Grid y: the row; z: the pair. keep_census: also store both census images. COST_BLOCK
+// threads: 512 where every row of the launch is in flight at once (four blocks a CU), else
+// 1024 (launch_all).
+template <int COST_BLOCK>
 __global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single,
                                                                  const PairDesc *table, int W,
                                                                  int H, int D, int keep_census) {
+    constexpr int COST_STAGED_BYTES = cost_staged_bytes(COST_BLOCK);
     extern __shared__ __align__(16) uint8_t lds[];
     const PairDesc pd = pair_of(single, table, blockIdx.z);
     const int y = blockIdx.y;
@@ -1321,13 +1327,38 @@ static hipError_t launch_all(const MatchParams &p, const PairDesc &single, const
     const int W = p.width, H = p.height, D = p.disparities;
     if (W > MAX_WIDTH || W < 1 || H < 1 || count < 1) return hipErrorInvalidValue;
     // Above 64 KB of LDS a kernel must say so first (up to the CU's 160 KB).
-    static const hipError_t lds_ok = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(census_cost_kernel),
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)census_cost_lds(MAX_WIDTH, MAX_D));
+    static const hipError_t lds_ok = [] {
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void *>(census_cost_kernel<512>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)census_cost_lds(512, MAX_WIDTH, MAX_D));
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void *>(census_cost_kernel<1024>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)census_cost_lds(1024, MAX_WIDTH, MAX_D));
+        return e;
+    }();
     if (lds_ok != hipSuccess) return lds_ok;
+    // 512-thread blocks when every row of the launch runs at once (four blocks a CU): the
+    // cost volume is then written in one sweep, and at 728x544x128, whose volumes fill the
+    // 256 MB Infinity Cache, the path kernel after it runs 8% faster (180 -> 165 us).
+    // Otherwise 1024 (at 1920x1080x256 the census and costs take 201 us so, 243 with 512).
+    static const int compute_units = [] {
+        int device = 0, n = 0;
+        if (hipGetDevice(&device) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+            n = 0;
+        return n > 0 ? n : 256;
+    }();
+    const bool one_sweep = (size_t)H * count <= 4 * (size_t)compute_units;
     int slot = profile_begin(PK_COST, stream);
-    hipLaunchKernelGGL(census_cost_kernel, dim3(1, H, count), dim3(COST_BLOCK),
-                       census_cost_lds(W, D), stream, single, table, W, H, D, p.keep_stages ? 1 : 0);
+    if (one_sweep)
+        hipLaunchKernelGGL(census_cost_kernel<512>, dim3(1, H, count), dim3(512),
+                           census_cost_lds(512, W, D), stream, single, table, W, H, D,
+                           p.keep_stages ? 1 : 0);
+    else
+        hipLaunchKernelGGL(census_cost_kernel<1024>, dim3(1, H, count), dim3(1024),
+                           census_cost_lds(1024, W, D), stream, single, table, W, H, D,
+                           p.keep_stages ? 1 : 0);
     profile_end(slot, stream);
     hipError_t e;
     switch (D) {
