@@ -10,10 +10,11 @@
 // one pixel's band is contiguous; one u8 path volume per aggregation direction.
 //
 // Aggregation: each line of a direction is owned by `band` consecutive lanes, lane k holding
-// candidate base + k as a u32. A path moving to a pixel with another base moves its values
-// by the difference (ds_bpermute within the line's lanes); a candidate the previous band
-// lacks is reached only by the p2 jump (or, one past its ends, from the end + p1). Path
-// values stay <= p2 + 48 <= 127, so every direction stores bytes and selection adds them.
+// candidate base + k in a u32. A path moving to a pixel with another base moves its values
+// by the difference (a DPP move for one, ds_bpermute within the line's lanes beyond); a
+// candidate the previous band lacks is reached only by the p2 jump (or, one past its ends,
+// from the end + p1). Path values stay <= p2 + 48 <= 127, so every direction stores bytes
+// and selection adds them.
 #include <type_traits>
 
 #include "device.hpp"
@@ -181,10 +182,11 @@ __global__ __launch_bounds__(BAND_COST_BLOCK) void band_cost_kernel(
 //             min(v[band-1] + p1, m + p2) - m + C          for k + delta = band,
 //             p2 + C                                       further out,
 // which is scan_band's min(window, p2, neighbours + p1) over the moved, normalised values.
-// A line entering the image starts from v = 0 in its own band (delta 0). The move is a
-// ds_bpermute, taken only when a line of the wave changes base.
+// A line entering the image starts from v = 0 in its own band (delta 0). The walk keeps
+// u = 1024 - v rather than v (BandLine below), and does the move only on steps where a line
+// of the wave changes base.
 
-// Neighbouring lanes of one line and the minimum over its B lanes (B = 8, 16, 32 or 64).
+// Neighbouring lanes of one line and the maximum over its B lanes (B = 8, 16, 32 or 64).
 template <int B>
 struct BandLanes {
     static __device__ __forceinline__ uint32_t from_left(uint32_t edge, uint32_t v, int lane) {
@@ -249,8 +251,8 @@ struct BandLine {
         const uint32_t part = ::max(u, near - p1);  // near >= K - 127 > p1
         const uint32_t m = BandLanes<B>::max(u);
         uint32_t sel = part;
-        // Rare (a base changes on under 1% of steps in smooth scenes): laid out away from the
-        // common step, which then falls through without a taken branch.
+        // Rare (coarse-to-fine's bases change on about 5% of a line's steps): laid out away
+        // from the common step, which then falls through without a taken branch.
         if (__builtin_expect(moving_here, false)) {
             const int d = delta_of();
             const uint32_t vp = u - p1;

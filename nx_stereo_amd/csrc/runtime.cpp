@@ -70,10 +70,20 @@ static std::map<std::tuple<int, int, int, int>, Schedule> g_schedules;
 // line's sequential walk. Loads 4 steps ahead below 256 disparities, 8 at 256, where the
 // aggregation is bound by HBM traffic (the fastest measured on MI355X at 728x544x128,
 // 1280x720x192 and 1920x1080x256). With 3 paths: one wave per line.
+// 8 paths: two waves per line, prefetch 4 below 256 disparities and 8 at 256. 3 paths (a
+// wave walks a row both ways): prefetch 2, and 32 lanes a line from 128 disparities (K = 4
+// or more a lane). Measured: 728x544x128 3 paths 155 us (8/16/1: over 229), 364x272x64
+// 3 paths 71 us (32 lanes: 77; prefetch 8: 92).
 static Schedule default_schedule(int disparities, int paths) {
     Schedule s;
-    s.prefetch = paths == 8 && disparities < 256 ? 4 : 8;
-    s.waves = paths == 8 ? 2 : 1;
+    if (paths == 8) {
+        s.prefetch = disparities < 256 ? 4 : 8;
+        s.waves = 2;
+    } else {
+        s.prefetch = 2;
+        s.lanes = disparities >= 128 ? 32 : 16;
+        s.waves = 1;
+    }
     return s;
 }
 
