@@ -84,23 +84,30 @@ __global__ __launch_bounds__(256) void band_base_kernel(const float *coarse, int
 
 constexpr int BAND_COST_BLOCK = 512;
 
-__host__ __device__ constexpr size_t band_cost_lds(int W) { return 16 * (size_t)W + 14 * (size_t)W; }
+// The image rows in LDS carry BAND_REACH replicated pixels on each side, so the census reads
+// its 9 columns without clamping each one.
+constexpr int BAND_REACH = 4;
 
-// Census of pixel x of the middle row of `img` (7 rows of W). WINDOW 0: sparse_9x7 (dy in
-// -3..3, dx in -4..4, both non-zero, 48 bits); 1: dense_9x5 (dy in -2..2, dx in -4..4 but
-// the centre, 44 bits); 2: dense_9x5 against the blended anchor (1,2,1)x(1,2,1) over the
-// 3x3 plus 16 centre plus 24, over 32. Bit order row-major over the window.
+__host__ __device__ constexpr size_t band_cost_lds(int W) {
+    return 16 * (size_t)W + 14 * (size_t)(W + 2 * BAND_REACH);
+}
+
+// Census of the pixel at `img`, in the middle of 7 padded rows `pitch` apart. WINDOW 0:
+// sparse_9x7 (dy in -3..3, dx in -4..4, both non-zero, 48 bits); 1: dense_9x5 (dy in
+// -2..2, dx in -4..4 but the centre, 44 bits); 2: dense_9x5 against the blended anchor
+// (1,2,1)x(1,2,1) over the 3x3 plus 16 centre plus 24, over 32. Bit order row-major over
+// the window.
 template <int WINDOW>
-__device__ __forceinline__ uint64_t census_at(const uint8_t *img, int x, int W) {
-    const uint8_t *mid = img + 3 * W;
-    int anchor = mid[x];
+__device__ __forceinline__ uint64_t census_at(const uint8_t *img, int pitch) {
+    const uint8_t *mid = img + 3 * pitch;
+    int anchor = mid[0];
     if constexpr (WINDOW == 2) {
         int sum = 0;
 #pragma unroll
         for (int dy = -1; dy <= 1; dy++)
 #pragma unroll
             for (int dx = -1; dx <= 1; dx++)
-                sum += (dy ? 1 : 2) * (dx ? 1 : 2) * img[(3 + dy) * W + min(max(x + dx, 0), W - 1)];
+                sum += (dy ? 1 : 2) * (dx ? 1 : 2) * mid[dy * pitch + dx];
         anchor = (sum + 16 * anchor + 24) >> 5;
     }
     constexpr int REACH = WINDOW == 0 ? 3 : 2;
@@ -109,11 +116,11 @@ __device__ __forceinline__ uint64_t census_at(const uint8_t *img, int x, int W) 
 #pragma unroll
     for (int dy = -REACH; dy <= REACH; dy++) {
         if (WINDOW == 0 && dy == 0) continue;
-        const uint8_t *row = img + (3 + dy) * W;
+        const uint8_t *row = img + (3 + dy) * pitch;
 #pragma unroll
         for (int dx = -4; dx <= 4; dx++) {
             if (dx == 0 && (WINDOW == 0 || dy == 0)) continue;
-            f |= (uint64_t)((int)row[min(max(x + dx, 0), W - 1)] + 1 < anchor) << bit;
+            f |= (uint64_t)((int)row[dx] + 1 < anchor) << bit;
             bit++;
         }
     }
@@ -128,20 +135,22 @@ __global__ __launch_bounds__(BAND_COST_BLOCK) void band_cost_kernel(
     extern __shared__ __align__(16) uint8_t band_lds[];
     uint64_t *cl = reinterpret_cast<uint64_t *>(band_lds);  // [W]
     uint64_t *cr = cl + W;                                    // [W]
-    uint8_t *rows = reinterpret_cast<uint8_t *>(cr + W);      // [2][7][W]
+    const int pitch = W + 2 * BAND_REACH;
+    uint8_t *rows = reinterpret_cast<uint8_t *>(cr + W);      // [2][7][pitch]
     const int y = blockIdx.x;
     for (int side = 0; side < 2; side++) {
         const uint8_t *img = side ? right : left;
         for (int r = 0; r < 7; r++) {
             const uint8_t *src = img + (size_t)min(max(y + r - 3, 0), H - 1) * W;
-            uint8_t *dst = rows + (side * 7 + r) * W;
-            for (int x = threadIdx.x; x < W; x += BAND_COST_BLOCK) dst[x] = src[x];
+            uint8_t *dst = rows + (side * 7 + r) * pitch;
+            for (int i = threadIdx.x; i < pitch; i += BAND_COST_BLOCK)
+                dst[i] = src[min(max(i - BAND_REACH, 0), W - 1)];
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 2 * W; i += BAND_COST_BLOCK) {
         const int side = i >= W ? 1 : 0, x = i - side * W;
-        const uint64_t f = census_at<WINDOW>(rows + side * 7 * W, x, W);
+        const uint64_t f = census_at<WINDOW>(rows + side * 7 * pitch + BAND_REACH + x, pitch);
         (side ? cr : cl)[x] = f;
         if (census_left) (side ? census_right : census_left)[(size_t)y * W + x] = f;
     }

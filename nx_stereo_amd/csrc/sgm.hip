@@ -58,10 +58,15 @@ __host__ __device__ constexpr int cost_staged_bytes(int block) {
     return block * 16 + block / 4 * COST_PITCH_PAD;
 }
 
+// The image rows in LDS carry CENSUS_REACH replicated pixels on each side, so the census
+// reads its 9 columns without clamping each one.
+constexpr int CENSUS_REACH = 4;
+
 // LDS bytes of one block: the staged tile, the two census rows (the right one with D zero
-// features in front), then 7 rows of each image.
+// features in front), then 7 padded rows of each image.
 __host__ __device__ constexpr size_t census_cost_lds(int block, int W, int D) {
-    return (size_t)cost_staged_bytes(block) + 8 * (size_t)(2 * W + D) + 14 * (size_t)W;
+    return (size_t)cost_staged_bytes(block) + 8 * (size_t)(2 * W + D) +
+           14 * (size_t)(W + 2 * CENSUS_REACH);
 }
 
 // Grid y: the row; z: the pair. keep_census: also store both census images. COST_BLOCK
@@ -78,32 +83,33 @@ __global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single
     uint4 *staged = reinterpret_cast<uint4 *>(lds);
     uint64_t *cl = reinterpret_cast<uint64_t *>(lds + COST_STAGED_BYTES);  // [W]
     uint64_t *cr = cl + W;                                                  // [D zeros, W]
-    uint8_t *rows = reinterpret_cast<uint8_t *>(cr + D + W);                // [2][7][W]
+    const int row_pitch = W + 2 * CENSUS_REACH;                             // padded row
+    uint8_t *rows = reinterpret_cast<uint8_t *>(cr + D + W);                // [2][7][row_pitch]
     for (int side = 0; side < 2; side++) {
         const uint8_t *img = side ? pd.right : pd.left;
         for (int r = 0; r < 7; r++) {
             const uint8_t *src = img + (size_t)min(max(y + r - 3, 0), H - 1) * W;
-            uint8_t *dst = rows + (side * 7 + r) * W;
-            for (int x = threadIdx.x; x < W; x += COST_BLOCK) dst[x] = src[x];
+            uint8_t *dst = rows + (side * 7 + r) * row_pitch;
+            for (int i = threadIdx.x; i < row_pitch; i += COST_BLOCK)
+                dst[i] = src[min(max(i - CENSUS_REACH, 0), W - 1)];
         }
     }
     for (int i = threadIdx.x; i < D; i += COST_BLOCK) cr[i] = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < 2 * W; i += COST_BLOCK) {
         const int side = i >= W ? 1 : 0, x = i - side * W;
-        const uint8_t *img = rows + side * 7 * W;
-        const int centre = img[3 * W + x];
+        const uint8_t *img = rows + side * 7 * row_pitch + CENSUS_REACH + x;  // column x
+        const int centre = img[3 * row_pitch];
         uint64_t f = 0;
         int bit = 0;
 #pragma unroll
         for (int dy = -3; dy <= 3; dy++) {
             if (dy == 0) continue;
-            const uint8_t *row = img + (3 + dy) * W;
+            const uint8_t *row = img + (3 + dy) * row_pitch;
 #pragma unroll
             for (int dx = -4; dx <= 4; dx++) {
                 if (dx == 0) continue;
-                const int sx = min(max(x + dx, 0), W - 1);
-                f |= (uint64_t)((int)row[sx] + 1 < centre) << bit;
+                f |= (uint64_t)((int)row[dx] + 1 < centre) << bit;
                 bit++;
             }
         }
