@@ -265,6 +265,52 @@ def coarse_to_fine_leg(runs: int = 20) -> dict:
     return out
 
 
+def streams_leg(pairs: int = 96) -> dict:
+    """The main-line pipeline (and the same with search coarse_to_fine) as 1, 2 and 4
+    segments, each on its own stream, frames resident, pairs enqueued round-robin: the
+    throughput of several camera streams on one GPU. Each segment keeps its own carries."""
+    from nx_stereo_amd import hip, segment, stages
+
+    geometry, members, left_frame, right_frame = workload()
+    c2f = [(name, stages.Disparity(num_disparities=128, paths=PATHS, search="coarse_to_fine")
+            if name == "disparity" else stage) for name, stage in members]
+    frames = [hip.Buffer.of(f) for f in (left_frame, right_frame)]
+    entries = [("left_frame", 0, (HEIGHT, WIDTH), frames[0].ptr),
+               ("right_frame", 0, (HEIGHT, WIDTH), frames[1].ptr)]
+    out = {"workload": "the main-line pipeline as n segments on n streams, pairs round-robin, "
+                       "frames resident; pairs/s over all segments"}
+    for name, stage_list in (("full", members), ("coarse_to_fine", c2f)):
+        rates = {}
+        for n in (1, 2, 4):
+            context = segment.Context(0)
+            records = []
+            for _ in range(n):
+                ok, seg = segment.HipSegment.build(stage_list, geometry, context)
+                if context.request(seg.configure_record()) is not True:
+                    raise RuntimeError("configure refused")
+                seg.configured_version = seg.version
+                records.append(seg.run_record({}))
+            streams = [hip.Stream() for _ in range(n)]
+
+            def round_robin(count):
+                for k in range(count):
+                    context.run_device(records[k % n], entries, stream=streams[k % n].handle)
+
+            round_robin(4 * n)
+            hip.synchronize()
+            start = time.perf_counter()
+            round_robin(pairs)
+            hip.synchronize()
+            rates[str(n)] = round(pairs / (time.perf_counter() - start), 1)
+            for st in streams:
+                st.destroy()
+            context.close()
+        out[name] = rates
+    for f in frames:
+        f.free()
+    return out
+
+
 BATCH_CLASSES = [(480, 270, 64, 24), (728, 544, 128, 48), (1280, 720, 192, 24)]
 
 
@@ -476,9 +522,9 @@ def main() -> int:
     parser.add_argument("--steps", type=int, default=200)
     parser.add_argument("--warmup", type=int, default=20)
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    parser.add_argument("--legs", default="large,batched,served,coarse_to_fine",
+    parser.add_argument("--legs", default="large,batched,served,coarse_to_fine,streams",
                         help="comma-separated extra legs: large, batched, served, "
-                             "coarse_to_fine (or none)")
+                             "coarse_to_fine, streams (or none)")
     parser.add_argument("--kernels-only", choices=sorted(TRAFFIC_FILES),
                         help="run three matchings and the calibration copy (c2f*: five "
                              "coarse-to-fine matchings), for rocprofv3")
@@ -616,6 +662,8 @@ def main() -> int:
         result["served"] = served_leg()
     if "coarse_to_fine" in legs:
         result["coarse_to_fine"] = coarse_to_fine_leg()
+    if "streams" in legs:
+        result["streams"] = streams_leg()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(geometry, members, left_frame, right_frame)
     print(json.dumps(result), flush=True)
