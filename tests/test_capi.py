@@ -53,3 +53,29 @@ def test_match_refuses_what_the_reference_refuses_before_any_device_work(args, r
     left = np.zeros((h, w), np.uint8)
     with pytest.raises(_native.NativeError, match=reason):
         _native.match_host(left, left, d, paths, p1, p2, u)
+
+
+@pytest.mark.parametrize("kwargs,reason", [
+    (dict(disparities=192), "disparities must be 64 or 128"),
+    (dict(paths=4), "paths must be 3 or 8"),
+    (dict(p1=10, p2=10), "0 <= p1 < p2"),
+    (dict(uniqueness=0.5), "uniqueness"),
+])
+def test_band_search_refuses_before_any_device_work(kwargs, reason):
+    """nx_stereo.ex:88-131, 286-308: sgm_band/4's checks, in the library."""
+    left = np.zeros((16, 80), np.uint8)
+    with pytest.raises(_native.NativeError, match=reason):
+        _native.sgm_band(left, left, np.zeros((16, 80), np.int32), **kwargs)
+
+
+@pytest.mark.parametrize("shape,kwargs,reason", [
+    ((32, 80), dict(margin=-1), "margin must be"),
+    ((32, 80), dict(coarse_paths=5), "paths must be 3 or 8"),
+    ((16, 16), {}, "wider than 8"),  # halves of 8x8 are too small for the 64-disparity match
+    ((32, 80), dict(band=12), "band must be 8, 16, 32 or 64"),
+])
+def test_coarse_to_fine_refuses_before_any_device_work(shape, kwargs, reason):
+    """nx_stereo.ex:164-185: sgm_coarse_to_fine/3's checks, in the library."""
+    left = np.zeros(shape, np.uint8)
+    with pytest.raises(_native.NativeError, match=reason):
+        _native.sgm_coarse_to_fine(left, left, **kwargs)
