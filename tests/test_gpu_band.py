@@ -137,3 +137,34 @@ def test_the_library_refuses_what_sgm_band_and_coarse_to_fine_refuse():
     # Halves of 16 x 8 are too small for the 64-disparity match (wider than 8 needed).
     with pytest.raises(_native.NativeError, match="wider than 8"):
         _native.sgm_coarse_to_fine(*pair(16, 16, 3))
+
+
+EXTREME_CASES = [
+    ((14, 4096), dict(band=64, margin=3)),            # the widest image, the lowest height
+    ((14, 4096), dict(band=8, paths=3, coarse_paths=3)),
+    ((14, 18), {}),                                   # the smallest: halves of 9 x 7
+    ((30, 250), dict(band=32, fine_census="dense_9x5")),  # widths off every line multiple
+    ((202, 66), dict(band=16, margin=0)),             # taller than wide
+]
+
+
+@pytest.mark.parametrize("size,opts", EXTREME_CASES)
+def test_coarse_to_fine_at_the_extremes(size, opts):
+    h, w = size
+    print(f"coarse_to_fine extreme {w}x{h} {opts}", flush=True)
+    left, right, _ = scenes.planar_pair(w, h, 13, low=1.0, high=min(110.0, w - 6))
+    got = _native.sgm_coarse_to_fine(left, right, intermediates=True, **opts)
+    expected = oracle.coarse_to_fine(left, right, **opts)
+    assert (got["disparity"].view(np.uint32) == expected.view(np.uint32)).all()
+
+
+@pytest.mark.parametrize("band", [16, 64])
+def test_band_search_at_the_widest_image(band):
+    h, w = 9, 4096
+    print(f"band {band} at {w}x{h}", flush=True)
+    left, right, _ = scenes.planar_pair(w, h, 14, low=2.0, high=100.0)
+    base = varying_bases(np.random.default_rng(band), h, w, band, 128)
+    got = _native.sgm_band(left, right, base, 128, band, 8, intermediates=True)
+    expected = oracle_band_stages(left, right, base, 128, band, 8, "sparse_9x7", 8, 64,
+                                  1.0526315789473684)
+    assert_stages_equal(got, expected)
