@@ -66,24 +66,18 @@ void profile_end(int slot, hipStream_t stream) {
 static std::mutex g_schedules_mu;
 static std::map<std::tuple<int, int, int, int>, Schedule> g_schedules;
 
-// Without a pick, with 8 paths: two waves per line (one per direction), which halves a
-// line's sequential walk. Loads 4 steps ahead below 256 disparities, 8 at 256, where the
-// aggregation is bound by HBM traffic (the fastest measured on MI355X at 728x544x128,
-// 1280x720x192 and 1920x1080x256). With 3 paths: one wave per line.
-// 8 paths: two waves per line, prefetch 4 below 256 disparities and 8 at 256. 3 paths (a
-// wave walks a row both ways): prefetch 2, and 32 lanes a line from 128 disparities (K = 4
-// or more a lane). Measured: 728x544x128 3 paths 155 us (8/16/1: over 229), 364x272x64
-// 3 paths 71 us (32 lanes: 77; prefetch 8: 92).
+// Without a pick: two waves per line (one per direction of a row), which halves a line's
+// sequential walk. With 8 paths, loads 4 steps ahead below 256 disparities and 8 at 256,
+// where the aggregation is bound by HBM traffic (the fastest measured on MI355X at
+// 728x544x128, 1280x720x192 and 1920x1080x256).
 static Schedule default_schedule(int disparities, int paths) {
     Schedule s;
-    if (paths == 8) {
-        s.prefetch = disparities < 256 ? 4 : 8;
-        s.waves = 2;
-    } else {
-        s.prefetch = 2;
-        s.lanes = disparities >= 128 ? 32 : 16;
-        s.waves = 1;
-    }
+    // At 3 paths the rows' walk bounds the kernel (the down-only columns take four lines a
+    // wave), and a prefetch of 8 covers it best: 38 us at 364x272x64 (coarse-to-fine's half
+    // images) and 89 us at 728x544x128, against 71 and 156 us with one wave a line.
+    s.prefetch = paths == 8 && disparities < 256 ? 4 : 8;
+    s.lanes = 16;
+    s.waves = 2;
     return s;
 }
 
@@ -1882,10 +1876,9 @@ struct nxs_batch {
         for (int lanes : {16, 32, 64})
             if (lanes_fit(c.p.disparities, lanes))
                 for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, lanes, 1});
-        if (c.p.paths == 8)
-            for (int waves : {2, 4, 8})
-                if (waves == 2 || slices_fit(c.p.disparities, waves / 2) > 1)
-                    for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, 16, waves});
+        for (int waves : {2, 4, 8})
+            if (waves == 2 || (c.p.paths == 8 && slices_fit(c.p.disparities, waves / 2) > 1))
+                for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, 16, waves});
         candidates.push_back(Schedule{4, true, 16, 1});
         for (const Schedule &s : candidates) {
             HIP_TRY(launch_match_batch(c.p, tables + c.first, c.count, s, stream));
@@ -2025,6 +2018,8 @@ int nxs_schedule_set(int width, int height, int disparities, int paths, const nx
         return nxs::fail(NXS_EINVAL, "waves must be 1, 2, 4 or 8");
     if (s->waves > 1 && s->lanes != 16)
         return nxs::fail(NXS_EINVAL, "two or more waves per line take 16 lanes per line");
+    if (s->waves > 2 && paths != 8)
+        return nxs::fail(NXS_EINVAL, std::to_string(s->waves) + " waves per line take 8 paths");
     if (s->waves > 2 && nxs::slices_fit(disparities, s->waves / 2) == 1)
         return nxs::fail(NXS_EINVAL, std::to_string(s->waves) + " waves per line do not fit " +
                                          std::to_string(disparities) + " disparities");

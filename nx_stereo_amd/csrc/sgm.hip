@@ -605,8 +605,9 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
 // A CPU model of these rules (tests/test_two_wave_rules.py: line lengths up to 260, blocks
 // of unequal lines, each prefetch depth, the wait's count over the accesses actually issued)
 // checks that every read is of a completed store or a visible ring slot, and that phases A
-// and C need no mask. Only the eight-path orientations use it (rows, columns, both
-// diagonals).
+// and C need no mask. The eight-path orientations use it (rows, columns, both diagonals),
+// and the rows of 3 paths; the down-only columns of 3 paths have nothing to meet, so there
+// each wave walks four columns of its own.
 
 constexpr int TWO_BLOCK = 128;
 
@@ -632,9 +633,55 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
             o = k;
             break;
         }
-    const int first_line = (b - a.block_start[o]) * LINES;
     const int W = a.W, H = a.H;
     const PairDesc pd = pair_of(a.single, a.table, blockIdx.y);
+    constexpr uint32_t D_BYTES = 16 * K;
+    if (o == 1 && !a.backward_columns) {
+        // Columns with 3 paths: down only. The block takes eight columns, each wave four of
+        // its own, walked alone (no meeting, no barrier), storing its own path costs.
+        const int first_column = (b - a.block_start[o]) * 2 * LINES + dir * LINES;
+        const LineGeometry gc = line_geometry(1, first_column + row, W, H);
+        int longest_column = 0;
+#pragma unroll
+        for (int r = 0; r < LINES; r++)
+            longest_column = max(longest_column, line_geometry(1, first_column + r, W, H).length);
+        if (longest_column == 0) return;  // wave-uniform
+        const int Lc = gc.length;
+        const uint32_t stride_c = (uint32_t)W * D_BYTES;
+        const uint32_t volume_c = (uint32_t)W * H * D_BYTES;
+        const Rsrc costs_c = buffer_of(pd.b.costs, volume_c);
+        const Rsrc paired_c = buffer_of(orientation_volume(pd.b, 1), volume_c);
+        const uint32_t p1x2c = (uint32_t)pd.p1 * 0x10001u, p2c = (uint32_t)pd.p2;
+        const uint32_t first_c = (uint32_t)gc.x0 * D_BYTES + lane * K;
+        PathState<K, 16> sc;
+        sc.reset(p1x2c, lane);
+        Bytes<K> ring_c[PREFETCH];
+        Bytes<K> none_c;
+#pragma unroll
+        for (int k = 0; k < (K + 3) / 4; k++) none_c.w[k] = 0;
+        // As a step issues: a (masked) store, then the load.
+#pragma unroll
+        for (int p = 0; p < PREFETCH; p++) {
+            __builtin_amdgcn_sched_barrier(0);
+            vstore<K>(paired_c, MASKED, none_c);
+            ring_c[p] = vload<K>(costs_c, first_c + p * stride_c);
+        }
+        uint32_t at_c = first_c, ahead_c = first_c + PREFETCH * stride_c;
+        for (int t0 = 0; t0 < longest_column; t0 += PREFETCH) {
+#pragma unroll
+            for (int p = 0; p < PREFETCH; p++) {
+                const int t = t0 + p;
+                __builtin_amdgcn_sched_barrier(0);
+                sc.step(ring_c[p], p1x2c, p2c, lane);
+                vstore<K>(paired_c, t < Lc ? at_c : MASKED, sc.bytes());
+                ring_c[p] = vload<K>(costs_c, ahead_c);
+                at_c += stride_c;
+                ahead_c += stride_c;
+            }
+        }
+        return;
+    }
+    const int first_line = (b - a.block_start[o]) * LINES;
     const LineGeometry g = line_geometry(o, first_line + row, W, H);
     // Block-uniform: the longest line, and the window around the lines' middles (in 2i).
     int longest = 0, mid_lo = 1 << 30, mid_hi = -(1 << 30);
@@ -1290,8 +1337,10 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
     a.W = W;
     a.H = H;
     a.backward_columns = p.paths == 8;
-    const bool two = sch.waves >= 2 && p.paths == 8;
-    const int slices = two ? slices_fit(16 * K, sch.waves / 2) : 1;
+    // Two waves per line at 8 paths, and at 3 paths for the rows (the down-only columns then
+    // take four lines a wave, eight a block); 4 or 8 waves (slices) at 8 paths only.
+    const bool two = sch.waves >= 2;
+    const int slices = two && p.paths == 8 ? slices_fit(16 * K, sch.waves / 2) : 1;
     const int per_block = two ? 4 : PATH_BLOCK / (lanes_fit(16 * K, sch.lanes) ? sch.lanes : 16);
     auto launch = [&](int first, int last) {
         a.first = first;
@@ -1299,7 +1348,8 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
         int blocks = 0;
         for (int o = 0; o < 4; o++) {
             a.block_start[o] = blocks;
-            if (o >= first && o <= last) blocks += (lines[o] + per_block - 1) / per_block;
+            const int lines_per_block = two && o == 1 && !a.backward_columns ? 8 : per_block;
+            if (o >= first && o <= last) blocks += (lines[o] + lines_per_block - 1) / lines_per_block;
         }
         if (two && slices > 1) {
             launch_sliced<16 * K>(a, blocks, count, sch.prefetch, slices, stream);

@@ -65,7 +65,8 @@ def test_every_schedule_gives_the_same_disparity(disparities, paths):
     ws = hip.Buffer(_native.workspace_size(w, h, disparities, paths))
     kept = _native.get_schedule(w, h, disparities, paths)
     try:
-        shapes = [(lanes, 1) for lanes in LANES[disparities]] + [(16, w) for w in WAVES[disparities]]
+        shapes = [(lanes, 1) for lanes in LANES[disparities]] + \
+            [(16, w) for w in WAVES[disparities] if paths == 8 or w == 2]
         for lanes, waves in shapes:
             for prefetch in (2, 4, 8):
                 for split in (False, True):
@@ -82,25 +83,27 @@ def test_every_schedule_gives_the_same_disparity(disparities, paths):
         _native.set_schedule(w, h, disparities, paths, *kept)
 
 
-@pytest.mark.parametrize("disparities", [64, 128])
-def test_several_waves_on_odd_and_short_lines(disparities):
+@pytest.mark.parametrize("disparities,paths", [(64, 8), (128, 8), (64, 3), (128, 3)])
+def test_several_waves_on_odd_and_short_lines(disparities, paths):
     """Two or more waves per line: lines shorter than the prefetch and LDS windows, odd and
-    even lengths (the middle pixel), every diagonal length from 1 up."""
+    even lengths (the middle pixel), every diagonal length from 1 up; at 3 paths, column
+    counts off the eight a block takes."""
     sizes = [(9, 7), (10, 8), (17, 40), (33, 17)]
-    kept = {size: _native.get_schedule(*size, disparities, 8) for size in sizes}
+    kept = {size: _native.get_schedule(*size, disparities, paths) for size in sizes}
     try:
         for w, h in sizes:
             left, right = scenes.scene(w, h, 3)
             want = oracle.aggregate(oracle.costs(oracle.census(left), oracle.census(right),
-                                                 disparities), 8, 8, 64)
-            for waves in WAVES[disparities]:
+                                                 disparities), paths, 8, 64)
+            for waves in WAVES[disparities] if paths == 8 else (2,):
                 for prefetch in (2, 4, 8):
-                    _native.set_schedule(w, h, disparities, 8, prefetch, False, 16, waves)
-                    sums = _native.match_host(left, right, disparities, 8, intermediates=True)["sums"]
+                    _native.set_schedule(w, h, disparities, paths, prefetch, False, 16, waves)
+                    sums = _native.match_host(left, right, disparities, paths,
+                                              intermediates=True)["sums"]
                     assert np.array_equal(sums, want), (w, h, waves, prefetch)
     finally:
         for size, schedule in kept.items():
-            _native.set_schedule(*size, disparities, 8, *schedule)
+            _native.set_schedule(*size, disparities, paths, *schedule)
 
 
 def test_lanes_that_do_not_fit_are_refused():
@@ -112,3 +115,5 @@ def test_lanes_that_do_not_fit_are_refused():
         _native.set_schedule(97, 31, 128, 8, 4, False, 32, 2)
     with pytest.raises(_native.NativeError, match="8 waves per line do not fit"):
         _native.set_schedule(97, 31, 64, 8, 4, False, 16, 8)
+    with pytest.raises(_native.NativeError, match="4 waves per line take 8 paths"):
+        _native.set_schedule(97, 31, 128, 3, 4, False, 16, 4)
