@@ -85,13 +85,21 @@ __global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single
     uint64_t *cr = cl + W;                                                  // [D zeros, W]
     const int row_pitch = W + 2 * CENSUS_REACH;                             // padded row
     uint8_t *rows = reinterpret_cast<uint8_t *>(cr + D + W);                // [2][7][row_pitch]
-    for (int side = 0; side < 2; side++) {
-        const uint8_t *img = side ? pd.right : pd.left;
-        for (int r = 0; r < 7; r++) {
-            const uint8_t *src = img + (size_t)min(max(y + r - 3, 0), H - 1) * W;
-            uint8_t *dst = rows + (side * 7 + r) * row_pitch;
-            for (int i = threadIdx.x; i < row_pitch; i += COST_BLOCK)
-                dst[i] = src[min(max(i - CENSUS_REACH, 0), W - 1)];
+    // A thread loads its byte of all 14 rows before it stores any, so that the loads are in
+    // flight together (a thread past the padded row loads a byte it does not store). Buffer
+    // loads: the rows' offsets are scalars, the column the one vector offset.
+    const Rsrc images[2] = {buffer_of(pd.left, (uint32_t)W * H), buffer_of(pd.right, (uint32_t)W * H)};
+    for (int i0 = 0; i0 < row_pitch; i0 += COST_BLOCK) {
+        const int i = i0 + (int)threadIdx.x;
+        const int column = min(max(i - CENSUS_REACH, 0), W - 1);
+        uint32_t v[14];
+#pragma unroll
+        for (int k = 0; k < 14; k++)
+            v[k] = __builtin_amdgcn_raw_buffer_load_b8(images[k / 7], column,
+                                                       min(max(y + k % 7 - 3, 0), H - 1) * W, 0);
+        if (i < row_pitch) {
+#pragma unroll
+            for (int k = 0; k < 14; k++) rows[k * row_pitch + i] = (uint8_t)v[k];
         }
     }
     for (int i = threadIdx.x; i < D; i += COST_BLOCK) cr[i] = 0;
@@ -982,13 +990,12 @@ __global__ __launch_bounds__(128 * G) void path_sliced_kernel(PathArgs a) {
 // for the lane's j-th disparity orders the unpenalised ones in 14 bits (penalised ones take
 // 0xFFFF). Sums, keys and the second-best search run on u16 pairs. The second best is the
 // least V outside [below, above]: those entries are lifted to 0xFFFD..0xFFFF, above every
-// V (at most 0x83F8). The pixel's first lane reads the winner's two neighbours back from
-// the four volumes.
+// V (at most 0x83F8). The winner's two neighbours come from the lanes holding them.
 
 // A block takes SELECT_GROUPS groups of 16 pixels of a row, one after another, loading the
 // next group's four volumes while it selects in the current one (so that each wave keeps
 // loads in flight).
-constexpr int SELECT_GROUPS = 4;
+constexpr int SELECT_GROUPS = 2;
 
 template <int K>
 __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const PairDesc *table,
@@ -1085,14 +1092,22 @@ __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const Pair
             smin = pk_min(smin, pk_max(value[r], lifted));
         }
         const uint32_t second = row_min_u32(pk_min_halves(smin)) & 0xFFFFu;
-        if (lane16 == 0 && valid) {
-            uint32_t lower = x < below ? 0x8000u : 0u, upper = x < above ? 0x8000u : 0u;
+        // The winner's two neighbours' sums, from the lanes that hold them: the owner of
+        // d = below gives S[below] in the low half (others 0xFFFF), of above in the high.
+        const int jb = below - d0, ja = above - d0;
+        uint32_t wb = sum[0], wa = sum[0];
 #pragma unroll
-            for (int m = 0; m < 4; m++) {
-                if (m >= orientations) break;
-                lower += __builtin_amdgcn_raw_buffer_load_b8(vol[m], pix * D + below, 0, 0);
-                upper += __builtin_amdgcn_raw_buffer_load_b8(vol[m], pix * D + above, 0, 0);
-            }
+        for (int r = 1; r < R; r++) {
+            wb = (jb >> 1) == r ? sum[r] : wb;
+            wa = (ja >> 1) == r ? sum[r] : wa;
+        }
+        const uint32_t sb = (jb & 1 ? wb >> 16 : wb) & 0xFFFFu;
+        const uint32_t sa = (ja & 1 ? wa >> 16 : wa) & 0xFFFFu;
+        const uint32_t near = row_min_pk(((uint32_t)jb < (uint32_t)K ? sb : 0xFFFFu) |
+                                         ((uint32_t)ja < (uint32_t)K ? sa : 0xFFFFu) << 16);
+        if (lane16 == 0 && valid) {
+            const uint32_t lower = (near & 0xFFFFu) + (x < below ? 0x8000u : 0u);
+            const uint32_t upper = (near >> 16) + (x < above ? 0x8000u : 0u);
             const float lower_f = (float)lower;
             const float upper_f = (float)upper;
             const float minimum_f32 = (float)minimum;
