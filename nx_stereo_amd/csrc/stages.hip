@@ -337,11 +337,11 @@ hipError_t launch_temporal(float *disparity, float *previous, float *missed, int
 //     It also moves the current disparity into the carried previous.
 //  2. One block: each column's total cast to f32 (and the total set back to zero for the
 //     next launch), then the columns of each group of QUALITY_COLS added by a fixed
-//     pairwise tree, and the groups in order.
+//     pairwise tree (a wave's shuffles), and the groups in order.
 
 constexpr int QUALITY_COLS = 64;
-constexpr int QUALITY_PHASES = 16;
-constexpr int QUALITY_ROWS = 64;
+constexpr int QUALITY_PHASES = 8;
+constexpr int QUALITY_ROWS = 16;
 constexpr int QUALITY_FINISH = 1024;
 
 __global__ __launch_bounds__(QUALITY_COLS * QUALITY_PHASES) void quality_chunks_kernel(
@@ -390,48 +390,36 @@ __global__ __launch_bounds__(QUALITY_COLS * QUALITY_PHASES) void quality_chunks_
     }
 }
 
-constexpr int QUALITY_FINISH_COLUMNS = 4;  // per thread: up to 4096 columns
+constexpr int QUALITY_FINISH_GROUPS = 64;  // groups of QUALITY_COLS: up to 4096 columns
 
 __global__ __launch_bounds__(QUALITY_FINISH) void quality_finish_kernel(
     double *__restrict__ totals, int W, float *__restrict__ sums5) {
-    extern __shared__ double columns[];  // up to a whole group past W, at zero
-    const int groups = (W + QUALITY_COLS - 1) / QUALITY_COLS, padded = groups * QUALITY_COLS;
-    // Every total this thread finishes, read (and set back to zero) at once.
-    float mine[5][QUALITY_FINISH_COLUMNS];
+    static_assert(QUALITY_COLS == 64, "a group of columns is one wave's lanes");
+    __shared__ double group_sum[5][QUALITY_FINISH_GROUPS];
+    const int groups = (W + QUALITY_COLS - 1) / QUALITY_COLS;
+    const int wave = threadIdx.x / QUALITY_COLS, lane = threadIdx.x % QUALITY_COLS;
+    // A wave per group: each column's total cast to f32 (a column past W is 0) and set back to
+    // zero for the next launch, then the pairwise tree, lane i + half into lane i.
+    for (int g = wave; g < groups; g += QUALITY_FINISH / QUALITY_COLS) {
+        const int x = g * QUALITY_COLS + lane;
+        double v[5];
 #pragma unroll
-    for (int k = 0; k < 5; k++)
+        for (int k = 0; k < 5; k++) v[k] = x < W ? (double)(float)totals[(size_t)k * W + x] : 0.0;
 #pragma unroll
-        for (int j = 0; j < QUALITY_FINISH_COLUMNS; j++) {
-            const int x = threadIdx.x + j * QUALITY_FINISH;
-            mine[k][j] = x < W ? (float)totals[(size_t)k * W + x] : 0.0f;
-        }
-#pragma unroll
-    for (int k = 0; k < 5; k++)
-#pragma unroll
-        for (int j = 0; j < QUALITY_FINISH_COLUMNS; j++) {
-            const int x = threadIdx.x + j * QUALITY_FINISH;
+        for (int k = 0; k < 5; k++)
             if (x < W) totals[(size_t)k * W + x] = 0.0;
-        }
-    for (int k = 0; k < 5; k++) {
+        for (int half = QUALITY_COLS / 2; half > 0; half >>= 1)
 #pragma unroll
-        for (int j = 0; j < QUALITY_FINISH_COLUMNS; j++) {
-            const int x = threadIdx.x + j * QUALITY_FINISH;
-            if (x < padded) columns[x] = (double)mine[k][j];
-        }
-        __syncthreads();
-        for (int half = QUALITY_COLS / 2; half > 0; half >>= 1) {
-            for (int i = threadIdx.x; i < groups * half; i += QUALITY_FINISH) {
-                const int g = i / half, lx = i - g * half;
-                columns[g * QUALITY_COLS + lx] += columns[g * QUALITY_COLS + lx + half];
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            double total = 0;
-            for (int g = 0; g < groups; g++) total += columns[g * QUALITY_COLS];
-            sums5[k] = (float)total;
-        }
-        __syncthreads();
+            for (int k = 0; k < 5; k++) v[k] += __shfl_down(v[k], half, QUALITY_COLS);
+        if (lane == 0)
+#pragma unroll
+            for (int k = 0; k < 5; k++) group_sum[k][g] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {  // the groups in order
+        double total = 0;
+        for (int g = 0; g < groups; g++) total += group_sum[threadIdx.x][g];
+        sums5[threadIdx.x] = (float)total;
     }
 }
 
@@ -442,14 +430,14 @@ size_t quality_scratch_bytes(int W, int H) {
 
 hipError_t launch_quality(const float *disparity, float *previous, int W, int H, double *partials,
                           float *sums5, hipStream_t stream) {
-    if (W > QUALITY_FINISH * QUALITY_FINISH_COLUMNS) return hipErrorInvalidValue;
+    if (W > QUALITY_FINISH_GROUPS * QUALITY_COLS) return hipErrorInvalidValue;
     const int groups = (W + QUALITY_COLS - 1) / QUALITY_COLS;
     const int n_chunks = (H + QUALITY_ROWS - 1) / QUALITY_ROWS;
     hipLaunchKernelGGL(quality_chunks_kernel, dim3(groups, n_chunks),
                        dim3(QUALITY_COLS * QUALITY_PHASES), 0, stream, disparity, previous, W, H,
                        partials);
-    hipLaunchKernelGGL(quality_finish_kernel, dim3(1), dim3(QUALITY_FINISH),
-                       (size_t)groups * QUALITY_COLS * sizeof(double), stream, partials, W, sums5);
+    hipLaunchKernelGGL(quality_finish_kernel, dim3(1), dim3(QUALITY_FINISH), 0, stream, partials,
+                       W, sums5);
     return hipGetLastError();
 }
 
