@@ -1188,23 +1188,30 @@ __global__ __launch_bounds__(CHECK_BLOCK) void check_median_kernel(PairDesc sing
     for (int r = 0; r < 3; r++) rows[r] = min(max(y - 1 + r, 0), H - 1);
     for (int i = threadIdx.x; i < 3 * W; i += blockDim.x) right[i] = INVALID_KEY;
     __syncthreads();
+    // The three rows' keys and selected values are loaded together; the selected values wait
+    // in `checked`, which the check then overwrites pixel by pixel (each its own).
+    for (int x = threadIdx.x; x < W; x += blockDim.x) {
+        uint32_t key[3];
+        float sel[3];
 #pragma unroll
-    for (int r = 0; r < 3; r++) {
-        const uint32_t *row = pd.b.keys + (size_t)rows[r] * W;
-        for (int x = threadIdx.x; x < W; x += blockDim.x) {
-            const uint32_t key = row[x];
-            if (key != INVALID_KEY) {
-                const uint32_t w = key & 0xFFFFu;
-                atomicMin(&right[r * W + x - (int)w], (key & 0xFFFF0000u) | (65535u - w));
+        for (int r = 0; r < 3; r++) {
+            key[r] = pd.b.keys[(size_t)rows[r] * W + x];
+            sel[r] = pd.b.selected[(size_t)rows[r] * W + x];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            checked[r * W + x] = sel[r];
+            if (key[r] != INVALID_KEY) {
+                const uint32_t w = key[r] & 0xFFFFu;
+                atomicMin(&right[r * W + x - (int)w], (key[r] & 0xFFFF0000u) | (65535u - w));
             }
         }
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        const float *sel = pd.b.selected + (size_t)rows[r] * W;
         for (int x = threadIdx.x; x < W; x += blockDim.x) {
-            const float c = checked_value(sel, right + r * W, x, W);
+            const float c = checked_value(checked + r * W, right + r * W, x, W);
             checked[r * W + x] = c;
             if (r == 1 && keep_checked) pd.b.checked[(size_t)y * W + x] = c;
         }
