@@ -52,7 +52,8 @@ LANES = {64: (16, 32), 128: (16, 32, 64), 192: (16,), 256: (16, 32, 64)}
 WAVES = {64: (2, 4), 128: (2, 4, 8), 192: (2,), 256: (2, 4, 8)}  # more than one wave per line
 
 
-@pytest.mark.parametrize("disparities,paths", [(64, 8), (128, 8), (128, 3), (192, 8), (256, 8)])
+@pytest.mark.parametrize("disparities,paths", [(64, 8), (128, 8), (64, 3), (128, 3), (192, 8),
+                                               (256, 8), (256, 3)])
 def test_every_schedule_gives_the_same_disparity(disparities, paths):
     """Tuning picks by time only: each candidate schedule is checked bit for bit here, on
     the path sums themselves as well as the output."""

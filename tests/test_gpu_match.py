@@ -55,10 +55,11 @@ def test_other_penalties_and_uniqueness(p1, p2, uniqueness):
         check_all(left, right, 64, paths, p1, p2, uniqueness)
 
 
+@pytest.mark.parametrize("paths", [3, 8])
 @pytest.mark.parametrize("disparities", [192, 256])
-def test_wide_disparity_ranges(disparities):
+def test_wide_disparity_ranges(disparities, paths):
     left, right, _ = scenes.planar_pair(300, 40, seed=5, low=4, high=disparities - 10)
-    check_all(left, right, disparities, 8)
+    check_all(left, right, disparities, paths)
 
 
 @pytest.mark.parametrize("size", [(9, 7), (10, 8), (33, 17), (17, 40)])
