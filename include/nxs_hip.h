@@ -186,10 +186,11 @@ typedef struct nxs_schedule {
     int split;    /* 1: one launch per line orientation */
     int lanes;    /* lanes sharing one line: 16, 32 or 64, where disparities / lanes is
                      2, 4, 8, 12 or 16 */
-    int waves;    /* waves per line (16 lanes): 2, a wave per direction of a row (and at
-                     3 paths four down-only columns a wave); 4 or 8 (8 paths), each
-                     direction's disparities over 2 or 4 waves as well, where
-                     disparities / (8 x waves) is 2, 4 or 8 */
+    int waves;    /* waves per line (16 lanes): 2, a wave per direction of a line (at
+                     3 paths, where columns go down only, four columns a wave); 4 or 8
+                     (8 paths only), each direction's disparities over 2 or 4 waves as
+                     well, where disparities / (8 x waves) is 2, 4 or 8. Without a pick:
+                     2 waves, prefetch 4 (8 at 256 disparities, and at 3 paths) */
 } nxs_schedule;
 
 int nxs_schedule_set(int width, int height, int disparities, int paths, const nxs_schedule *s);

@@ -54,9 +54,9 @@ struct Schedule {
     int prefetch = 4;  // 2, 4 or 8
     bool split = false;
     int lanes = 16;
-    int waves = 1;  // 16 lanes: 2 = a row's two directions in two waves of one block (and
-                    // at 3 paths four down-only columns a wave); 8 paths: 4 or 8 = each
-                    // direction's D split over 2 or 4 waves as well
+    int waves = 1;  // 16 lanes: 2 = a line's two directions in two waves of one block (at
+                    // 3 paths, whose columns go down only, four columns a wave); 8 paths:
+                    // 4 or 8 = each direction's D split over 2 or 4 waves as well
 };
 
 // Slices per direction (2 or 4) that fit D disparities, or 1 where they do not.
