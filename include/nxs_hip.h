@@ -190,7 +190,7 @@ typedef struct nxs_schedule {
                      3 paths, where columns go down only, four columns a wave); 4 or 8
                      (8 paths only), each direction's disparities over 2 or 4 waves as
                      well, where disparities / (8 x waves) is 2, 4 or 8. Without a pick:
-                     2 waves, prefetch 4 (8 at 256 disparities, and at 3 paths) */
+                     2 waves, prefetch 8 (4 at 192 disparities) */
 } nxs_schedule;
 
 int nxs_schedule_set(int width, int height, int disparities, int paths, const nxs_schedule *s);

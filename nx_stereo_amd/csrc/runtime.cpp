@@ -66,16 +66,17 @@ void profile_end(int slot, hipStream_t stream) {
 static std::mutex g_schedules_mu;
 static std::map<std::tuple<int, int, int, int>, Schedule> g_schedules;
 
-// Without a pick: two waves per line (one per direction of a row), which halves a line's
-// sequential walk. With 8 paths, loads 4 steps ahead below 256 disparities and 8 at 256,
-// where the aggregation is bound by HBM traffic (the fastest measured on MI355X at
-// 728x544x128, 1280x720x192 and 1920x1080x256).
+// Without a pick: two waves per line (one per direction), which halves a line's sequential
+// walk, loading 8 steps ahead; at 192 disparities (12 bytes a lane) 4, where eight steps'
+// registers cost occupancy. Measured on MI355X at 8 paths: 728x544x128 170 us (prefetch 4:
+// 179), 1280x720x192 637 us (prefetch 8: 666), 1920x1080x256 fastest with 8, 960x540x64
+// even.
 static Schedule default_schedule(int disparities, int paths) {
     Schedule s;
     // At 3 paths the rows' walk bounds the kernel (the down-only columns take four lines a
     // wave), and a prefetch of 8 covers it best: 38 us at 364x272x64 (coarse-to-fine's half
     // images) and 89 us at 728x544x128, against 71 and 156 us with one wave a line.
-    s.prefetch = paths == 8 && disparities < 256 ? 4 : 8;
+    s.prefetch = disparities == 192 ? 4 : 8;
     s.lanes = 16;
     s.waves = 2;
     return s;
