@@ -138,13 +138,21 @@ __global__ __launch_bounds__(BAND_COST_BLOCK) void band_cost_kernel(
     const int pitch = W + 2 * BAND_REACH;
     uint8_t *rows = reinterpret_cast<uint8_t *>(cr + W);      // [2][7][pitch]
     const int y = blockIdx.x;
-    for (int side = 0; side < 2; side++) {
-        const uint8_t *img = side ? right : left;
-        for (int r = 0; r < 7; r++) {
-            const uint8_t *src = img + (size_t)min(max(y + r - 3, 0), H - 1) * W;
-            uint8_t *dst = rows + (side * 7 + r) * pitch;
-            for (int i = threadIdx.x; i < pitch; i += BAND_COST_BLOCK)
-                dst[i] = src[min(max(i - BAND_REACH, 0), W - 1)];
+    // A thread loads its byte of all 14 rows before it stores any, so that the loads are in
+    // flight together (a thread past the padded row loads a byte it does not store); the
+    // rows' offsets are scalars, the column the one vector offset.
+    const Rsrc images[2] = {buffer_of(left, (uint32_t)W * H), buffer_of(right, (uint32_t)W * H)};
+    for (int i0 = 0; i0 < pitch; i0 += BAND_COST_BLOCK) {
+        const int i = i0 + (int)threadIdx.x;
+        const int column = min(max(i - BAND_REACH, 0), W - 1);
+        uint32_t v[14];
+#pragma unroll
+        for (int k = 0; k < 14; k++)
+            v[k] = __builtin_amdgcn_raw_buffer_load_b8(images[k / 7], column,
+                                                       min(max(y + k % 7 - 3, 0), H - 1) * W, 0);
+        if (i < pitch) {
+#pragma unroll
+            for (int k = 0; k < 14; k++) rows[k * pitch + i] = (uint8_t)v[k];
         }
     }
     __syncthreads();
