@@ -351,7 +351,7 @@ static void check_matching(int W, int H, int D, int paths, int p1, int p2, float
     if ((uint64_t)W * H * D >= (1ull << 31))
         throw Error{"the cost volume (width x height x disparities) must stay under 2^31 bytes, got " +
                     std::to_string(W) + "x" + std::to_string(H) + "x" + std::to_string(D)};
-    if (!(p1 < p2 && p2 + 48 <= 127))
+    if (!(0 <= p1 && p1 < p2 && p2 + 48 <= 127))
         throw Error{"expected 0 <= p1 < p2 <= 79, got " + std::to_string(p1) + " and " +
                     std::to_string(p2)};
     if (std::isnan(uniqueness) || uniqueness < 1.0f) {
@@ -372,7 +372,6 @@ static void check_band(int W, int H, int D, int band, int paths, int census, int
     if (census < 0 || census > 2)
         throw Error{"census must be 0 (sparse_9x7), 1 (dense_9x5) or 2 (dense_9x5_blend), got " +
                     std::to_string(census)};
-    if (p1 < 0) throw Error{"p1 must be at least 0"};
 }
 
 // sgm_coarse_to_fine/3's (nx_stereo.ex:164-185): even sizes whose halves match at 64.
@@ -556,7 +555,16 @@ struct Segment {
     RunSlot slots[RUN_SLOTS];
     int next_slot = 0;
     size_t workspace_bytes = 0;  // of its largest matcher member, 0 without one
-    Workspace *device_workspace = nullptr;  // the pool's, held for nxs_run_device
+    // nxs_run_device's matcher workspace: the segment's own, outside the serving pool and
+    // its budget, held until the segment is released (a run on a caller's stream may still
+    // use it when nxs_run_device returns).
+    void *device_workspace = nullptr;
+    // Runs of one segment keep submission order across streams: the stream the last run was
+    // enqueued on, and `order`, recorded after each run on a stream not the segment's own
+    // (and on the segment's own stream before a run on another follows it).
+    hipStream_t last_stream = nullptr;
+    hipEvent_t order = nullptr;
+    bool foreign = false;  // a run has been enqueued on a caller's stream
 
     // device buffers carved from the arena
     uint8_t *frames[2]{}, *images[2]{};
@@ -572,6 +580,9 @@ struct Segment {
 
     ~Segment() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (foreign && order) (void)hipEventSynchronize(order);  // the last caller-stream run
+        if (order) (void)hipEventDestroy(order);
+        if (device_workspace) (void)hipFree(device_workspace);
         for (auto &slot : slots) {
             for (auto e : slot.events) (void)hipEventDestroy(e);
             if (slot.upload) (void)hipHostFree(slot.upload);
@@ -693,6 +704,7 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
                    up256(sharp) + up256(32) + up256(align) + up256(4 * align_sums) +
                    up256(4 * P) * carries + up256(taps_bytes) + 256;
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&s->order, hipEventDisableTiming | hipEventDisableSystemFence));
     HIP_TRY(hipMalloc(&s->arena, bytes));
     uint8_t *p = static_cast<uint8_t *>(s->arena);
     auto carve = [&](size_t n) {
@@ -824,6 +836,24 @@ struct Enqueued {
     std::map<std::string, Slot> tensors;
     std::vector<const char *> filled;
 };
+
+// Runs of one segment share its arena, carries and workspace, so each waits for the one
+// before it, whichever stream that went on: before a run on `stream`, the stream waits for
+// the segment's last run when that ran elsewhere (the segment's own stream marks its work
+// then, a caller's stream was marked after its run). Called with the segment's lock held.
+static void order_before(Segment &s, hipStream_t stream) {
+    if (!s.last_stream || s.last_stream == stream) return;
+    if (s.last_stream == s.stream) HIP_TRY(hipEventRecord(s.order, s.stream));
+    HIP_TRY(hipStreamWaitEvent(stream, s.order, 0));
+}
+
+static void order_after(Segment &s, hipStream_t stream) {
+    if (stream != s.stream) {
+        HIP_TRY(hipEventRecord(s.order, stream));
+        s.foreign = true;
+    }
+    s.last_stream = stream;
+}
 
 // Enqueues every member on `stream`: entries go to the segment's buffers (device entries
 // a stage only reads are read in place), then each stage in order (segment.rs:206-239).
@@ -1198,7 +1228,11 @@ struct nxs_ctx {
                     }
                 if (allocated + bytes <= budget) continue;
             }
-            cv_free.wait(lock);  // admission: a run completes and frees memory
+            // Admission: a run in flight completes and frees memory. Every busy workspace
+            // belongs to such a run (device runs keep workspaces of their own), so the wait
+            // ends, unless the context closes first.
+            if (stopping) throw Error{"the context is closed"};
+            cv_free.wait(lock);
         }
     }
 
@@ -1225,44 +1259,71 @@ struct nxs_ctx {
                     if (!seg->slots[s].busy) slot_index = s;
                 }
                 if (slot_index >= 0) break;
+                if (stopping) {
+                    lock.unlock();
+                    post(ticket, done, user, refused_record("the context is closed"));
+                    return;
+                }
                 cv_free.wait(lock);
             }
             seg->slots[slot_index].busy = true;
             seg->next_slot = (slot_index + 1) % RUN_SLOTS;
+            std::string refused;
             try {
                 if (seg->workspace_bytes) ws = acquire_workspace(seg->workspace_bytes, lock);
             } catch (const Error &e) {
+                refused = e.reason;
+            } catch (const std::bad_alloc &) {
+                refused = "out of host memory";
+            }
+            if (!refused.empty()) {
                 seg->slots[slot_index].busy = false;
+                cv_free.notify_all();
                 lock.unlock();
-                post(ticket, done, user, refused_record(e.reason));
+                post(ticket, done, user, refused_record(refused));
                 return;
             }
         }
         RunSlot &slot = seg->slots[slot_index];
+        std::string refused;
         try {
             std::lock_guard<std::mutex> guard(seg->mu);
-            Enqueued e = enqueue(*seg, q.inputs, q.entries, {}, seg->stream, &slot, slot.events,
-                                 ws ? ws->ptr : nullptr);
-            std::vector<Output> outputs = enqueue_downloads(e, slot, seg->stream);
-            HIP_TRY(hipEventRecord(slot.events.back(), seg->stream));
-            std::lock_guard<std::mutex> lock(mu);
-            pending.push_back(Pending{ticket, seg, slot_index, ws, std::move(outputs), done, user});
-            cv_pending.notify_all();
-        } catch (const Error &e) {
-            // Work already enqueued may still use the slot and workspace: let it finish.
-            (void)hipStreamSynchronize(seg->stream);
-            {
+            try {
+                order_before(*seg, seg->stream);
+                Enqueued e = enqueue(*seg, q.inputs, q.entries, {}, seg->stream, &slot,
+                                     slot.events, ws ? ws->ptr : nullptr);
+                std::vector<Output> outputs = enqueue_downloads(e, slot, seg->stream);
+                HIP_TRY(hipEventRecord(slot.events.back(), seg->stream));
+                order_after(*seg, seg->stream);
                 std::lock_guard<std::mutex> lock(mu);
-                slot.busy = false;
-                if (ws) ws->busy = false;
-                cv_free.notify_all();
+                pending.push_back(Pending{ticket, seg, slot_index, ws, std::move(outputs), done, user});
+                cv_pending.notify_all();
+                return;
+            } catch (...) {
+                seg->last_stream = seg->stream;  // whatever was enqueued went there
+                throw;
             }
-            post(ticket, done, user, refused_record(e.reason));
+        } catch (const Error &e) {
+            refused = e.reason;
+        } catch (const std::bad_alloc &) {
+            refused = "out of host memory";
+        } catch (const std::exception &e) {
+            refused = std::string("internal error: ") + e.what();
         }
+        // Work already enqueued may still use the slot and workspace: let it finish.
+        (void)hipStreamSynchronize(seg->stream);
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            slot.busy = false;
+            if (ws) ws->busy = false;
+            cv_free.notify_all();
+        }
+        post(ticket, done, user, refused_record(refused));
     }
 
-    // Forgets a segment. Runs in flight keep it alive until they are answered; the
-    // workspace its device runs held goes back to the pool once its stream is idle.
+    // Forgets a segment. Runs in flight keep it alive until they are answered; its own
+    // device memory (arena, device-run workspace) goes when the last of them is, after its
+    // streams' work (the segment's destructor).
     void drop_segment(uint32_t id) {
         std::shared_ptr<nxs::Segment> old;
         {
@@ -1271,13 +1332,6 @@ struct nxs_ctx {
             if (it == segments.end()) return;
             old = it->second;
             segments.erase(it);
-        }
-        if (old->device_workspace) {
-            (void)hipStreamSynchronize(old->stream);
-            std::lock_guard<std::mutex> lock(mu);
-            old->device_workspace->busy = false;
-            old->device_workspace = nullptr;
-            cv_free.notify_all();
         }
     }
 
@@ -1369,6 +1423,10 @@ struct nxs_ctx {
                     reply = outputs_record(slot, p.segment->members.size(), p.outputs);
                 } catch (const Error &e) {
                     reply = refused_record(e.reason);
+                } catch (const std::bad_alloc &) {
+                    reply = refused_record("out of host memory");
+                } catch (const std::exception &e) {
+                    reply = refused_record(std::string("internal error: ") + e.what());
                 }
                 {
                     std::lock_guard<std::mutex> lock(mu);
@@ -1448,6 +1506,7 @@ void nxs_close(nxs_ctx *ctx) {
         pending.swap(ctx->jobs);
     }
     ctx->cv_jobs.notify_all();
+    ctx->cv_free.notify_all();  // a run waiting for admission is refused
     // Requests not started fail; runs already on the device finish and are answered.
     for (auto &job : pending)
         ctx->post(job.ticket, job.done, job.user, nxs::refused_record("the context is closed"));
@@ -1523,7 +1582,6 @@ int nxs_match_device(const uint8_t *left, const uint8_t *right, int width, int h
     } catch (const nxs::Error &e) {
         return nxs::fail(NXS_EINVAL, e.reason);
     }
-    if (p1 < 0) return nxs::fail(NXS_EINVAL, "p1 must be at least 0");
     if (!left || !right || !disparity || !workspace)
         return nxs::fail(NXS_EINVAL, "a device pointer is null");
     if (workspace_bytes < nxs::match_bytes(width, height, disparities, paths))
@@ -1545,7 +1603,6 @@ int nxs_match_host(const uint8_t *left, const uint8_t *right, int width, int hei
     } catch (const nxs::Error &e) {
         return nxs::fail(NXS_EINVAL, e.reason);
     }
-    if (p1 < 0) return nxs::fail(NXS_EINVAL, "p1 must be at least 0");
     if (!left || !right) return nxs::fail(NXS_EINVAL, "no input image");
     const size_t P = (size_t)width * height, N = P * disparities;
     const size_t ws = nxs::match_bytes(width, height, disparities, paths);
@@ -1742,20 +1799,26 @@ int nxs_run_device(nxs_ctx *ctx, const uint8_t *record, size_t len, const nxs_te
         Request q = parse(record, len);
         if (q.kind != RUN) return fail(NXS_EINVAL, "nxs_run_device takes a run record");
         std::shared_ptr<Segment> held;
-        void *workspace = nullptr;
         {
-            std::unique_lock<std::mutex> lock(ctx->mu);
+            std::lock_guard<std::mutex> lock(ctx->mu);
             auto it = ctx->segments.find(q.segment);
             if (it == ctx->segments.end())
                 return fail(NXS_EINVAL, "segment " + std::to_string(q.segment) + " is not configured");
-            held = it->second;
-            // Device runs of a segment keep one workspace of the pool for themselves.
-            if (held->workspace_bytes && !held->device_workspace)
-                held->device_workspace = ctx->acquire_workspace(held->workspace_bytes, lock);
-            if (held->device_workspace) workspace = held->device_workspace->ptr;
+            held = it->second;  // alive (with its workspace) until this call returns
         }
         Segment &seg = *held;
         std::lock_guard<std::mutex> guard(seg.mu);
+        // Device runs of a segment keep a workspace of their own, outside the serving pool.
+        if (seg.workspace_bytes && !seg.device_workspace) {
+            hipError_t e = hipMalloc(&seg.device_workspace, seg.workspace_bytes);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                seg.device_workspace = nullptr;
+                return fail(NXS_ENOMEM, std::string("out of device memory for the matcher: ") +
+                                            hipGetErrorString(e));
+            }
+        }
+        void *workspace = seg.device_workspace;
         std::vector<DeviceEntry> device;
         for (int e = 0; e < n_entries; e++) {
             const nxs_tensor &t = entries[e];
@@ -1763,7 +1826,15 @@ int nxs_run_device(nxs_ctx *ctx, const uint8_t *record, size_t len, const nxs_te
             device.push_back(DeviceEntry{t.name, t.type, std::vector<uint32_t>(t.shape, t.shape + t.rank), t.data});
         }
         hipStream_t s = stream ? (hipStream_t)stream : seg.stream;
-        Enqueued out = enqueue(seg, q.inputs, q.entries, device, s, nullptr, {}, workspace);
+        order_before(seg, s);
+        Enqueued out;
+        try {
+            out = enqueue(seg, q.inputs, q.entries, device, s, nullptr, {}, workspace);
+        } catch (...) {
+            order_after(seg, s);  // what was enqueued before the error still orders
+            throw;
+        }
+        order_after(seg, s);
         *n_outputs = (int)out.filled.size();
         for (int k = 0; k < (int)out.filled.size() && k < max_outputs; k++) {
             const Slot &slot = out.tensors[out.filled[k]];
@@ -1777,6 +1848,10 @@ int nxs_run_device(nxs_ctx *ctx, const uint8_t *record, size_t len, const nxs_te
         return NXS_OK;
     } catch (const Error &e) {
         return fail(NXS_EINVAL, e.reason);
+    } catch (const std::bad_alloc &) {
+        return fail(NXS_ENOMEM, "out of host memory");
+    } catch (const std::exception &e) {
+        return fail(NXS_EUNKNOWN, std::string("internal error: ") + e.what());
     }
 }
 
@@ -1919,7 +1994,6 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
             const nxs_pair &q = pairs[i];
             if (!q.left || !q.right || !q.disparity) throw Error{"a pair has a null pointer"};
             check_matching(q.width, q.height, q.disparities, q.paths, q.p1, q.p2, q.uniqueness);
-            if (q.p1 < 0) throw Error{"p1 must be at least 0"};
             int c = 0;
             for (; c < (int)batch->classes.size(); c++) {
                 const MatchParams &p = batch->classes[c].p;

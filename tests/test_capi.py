@@ -43,6 +43,7 @@ def test_workspace_size_grows_with_the_volume():
     ((64, 48, 96, 8, 8, 64, 1.0), "disparities must be"),
     ((64, 48, 64, 4, 8, 64, 1.0), "paths must be 3 or 8"),
     ((64, 48, 64, 8, 64, 64, 1.0), "p1 < p2"),
+    ((64, 48, 64, 8, -1, 64, 1.0), "0 <= p1"),
     ((64, 48, 64, 8, 8, 80, 1.0), "p1 < p2"),
     ((64, 48, 64, 8, 8, 64, 0.9), "uniqueness"),
     ((64, 48, 64, 8, 8, 64, float("nan")), "uniqueness"),

@@ -357,7 +357,9 @@ def test_quality_of_a_tall_image(context):
 
 
 def test_full_pipeline_728x544x128(context):
-    """The benchmark's workload, every output against the oracle's chain."""
+    """The benchmark's workload, every output against the oracle's chain: the rectified
+    images, the disparity, depth_mm, the cloud grid and its kept count, and the quality sums
+    of two pairs (the second against the first's carried map)."""
     geometry, cal = rig_geometry(728, 544)
     left_frame, right_frame, _ = scenes.raw_frames(cal, 7)
     members = [("rectification", stages.Rectification()),
@@ -374,9 +376,30 @@ def test_full_pipeline_728x544x128(context):
         c = cams[side]
         rect[side] = oracle.rectify(frame, c["k"], c["d"], c["inverse_pr"], (544, 728))
         same_bits(out[side], rect[side])
-    d = oracle.sgm(rect["left"], rect["right"], 128, 8)
-    same_bits(out["disparity"], oracle.despeckle(d, 12, 1.0))
+    d = oracle.despeckle(oracle.sgm(rect["left"], rect["right"], 128, 8), 12, 1.0)
+    same_bits(out["disparity"], d)
     assert (out["disparity"] > 0).mean() > 0.3, "the scene is mostly matched"
+    f, b = geometry["focal_length_px"], geometry["baseline_m"]
+    mm, m = oracle.depth(d, f, b)
+    same_bits(out["depth_mm"], mm)
+    cx, cy = geometry["principal_point"]
+    points, kept = oracle.cloud(m, 4, f, cx, cy)
+    assert out["cloud_grid"].shape == (136, 182, 3)
+    same_bits(out["cloud_grid"], points)
+    assert int(out["cloud_points"]) == kept and kept > 0
+    # Quality against the previous pair's map: a second pair (the frames one pixel apart)
+    # gives the both-matched and jitter terms something to count.
+    want_first = oracle.Quality((544, 728))
+    np.testing.assert_allclose(out["quality_sums"], want_first.sums(d), rtol=1e-5)
+    moved = {"left_frame": np.roll(left_frame, 1, 1), "right_frame": np.roll(right_frame, 1, 1)}
+    out2 = run_ok(seg, moved)
+    rect2 = {side: oracle.rectify(moved[f"{side}_frame"], cams[side]["k"], cams[side]["d"],
+                                  cams[side]["inverse_pr"], (544, 728)) for side in ("left", "right")}
+    d2 = oracle.despeckle(oracle.sgm(rect2["left"], rect2["right"], 128, 8), 12, 1.0)
+    same_bits(out2["disparity"], d2)
+    sums = want_first.sums(d2)
+    assert sums[3] > 0, "both maps match somewhere"
+    np.testing.assert_allclose(out2["quality_sums"], sums, rtol=1e-5)
 
 
 def test_refusals_map_like_the_port(context):

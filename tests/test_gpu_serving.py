@@ -96,3 +96,70 @@ def test_poisson_arrivals_complete_in_order_per_stream(frames):
     assert (served.latency_ms > 0).all()
     summary = served.summary()
     assert summary["pairs"] == FRAMES * len(SPECS)
+
+
+def test_a_segment_with_device_runs_does_not_block_admission():
+    """With a 1-byte budget, a segment that has run through nxs_run_device (and keeps its
+    workspace) must not hold back the admission of another segment's submitted run, and the
+    context must close afterwards."""
+    from nx_stereo_amd import hip
+
+    specs = [serving.StreamSpec(64, 48, 64, 30.0, 1), serving.StreamSpec(97, 31, 64, 30.0, 2)]
+    (left_a, right_a), = serving.stream_frames(specs[0], 1)
+    (left_b, right_b), = serving.stream_frames(specs[1], 1)
+    ctx = segment.Context(0)
+    try:
+        assert ctx.lib.nxs_set_hbm_budget(ctx.handle, 1) == _native.NXS_OK
+        a, b = serving.configure_streams(ctx, specs)
+        dl, dr = hip.Buffer.of(left_a), hip.Buffer.of(right_a)
+        ctx.run_device(a.run_record({}), [("left", 0, (48, 64), dl.ptr),
+                                          ("right", 0, (48, 64), dr.ptr)])
+        hip.synchronize()
+        ticket = ctx.submit(b.run_record({"left": left_b, "right": right_b}))
+        assert ctx.wait(ticket, 60000), "the submitted run was never admitted"
+        got, _ = decoded(ctx.reply(ticket))
+        want = oracle.Temporal((31, 97)).filter(oracle.sgm(left_b, right_b, 64, 8), 0.4, 2.0, 2)
+        assert same(got, want)
+    finally:
+        ctx.close()
+
+
+def test_runs_on_a_caller_stream_and_submitted_runs_keep_their_order():
+    """A temporal filter segment fed alternately through nxs_run_device on a caller's stream
+    and through nxs_submit: each run sees the carry the run before it left, as when every
+    run goes one way."""
+    from nx_stereo_amd import hip, stages
+
+    h, w = 16, 48
+    ctx = segment.Context(0)
+    stream = hip.Stream()
+    try:
+        geometry = {"width": w, "height": h, "frame_width": w, "frame_height": h}
+        ok, seg = segment.HipSegment.build(
+            [("temporal_filter", stages.TemporalFilter(alpha=0.5, delta=8.0, hold=0))],
+            geometry, ctx)
+        assert ok == "ok"
+        assert ctx.request(seg.configure_record()) is True
+        seg.configured_version = seg.version
+        values = [8.0, 12.0, 6.0, 11.0, 14.0, 9.0]
+        maps = [hip.Buffer.of(np.full((h, w), v, np.float32)) for v in values]
+        got = []
+        for k, v in enumerate(values):
+            if k % 2 == 0:
+                outs = ctx.run_device(seg.run_record({}), [("disparity", 2, (h, w), maps[k].ptr)],
+                                      stream=stream.handle)
+                stream.synchronize()
+                got.append(float(hip.copy_to_host(outs[0][3], (h, w), np.float32)[0, 0]))
+            else:
+                ticket = ctx.submit(seg.run_record({"disparity": np.full((h, w), v, np.float32)}))
+                assert ctx.wait(ticket, 60000)
+                reply = protocol.decode(ctx.reply(ticket))
+                got.append(float(reply.tensors["disparity"][0, 0]))
+        want, previous = [], None
+        for v in values:
+            previous = v if previous is None else 0.5 * v + 0.5 * previous
+            want.append(previous)
+        assert got == want
+    finally:
+        stream.destroy()
+        ctx.close()

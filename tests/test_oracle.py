@@ -4,6 +4,8 @@ recurrence computed by brute force, stereo_match's byte-state recurrence
 (paths.rs), sorting, and hand-worked closed forms. The reference ships no
 golden vector files; its tests generate every input, and these restate them.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -320,7 +322,7 @@ def test_a_match_at_disparity_0_reads_as_unmatched():
 
 
 @pytest.mark.parametrize("size", [(64, 48), (97, 31)])
-@pytest.mark.parametrize("disparities", [64, 128])
+@pytest.mark.parametrize("disparities", [64, 128, 192, 256])
 @pytest.mark.parametrize("paths", [3, 8])
 def test_cpu_baseline_matches_the_oracle(size, disparities, paths):
     w, h = size
@@ -329,6 +331,16 @@ def test_cpu_baseline_matches_the_oracle(size, disparities, paths):
     for threads in (1, 3, 4):
         got = oracle.cpu_sgm(left, right, disparities, paths, threads=threads).view(np.uint32)
         assert (got == want).all()
+
+
+def test_cpu_baseline_matches_the_oracle_on_a_planar_pair_at_256():
+    """The checker of the large-pair GPU test (1920x1080x256 is beyond the oracle's full
+    i32 volumes): a planar pair with disparities up to 248, where every range of the 256
+    is in use."""
+    left, right, _ = scenes.planar_pair(300, 40, seed=9, low=4, high=248)
+    want = oracle.sgm(left, right, 256, 8).view(np.uint32)
+    for threads in (0, 1, 5):
+        assert (oracle.cpu_sgm(left, right, 256, 8, threads=threads).view(np.uint32) == want).all()
 
 
 def test_cpu_baseline_other_penalties_and_sizes():
@@ -610,3 +622,49 @@ def test_maps_sample_where_the_projection_says():
     assert rectify(frame, cam)[10, 20] == 17
     rows = np.tile(np.arange(48, dtype=np.uint8)[:, None], (1, 64))
     assert rectify(rows, cam)[10, 20] == 10
+
+
+def test_moving_rows_samples_that_many_rows_further_down():
+    """stage/rectification.ex:84-89 (right_row_offset): column 2 of inverse(P3 R) gains
+    rows x column 1, so output row v samples frame row v + rows. With an identity camera
+    that is a plain shift by whole rows (0 past the frame), and half a row averages two."""
+    from nx_stereo_amd import stages
+
+    image = np.random.default_rng(5).integers(0, 256, (48, 64), dtype=np.uint8)
+    cam = scenes.camera(K, [], IDENTITY, projection(100.0, 32.0, 24.0))
+    for rows in (3, -2, 7):
+        moved = dict(cam, inverse_pr=stages.move_rows(cam["inverse_pr"], rows))
+        out = rectify(image, moved)
+        lo, hi = max(0, -rows), min(48, 48 - rows)
+        assert (out[lo:hi] == image[lo + rows:hi + rows]).all(), rows
+        assert (out[:lo] == 0).all() and (out[hi:] == 0).all(), rows
+    stripes = np.repeat((np.arange(48) % 2 * 100).astype(np.uint8)[:, None], 64, 1)
+    moved = dict(cam, inverse_pr=stages.move_rows(cam["inverse_pr"], 0.5))
+    assert (rectify(stripes, moved)[:47] == 50).all()
+    # Through the stage's own params: the right camera moves, the left does not.
+    geometry = {"left_camera": cam, "right_camera": cam}
+    params = stages.Rectification(right_row_offset=3.0).params(geometry)
+    assert (rectify(image, params["right"])[:45] == image[3:]).all()
+    assert (rectify(image, params["left"]) == image).all()
+
+
+def test_camera_inverse_is_the_inverse_of_the_rotated_projection():
+    """rectify.ex:34-50: inverse_pr = inverse(P3 R) in f64, cast to f32. For the rig's
+    rotations about the vertical axis, inverse(K R) = R^T inverse(K) in closed form."""
+    cal = scenes.rig(728, 544)
+    for side, sign in (("left", 1.0), ("right", -1.0)):
+        c = cal[side]
+        cam = scenes.camera(c["k"], c["d"], c["r"], c["p"])
+        t = math.radians(sign * 0.25)
+        r = np.array([[math.cos(t), 0, math.sin(t)], [0, 1, 0], [-math.sin(t), 0, math.cos(t)]])
+        f, cx, cy = 873.0, 364.0, 272.0
+        k_inv = np.array([[1 / f, 0, -cx / f], [0, 1 / f, -cy / f], [0, 0, 1]])
+        want = (r.T @ k_inv).astype(np.float32).reshape(-1)
+        got = cam["inverse_pr"]
+        assert got.dtype == np.float32
+        # f64 cofactors against the closed form: the same f32 value, or one ulp from it.
+        ulps = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+        ulps[(got == 0) & (want == 0)] = 0  # +0 and -0
+        assert (ulps <= 1).all(), (side, got, want)
+        assert np.allclose(np.asarray(got, np.float64).reshape(3, 3)
+                           @ (np.array([[f, 0, cx], [0, f, cy], [0, 0, 1]]) @ r), np.eye(3), atol=1e-6)
