@@ -101,6 +101,20 @@ int nxs_match_device(const uint8_t *left, const uint8_t *right, int width, int h
                      int disparities, int paths, int p1, int p2, float uniqueness,
                      float *disparity, void *workspace, size_t workspace_bytes, void *stream);
 
+/* The matcher of nxs_match_device with no workspace argument and no allocation: for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). No counterpart in the reference. Its
+ * workspace is a scratch array of NXS_NOALLOC_SCRATCH_BYTES in the library's device
+ * memory, one per device, held from the library's load until the process ends; sizes
+ * whose nxs_match_workspace_size exceeds it are refused (NXS_EINVAL, with the reason).
+ * It calls no hipMalloc or hipFree. Every call on one device shares that device's array:
+ * calls that overlap on a device (on different streams, or from different threads
+ * without ordering) race and give wrong results; calls on one stream are ordered. */
+#define NXS_NOALLOC_SCRATCH_BYTES (64u << 20)
+int nxs_match_device_noalloc(const uint8_t *left, const uint8_t *right, int width, int height,
+                             int disparities, int paths, int p1, int p2, float uniqueness,
+                             float *disparity, void *stream);
+
 /* The same on host buffers, returning the intermediate products for
  * inspection (any output pointer may be NULL): census features u64 {H,W}
  * of each image, matching costs u8 {H,W,D}, aggregated path costs u16

@@ -19,7 +19,8 @@ NXS_OK, NXS_TIMEOUT, NXS_EINVAL, NXS_EDEVICE, NXS_ENOMEM, NXS_EUNKNOWN = 0, 1, -
 # Every symbol include/nxs_hip.h declares.
 SYMBOLS = (
     "nxs_open", "nxs_close", "nxs_submit", "nxs_wait", "nxs_reply", "nxs_release_reply",
-    "nxs_poll", "nxs_set_hbm_budget", "nxs_stats", "nxs_match_workspace_size", "nxs_match_device", "nxs_match_host",
+    "nxs_poll", "nxs_set_hbm_budget", "nxs_stats", "nxs_match_workspace_size", "nxs_match_device",
+    "nxs_match_device_noalloc", "nxs_match_host",
     "nxs_band_workspace_size", "nxs_match_band_device", "nxs_match_band_host",
     "nxs_coarse_to_fine_workspace_size", "nxs_coarse_to_fine_device", "nxs_coarse_to_fine_host",
     "nxs_run_device", "nxs_profile_start", "nxs_profile_stop", "nxs_schedule_set",
@@ -90,6 +91,8 @@ def load() -> ctypes.CDLL:
     lib.nxs_match_workspace_size.restype = sz
     lib.nxs_match_device.argtypes = [P, P, i, i, i, i, i, i, f, P, P, sz, P]
     lib.nxs_match_device.restype = i
+    lib.nxs_match_device_noalloc.argtypes = [P, P, i, i, i, i, i, i, f, P, P]
+    lib.nxs_match_device_noalloc.restype = i
     lib.nxs_match_host.argtypes = [P, P, i, i, i, i, i, i, f] + [P] * 8
     lib.nxs_match_host.restype = i
     lib.nxs_band_workspace_size.argtypes = [i, i, i, i]
@@ -180,6 +183,22 @@ def match_device(left_ptr: int, right_ptr: int, width: int, height: int, dispari
         paths, p1, p2, float(np.float32(uniqueness)), ctypes.c_void_p(out_ptr),
         ctypes.c_void_p(workspace_ptr), workspace_bytes, ctypes.c_void_p(stream or None))
     _check(rc, "nxs_match_device")
+
+
+# nxs_match_device_noalloc's scratch (NXS_NOALLOC_SCRATCH_BYTES in include/nxs_hip.h).
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
+
+def match_device_noalloc(left_ptr: int, right_ptr: int, width: int, height: int,
+                         disparities: int, paths: int, p1: int, p2: int, uniqueness: float,
+                         out_ptr: int, stream: int = 0) -> None:
+    """Stage.Disparity on device buffers with the library's scratch as workspace (nothing
+    allocated), enqueued on `stream`."""
+    rc = load().nxs_match_device_noalloc(
+        ctypes.c_void_p(left_ptr), ctypes.c_void_p(right_ptr), width, height, disparities,
+        paths, p1, p2, float(np.float32(uniqueness)), ctypes.c_void_p(out_ptr),
+        ctypes.c_void_p(stream or None))
+    _check(rc, "nxs_match_device_noalloc")
 
 
 # sgm_band/4's census windows (nx_stereo.ex:28), by the index the C ABI takes.

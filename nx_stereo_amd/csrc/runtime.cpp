@@ -1594,6 +1594,77 @@ int nxs_match_device(const uint8_t *left, const uint8_t *right, int width, int h
     return NXS_OK;
 }
 
+}  // extern "C"
+
+// ---- The matcher without allocation ----
+// The workspace of nxs_match_device_noalloc: one array per device, in the code object's
+// device memory, which each device holds from the library's load until the process ends.
+// Host code never uses its name as an address: that is a host-side stand-in, and a kernel
+// given it would fault. Its device address, per device, comes from hipGetSymbolAddress with
+// that device current, on the entry point's first call there.
+static __device__ __align__(256) uint8_t scratch[NXS_NOALLOC_SCRATCH_BYTES];
+
+namespace nxs {
+
+static std::mutex g_scratch_mu;
+static std::map<int, void *> g_scratch;  // device -> the array's device address there
+
+// The array's address on the current device (taken once per device), or nullptr with the
+// reason in *why.
+static void *scratch_address(std::string *why) {
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) {
+        *why = std::string("hipGetDevice: ") + hipGetErrorString(e);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    if (it != g_scratch.end()) return it->second;
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        *why = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return nullptr;
+    }
+    g_scratch[device] = ptr;
+    return ptr;
+}
+
+}  // namespace nxs
+
+extern "C" {
+
+int nxs_match_device_noalloc(const uint8_t *left, const uint8_t *right, int width, int height,
+                             int disparities, int paths, int p1, int p2, float uniqueness,
+                             float *disparity, void *stream) {
+    try {
+        nxs::check_matching(width, height, disparities, paths, p1, p2, uniqueness);
+    } catch (const nxs::Error &e) {
+        return nxs::fail(NXS_EINVAL, e.reason);
+    }
+    if (!left || !right || !disparity) return nxs::fail(NXS_EINVAL, "a device pointer is null");
+    const size_t need = nxs::match_bytes(width, height, disparities, paths);
+    if (need > (size_t)NXS_NOALLOC_SCRATCH_BYTES)
+        return nxs::fail(NXS_EINVAL,
+                         "a " + std::to_string(width) + "x" + std::to_string(height) + "x" +
+                             std::to_string(disparities) + " match needs " + std::to_string(need) +
+                             " bytes of workspace, more than the " +
+                             std::to_string((size_t)NXS_NOALLOC_SCRATCH_BYTES) +
+                             " of nxs_match_device_noalloc's scratch: use nxs_match_device");
+    std::string why;
+    void *workspace = nxs::scratch_address(&why);
+    if (!workspace) return nxs::fail(NXS_EDEVICE, why);
+    // Every buffer carved from it is written before it is read, so what an earlier call left
+    // there needs no clearing.
+    nxs::MatchBuffers b;
+    nxs::match_carve(b, workspace, width, height, disparities, paths);
+    nxs::MatchParams mp{width, height, disparities, paths, p1, p2, uniqueness};
+    hipError_t e = nxs::launch_match(mp, left, right, b, disparity, nullptr, (hipStream_t)stream);
+    if (e != hipSuccess) return nxs::fail(NXS_EDEVICE, hipGetErrorString(e));
+    return NXS_OK;
+}
+
 int nxs_match_host(const uint8_t *left, const uint8_t *right, int width, int height,
                    int disparities, int paths, int p1, int p2, float uniqueness,
                    uint64_t *census_left, uint64_t *census_right, uint8_t *costs, uint16_t *sums,

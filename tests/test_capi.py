@@ -80,3 +80,18 @@ def test_coarse_to_fine_refuses_before_any_device_work(shape, kwargs, reason):
     left = np.zeros(shape, np.uint8)
     with pytest.raises(_native.NativeError, match=reason):
         _native.sgm_coarse_to_fine(left, left, **kwargs)
+
+
+def test_the_no_alloc_matcher_refuses_what_its_scratch_cannot_hold_before_any_device_work():
+    """nxs_match_device_noalloc checks the settings and the size against its scratch before
+    it touches a device (the pointers here are never read)."""
+    lib = _native.load()
+    fake = ctypes.c_void_p(0x1000)
+    rc = lib.nxs_match_device_noalloc(fake, fake, 728, 544, 128, 8, 8, 64, 1.0526316, fake, None)
+    assert rc == _native.NXS_EINVAL
+    assert "more than the 67108864" in _native.last_error()
+    rc = lib.nxs_match_device_noalloc(fake, fake, 64, 48, 64, 8, -1, 64, 1.0526316, fake, None)
+    assert rc == _native.NXS_EINVAL and "0 <= p1" in _native.last_error()
+    assert _native.NOALLOC_SCRATCH_BYTES == 64 << 20
+    text = open(os.path.join(ROOT, "include", "nxs_hip.h")).read()
+    assert "#define NXS_NOALLOC_SCRATCH_BYTES (64u << 20)" in text
