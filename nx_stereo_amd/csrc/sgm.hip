@@ -45,28 +45,33 @@ __device__ __forceinline__ PairDesc pair_of(const PairDesc &single, const PairDe
 // bottom), and the block makes row y's 48-bit census of both there: bit k is set when
 // neighbour + 1 < centre, row-major over dy in -3..3 and dx in -4..4 (0 skipped), columns
 // clamped. Then C[y,x,d] = popcount(FL[y,x] ^ FR[y,x-d]), with FR = 0 left of the image
-// (the right census row has D zero features in front). Costs leave `tile` pixels at a
-// time: thread t takes pixel t % tile and the 16 disparities of chunk t / tile, so a
-// wave's LDS reads are of consecutive features (no bank conflicts); the bytes go through
-// an LDS tile and leave as one contiguous run of tile*D bytes, 16 bytes per thread.
+// (the right census row has D zero features in front).
+//
+// The costs leave straight from registers. A thread takes the 16 disparities of chunk c of
+// four consecutive pixels x0..x0+3, chunks fastest, so the 16-byte stores of a wave's lanes
+// with the same pixels cover one contiguous run of the cost volume. The four pixels' 64
+// costs read 19 right features, FR[x0-16c-15 .. x0-16c+3], once each. The right census
+// row is kept in LDS with one spare feature after every 16, so that neighbouring lanes'
+// features, 16c apart, fall into different banks (17c slots apart: 34c banks, distinct
+// for c < 32).
+
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+typedef uint32_t v3u __attribute__((ext_vector_type(3)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 
 constexpr int MAX_D = 256;
-constexpr int COST_PITCH_PAD = 16;  // LDS bytes between pixels of the staged tile, spreading banks
-// The staged tile of a block of `block` threads: 16 cost bytes per thread, and a pad per
-// pixel (at most block / 4 pixels, D = 64).
-__host__ __device__ constexpr int cost_staged_bytes(int block) {
-    return block * 16 + block / 4 * COST_PITCH_PAD;
-}
 
 // The image rows in LDS carry CENSUS_REACH replicated pixels on each side, so the census
 // reads its 9 columns without clamping each one.
 constexpr int CENSUS_REACH = 4;
 
-// LDS bytes of one block: the staged tile, the two census rows (the right one with D zero
-// features in front), then 7 padded rows of each image.
-__host__ __device__ constexpr size_t census_cost_lds(int block, int W, int D) {
-    return (size_t)cost_staged_bytes(block) + 8 * (size_t)(2 * W + D) +
-           14 * (size_t)(W + 2 * CENSUS_REACH);
+// The LDS slot of right feature f (f = D + x - d, 0 <= f < D + W).
+__host__ __device__ constexpr int right_slot(int f) { return f + (f >> 4); }
+
+// LDS bytes of one block: the two census rows (the right one padded as right_slot says),
+// then 7 padded rows of each image.
+__host__ __device__ constexpr size_t census_cost_lds(int W, int D) {
+    return 8 * ((size_t)W + (size_t)right_slot(D + W) + 1) + 14 * (size_t)(W + 2 * CENSUS_REACH);
 }
 
 // Grid y: the row; z: the pair. keep_census: also store both census images. COST_BLOCK
@@ -76,15 +81,13 @@ template <int COST_BLOCK>
 __global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single,
                                                                  const PairDesc *table, int W,
                                                                  int H, int D, int keep_census) {
-    constexpr int COST_STAGED_BYTES = cost_staged_bytes(COST_BLOCK);
     extern __shared__ __align__(16) uint8_t lds[];
     const PairDesc pd = pair_of(single, table, blockIdx.z);
     const int y = blockIdx.y;
-    uint4 *staged = reinterpret_cast<uint4 *>(lds);
-    uint64_t *cl = reinterpret_cast<uint64_t *>(lds + COST_STAGED_BYTES);  // [W]
-    uint64_t *cr = cl + W;                                                  // [D zeros, W]
-    const int row_pitch = W + 2 * CENSUS_REACH;                             // padded row
-    uint8_t *rows = reinterpret_cast<uint8_t *>(cr + D + W);                // [2][7][row_pitch]
+    uint64_t *cl = reinterpret_cast<uint64_t *>(lds);  // [W]
+    uint64_t *cr = cl + W;                             // [right_slot(D + W) + 1]
+    const int row_pitch = W + 2 * CENSUS_REACH;        // padded row
+    uint8_t *rows = reinterpret_cast<uint8_t *>(cr + right_slot(D + W) + 1);  // [2][7][row_pitch]
     // A thread loads its byte of all 14 rows before it stores any, so that the loads are in
     // flight together (a thread past the padded row loads a byte it does not store). Buffer
     // loads: the rows' offsets are scalars, the column the one vector offset.
@@ -102,7 +105,7 @@ __global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single
             for (int k = 0; k < 14; k++) rows[k * row_pitch + i] = (uint8_t)v[k];
         }
     }
-    for (int i = threadIdx.x; i < D; i += COST_BLOCK) cr[i] = 0;
+    for (int f = threadIdx.x; f < D; f += COST_BLOCK) cr[right_slot(f)] = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < 2 * W; i += COST_BLOCK) {
         const int side = i >= W ? 1 : 0, x = i - side * W;
@@ -121,41 +124,39 @@ __global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single
                 bit++;
             }
         }
-        if (side) cr[D + x] = f;
+        if (side) cr[right_slot(D + x)] = f;
         else cl[x] = f;
         if (keep_census) (side ? pd.b.census_right : pd.b.census_left)[(size_t)y * W + x] = f;
     }
     __syncthreads();
-    uint8_t *__restrict__ costs = pd.b.costs;
     const int chunks = D >> 4;
-    const int tile = COST_BLOCK / chunks;
-    const int pitch = D + COST_PITCH_PAD;  // LDS bytes per pixel
-    const int p = threadIdx.x % tile, chunk = threadIdx.x / tile;
-    for (int x0 = 0; x0 < W; x0 += tile) {
-        const int pixels = min(tile, W - x0);
-        if (chunk < chunks && p < pixels) {
-            const uint64_t l = cl[x0 + p];
-            // FR[x - d] for x = x0 + p sits at cr[D + x - d].
-            const uint64_t *r = cr + D + x0 + p - chunk * 16;
+    const int groups = (W + 3) >> 2;
+    const Rsrc out = buffer_of(pd.b.costs + (size_t)y * W * D, (uint32_t)W * D);
+    for (int q = threadIdx.x; q < groups * chunks; q += COST_BLOCK) {
+        const int g = q / chunks, c = q - g * chunks;
+        const int x0 = 4 * g;
+        // Pixel x0 + k at disparity 16c + j reads right feature f0 + k - j, f0 = D + x0 - 16c.
+        const int f0 = D + x0 - 16 * c;
+        uint64_t r[19];  // r[i]: feature f0 - 15 + i (past the row's end: never used)
+#pragma unroll
+        for (int i = 0; i < 19; i++) r[i] = cr[right_slot(min(f0 - 15 + i, D + W - 1))];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint64_t l = cl[min(x0 + k, W - 1)];
             uint32_t words[4];
 #pragma unroll
             for (int w = 0; w < 4; w++) {
                 uint32_t word = 0;
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    word |= (uint32_t)__builtin_popcountll(l ^ r[-(w * 4 + j)]) << (8 * j);
+                    word |= (uint32_t)__builtin_popcountll(l ^ r[15 + k - 4 * w - j]) << (8 * j);
                 words[w] = word;
             }
-            staged[(p * pitch + chunk * 16) / 16] = make_uint4(words[0], words[1], words[2], words[3]);
+            // A pixel past the row's end stores nothing (its offset is past the buffer).
+            const uint32_t at = x0 + k < W ? (uint32_t)(x0 + k) * D + 16 * c : MASKED;
+            __builtin_amdgcn_raw_buffer_store_b128((v4u){words[0], words[1], words[2], words[3]},
+                                                   out, at, 0, 0);
         }
-        __syncthreads();
-        // The run of pixels*D bytes at (y*W + x0)*D, 16 bytes per thread.
-        uint4 *out = reinterpret_cast<uint4 *>(costs + ((size_t)y * W + x0) * D);
-        for (int q = threadIdx.x; q < pixels * chunks; q += COST_BLOCK) {
-            const int pixel = q / chunks, c = q - pixel * chunks;
-            out[q] = staged[(pixel * pitch + c * 16) / 16];
-        }
-        __syncthreads();
     }
 }
 
@@ -215,9 +216,6 @@ struct Bytes<16> {
     }
 };
 
-typedef uint32_t v2u __attribute__((ext_vector_type(2)));
-typedef uint32_t v3u __attribute__((ext_vector_type(3)));
-typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 
 // Orientation o's byte volume. A choice among the four pointers as scalar values: a
 // run-time index into the descriptor's array (or selects the compiler folds back into one)
@@ -1408,11 +1406,11 @@ static hipError_t launch_all(const MatchParams &p, const PairDesc &single, const
     static const hipError_t lds_ok = [] {
         hipError_t e = hipFuncSetAttribute(
             reinterpret_cast<const void *>(census_cost_kernel<512>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)census_cost_lds(512, MAX_WIDTH, MAX_D));
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)census_cost_lds(MAX_WIDTH, MAX_D));
         if (e == hipSuccess)
             e = hipFuncSetAttribute(reinterpret_cast<const void *>(census_cost_kernel<1024>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)census_cost_lds(1024, MAX_WIDTH, MAX_D));
+                                    (int)census_cost_lds(MAX_WIDTH, MAX_D));
         return e;
     }();
     if (lds_ok != hipSuccess) return lds_ok;
@@ -1431,11 +1429,11 @@ static hipError_t launch_all(const MatchParams &p, const PairDesc &single, const
     int slot = profile_begin(PK_COST, stream);
     if (one_sweep)
         hipLaunchKernelGGL(census_cost_kernel<512>, dim3(1, H, count), dim3(512),
-                           census_cost_lds(512, W, D), stream, single, table, W, H, D,
+                           census_cost_lds(W, D), stream, single, table, W, H, D,
                            p.keep_stages ? 1 : 0);
     else
         hipLaunchKernelGGL(census_cost_kernel<1024>, dim3(1, H, count), dim3(1024),
-                           census_cost_lds(1024, W, D), stream, single, table, W, H, D,
+                           census_cost_lds(W, D), stream, single, table, W, H, D,
                            p.keep_stages ? 1 : 0);
     profile_end(slot, stream);
     hipError_t e;
