@@ -1983,6 +1983,21 @@ struct BatchClass {
     float tuned_ms = 0;
 };
 
+// How the batch spreads over the device: each class's pairs go in launches of `chunk` pairs
+// (0: the whole class in one), and the launches are dealt to `branches` concurrent streams
+// (parallel branches of the captured graph), the largest first to the least loaded.
+struct BatchPlan {
+    int branches = 1;
+    int chunk = 0;
+};
+
+// One launch chain of the batch: pairs tables[first .. first+count) of class `cls`.
+struct BatchLaunch {
+    int cls, first, count, branch;
+};
+
+constexpr int MAX_BRANCHES = 4;
+
 }  // namespace nxs
 
 struct nxs_batch {
@@ -1990,6 +2005,12 @@ struct nxs_batch {
     void *workspace = nullptr;
     nxs::PairDesc *tables = nullptr;
     hipStream_t stream = nullptr;
+    // The graph's parallel branches, with the events that fork them from `stream` and join
+    // them back (all at the default priority).
+    hipStream_t branch[nxs::MAX_BRANCHES] = {};
+    hipEvent_t fork = nullptr, join[nxs::MAX_BRANCHES] = {};
+    nxs::BatchPlan plan;
+    std::vector<nxs::BatchLaunch> launches;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string description;
@@ -1998,15 +2019,94 @@ struct nxs_batch {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto b : branch)
+            if (b) (void)hipStreamSynchronize(b);
         if (tables) (void)hipFree(tables);
         if (workspace) (void)hipFree(workspace);
         if (stream) (void)hipStreamDestroy(stream);
+        for (auto b : branch)
+            if (b) (void)hipStreamDestroy(b);
+        if (fork) (void)hipEventDestroy(fork);
+        for (auto e : join)
+            if (e) (void)hipEventDestroy(e);
     }
 
-    void enqueue_all(hipStream_t s) {
+    // The launches of a plan: each class cut into chunks, dealt largest first (by cost
+    // evaluations) to the least loaded branch.
+    std::vector<nxs::BatchLaunch> deal(const nxs::BatchPlan &plan) const {
         using namespace nxs;
-        for (auto &c : classes)
-            HIP_TRY(nxs::launch_match_batch(c.p, tables + c.first, c.count, c.schedule, s));
+        std::vector<BatchLaunch> out;
+        for (int c = 0; c < (int)classes.size(); c++) {
+            const int step = plan.chunk > 0 ? plan.chunk : classes[c].count;
+            for (int k = 0; k < classes[c].count; k += step)
+                out.push_back(BatchLaunch{c, classes[c].first + k, std::min(step, classes[c].count - k), 0});
+        }
+        auto work = [&](const BatchLaunch &l) {
+            const MatchParams &p = classes[l.cls].p;
+            return (double)p.width * p.height * p.disparities * l.count;
+        };
+        std::stable_sort(out.begin(), out.end(),
+                         [&](const BatchLaunch &a, const BatchLaunch &b) { return work(a) > work(b); });
+        double load[MAX_BRANCHES] = {};
+        for (auto &l : out) {
+            int best = 0;
+            for (int b = 1; b < plan.branches; b++)
+                if (load[b] < load[best]) best = b;
+            l.branch = best;
+            load[best] += work(l);
+        }
+        return out;
+    }
+
+    // Every launch of the plan, forked from `s` onto the branches and joined back into it.
+    void enqueue_all(hipStream_t s, const nxs::BatchPlan &p,
+                     const std::vector<nxs::BatchLaunch> &ls) {
+        using namespace nxs;
+        if (p.branches <= 1) {
+            for (const auto &l : ls)
+                HIP_TRY(launch_match_batch(classes[l.cls].p, tables + l.first, l.count,
+                                           classes[l.cls].schedule, s));
+            return;
+        }
+        HIP_TRY(hipEventRecord(fork, s));
+        for (int b = 0; b < p.branches; b++) HIP_TRY(hipStreamWaitEvent(branch[b], fork, 0));
+        for (const auto &l : ls)
+            HIP_TRY(launch_match_batch(classes[l.cls].p, tables + l.first, l.count,
+                                       classes[l.cls].schedule, branch[l.branch]));
+        for (int b = 0; b < p.branches; b++) {
+            HIP_TRY(hipEventRecord(join[b], branch[b]));
+            HIP_TRY(hipStreamWaitEvent(s, join[b], 0));
+        }
+    }
+
+    // Times each plan on the whole batch (one warm run, then three) and keeps the fastest.
+    void tune_plan() {
+        using namespace nxs;
+        hipEvent_t start, stop;
+        HIP_TRY(hipEventCreate(&start));
+        HIP_TRY(hipEventCreate(&stop));
+        float best = 1e30f;
+        for (int branches : {1, 2, 4})
+            for (int chunk : {0, 8, 2}) {
+                const BatchPlan p{branches, chunk};
+                const std::vector<BatchLaunch> ls = deal(p);
+                enqueue_all(stream, p, ls);
+                HIP_TRY(hipEventRecord(start, stream));
+                for (int r = 0; r < 3; r++) enqueue_all(stream, p, ls);
+                HIP_TRY(hipEventRecord(stop, stream));
+                HIP_TRY(hipEventSynchronize(stop));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, start, stop));
+                trials += "  batch: " + std::to_string(branches) + " branches, " +
+                          (chunk ? std::to_string(chunk) + " pairs a launch" : std::string("a class a launch")) +
+                          ": " + std::to_string(ms / 3) + " ms\n";
+                if (ms < best) {
+                    best = ms;
+                    plan = p;
+                }
+            }
+        (void)hipEventDestroy(start);
+        (void)hipEventDestroy(stop);
     }
 
     std::string trials;  // every candidate's time, for the description
@@ -2089,6 +2189,11 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
         HIP_TRY(hipMalloc(&batch->workspace, total));
         HIP_TRY(hipMalloc((void **)&batch->tables, sizeof(PairDesc) * count));
         HIP_TRY(hipStreamCreateWithFlags(&batch->stream, hipStreamNonBlocking));
+        for (int b = 0; b < MAX_BRANCHES; b++) {
+            HIP_TRY(hipStreamCreateWithFlags(&batch->branch[b], hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&batch->join[b], hipEventDisableTiming));
+        }
+        HIP_TRY(hipEventCreateWithFlags(&batch->fork, hipEventDisableTiming));
         std::vector<PairDesc> host(count);
         uint8_t *ws = static_cast<uint8_t *>(batch->workspace);
         int next = 0;
@@ -2119,12 +2224,16 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
                                  ":\n";
                 batch->tune(c);
             }
+        // Without tuning: the classes in two branches, a class a launch.
+        batch->plan = BatchPlan{std::min(2, (int)batch->classes.size()), 0};
+        if (autotune) batch->tune_plan();
+        batch->launches = batch->deal(batch->plan);
         // One graph for the whole batch.
         t_capturing = true;
         hipError_t e = hipStreamBeginCapture(batch->stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
             try {
-                batch->enqueue_all(batch->stream);
+                batch->enqueue_all(batch->stream, batch->plan, batch->launches);
             } catch (const Error &) {
                 hipGraph_t broken = nullptr;
                 (void)hipStreamEndCapture(batch->stream, &broken);
@@ -2138,6 +2247,10 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
         HIP_TRY(e);
         HIP_TRY(hipGraphInstantiate(&batch->exec, batch->graph, nullptr, nullptr, 0));
         std::ostringstream text;
+        text << "plan: " << batch->plan.branches << " branches, "
+             << (batch->plan.chunk ? std::to_string(batch->plan.chunk) + " pairs a launch"
+                                   : std::string("a class a launch"))
+             << ", " << batch->launches.size() << " launch chains\n";
         for (auto &c : batch->classes)
             text << c.count << " x " << c.p.width << "x" << c.p.height << "x" << c.p.disparities
                  << " paths " << c.p.paths << ": prefetch " << c.schedule.prefetch << ", "

@@ -256,6 +256,32 @@ __device__ __forceinline__ Bytes<K> vload(Rsrc r, uint32_t at) {
     return b;
 }
 
+// The same with a cache policy for the load (2: non-temporal, streaming).
+template <int K, int AUX>
+__device__ __forceinline__ Bytes<K> vload_policy(Rsrc r, uint32_t at) {
+    static_assert(K >= 4, "whole dwords");
+    Bytes<K> b;
+    if constexpr (K == 4) {
+        b.w[0] = __builtin_amdgcn_raw_buffer_load_b32(r, at, 0, AUX);
+    } else if constexpr (K == 8) {
+        const v2u v = __builtin_amdgcn_raw_buffer_load_b64(r, at, 0, AUX);
+        b.w[0] = v[0];
+        b.w[1] = v[1];
+    } else if constexpr (K == 12) {
+        const v3u v = __builtin_amdgcn_raw_buffer_load_b96(r, at, 0, AUX);
+        b.w[0] = v[0];
+        b.w[1] = v[1];
+        b.w[2] = v[2];
+    } else {
+        const v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, at, 0, AUX);
+        b.w[0] = v[0];
+        b.w[1] = v[1];
+        b.w[2] = v[2];
+        b.w[3] = v[3];
+    }
+    return b;
+}
+
 template <int K>
 __device__ __forceinline__ void vstore(Rsrc r, uint32_t at, const Bytes<K> &b) {
     if constexpr (K == 2) {
@@ -995,7 +1021,10 @@ __global__ __launch_bounds__(128 * G) void path_sliced_kernel(PathArgs a) {
 // loads in flight).
 constexpr int SELECT_GROUPS = 2;
 
-template <int K>
+// STREAM: the volumes are read with non-temporal loads, which at sizes whose volumes
+// overflow the Infinity Cache keep them from displacing what later launches reuse
+// (1920x1080x256: 391 -> 347 us; at 728x544x128, whose volumes it holds, plain loads).
+template <int K, bool STREAM>
 __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const PairDesc *table,
                                                      int orientations, int W) {
     constexpr int D = 16 * K, R = K / 2;
@@ -1020,7 +1049,7 @@ __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const Pair
         const uint32_t at = offset_of(g);
 #pragma unroll
         for (int m = 0; m < 4; m++)
-            if (m < orientations) b[m] = vload<K>(vol[m], at);
+            if (m < orientations) b[m] = vload_policy<K, STREAM ? 2 : 0>(vol[m], at);
     };
     load(cur, 0);
 #pragma unroll
@@ -1392,8 +1421,15 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
     profile_end(slot, stream);
     slot = profile_begin(PK_SELECT, stream);
     const int select_pixels = 16 * SELECT_GROUPS;
-    hipLaunchKernelGGL(select_kernel<K>, dim3((W + select_pixels - 1) / select_pixels, H, count),
-                       dim3(256), 0, stream, single, table, n, W);
+    const dim3 select_grid((W + select_pixels - 1) / select_pixels, H, count);
+    // Volumes the 256 MB Infinity Cache cannot hold are streamed.
+    const bool stream_volumes = (size_t)n * W * H * 16 * K * count > ((size_t)256 << 20);
+    if (stream_volumes)
+        hipLaunchKernelGGL((select_kernel<K, true>), select_grid, dim3(256), 0, stream, single,
+                           table, n, W);
+    else
+        hipLaunchKernelGGL((select_kernel<K, false>), select_grid, dim3(256), 0, stream, single,
+                           table, n, W);
     profile_end(slot, stream);
     return hipGetLastError();
 }
