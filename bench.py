@@ -409,7 +409,20 @@ def served_leg(duration_s: float = 1.5) -> dict:
                                                     all_at_once=True).summary()["pairs_per_s"],
             "one_at_a_time_pairs_per_s": serving.serve_one_at_a_time(
                 ctx, segs, timed, arrivals, all_at_once=True).summary()["pairs_per_s"]}
+        # Offered load against latency: the streams' rates scaled, 1 s of arrivals a point,
+        # each stream's first 12 frames sent round again.
+        sweep = serving.load_sweep(ctx, segs, [r[:12] for r in timed], specs, SWEEP_FACTORS,
+                                   1.0, seed=1)
+    result["load_sweep"] = sweep
+    result["at_equal_p99"] = [serving.at_equal_p99(sweep, target) for target in SWEEP_P99_MS]
     return result
+
+
+# The served leg's offered loads (multiples of the streams' 840 pairs/s) and the p99 targets
+# at which the two forms' carried pairs/s are compared (the second is stream.exs's 33.3 ms
+# frame budget at 30 Hz).
+SWEEP_FACTORS = (0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 4.0)
+SWEEP_P99_MS = (5.0, 1000.0 / 30)
 
 
 # ---------------------------------------------------------------------------------------

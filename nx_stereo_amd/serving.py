@@ -111,12 +111,27 @@ class Served:
         return (self.done - self.arrival) * 1e3
 
     def summary(self) -> dict:
+        """pairs/s over the trace, and latency percentiles by nearest rank as
+        test/bench/stream.exs:208-225 reports them (the value at ceil(n p) - 1 of the sorted
+        latencies), with the count over its 33.3 ms frame budget."""
         lat = self.latency_ms
         span = float(self.done.max() - self.arrival.min())
         return {"pairs": int(len(lat)), "pairs_per_s": round(len(lat) / span, 2),
-                "p50_ms": round(float(np.percentile(lat, 50)), 3),
-                "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                "max_ms": round(float(lat.max()), 3)}
+                "p50_ms": round(nearest_rank(lat, 0.5), 3),
+                "p90_ms": round(nearest_rank(lat, 0.9), 3),
+                "p99_ms": round(nearest_rank(lat, 0.99), 3),
+                "max_ms": round(float(lat.max()), 3),
+                "over_budget": int((lat > BUDGET_MS).sum())}
+
+
+# stream.exs:37: a frame's budget at 30 Hz.
+BUDGET_MS = 1000.0 / 30
+
+
+def nearest_rank(values, fraction: float) -> float:
+    """stream.exs:223: the sorted values' element ceil(n * fraction) - 1."""
+    ordered = np.sort(np.asarray(values, np.float64))
+    return float(ordered[max(int(np.ceil(len(ordered) * fraction)) - 1, 0)])
 
 
 def _schedule(arrivals):
@@ -147,9 +162,16 @@ def serve(context: Context, segments, records, arrivals, keep_replies: bool = Fa
             replies[i] = ctypes.string_at(reply, length)
         remaining.release()
 
-    # Records as ctypes buffers up front: submitting copies them, nothing else.
-    buffers = [(ctypes.c_uint8 * len(records[s][k])).from_buffer_copy(records[s][k])
-               for _, s, k in order]
+    # Records as ctypes buffers up front: submitting copies them, nothing else. A stream
+    # with fewer records than arrivals sends them round again.
+    made = {}
+    buffers = []
+    for _, s, k in order:
+        key = (s, k % len(records[s]))
+        if key not in made:
+            rec = records[s][key[1]]
+            made[key] = (ctypes.c_uint8 * len(rec)).from_buffer_copy(rec)
+        buffers.append(made[key])
     lib = context.lib
     start[0] = time.perf_counter()
     for i, (t, s, k) in enumerate(order):
@@ -184,7 +206,7 @@ def serve_one_at_a_time(context: Context, segments, records, arrivals,
             wait = t - (time.perf_counter() - start)
             if wait > 0:
                 time.sleep(wait)
-        ticket = context.submit(records[s][k])
+        ticket = context.submit(records[s][k % len(records[s])])
         context.wait(ticket)
         reply = context.reply(ticket)
         done[i] = time.perf_counter() - start
@@ -199,3 +221,32 @@ def run_records(segments, frames):
     """records[s][k]: stream s's k-th pair as a run record."""
     return [[seg.run_record({"left": left, "right": right}) for left, right in fs]
             for seg, fs in zip(segments, frames)]
+
+
+def load_sweep(context: Context, segments, records, specs, factors, duration_s: float,
+               seed: int = 0) -> list[dict]:
+    """Offered load against latency, both ways: for each factor, every stream's Poisson
+    arrivals at factor x its rate for `duration_s`, served continuously and one pair at a
+    time (the Port's way); pairs/s carried and nearest-rank p50/p90/p99 for each."""
+    points = []
+    for factor in factors:
+        arrivals = [poisson_arrivals(spec.rate_hz * factor, duration_s, spec.seed + seed + 7919 * i)
+                    for i, spec in enumerate(specs)]
+        offered = sum(len(a) for a in arrivals) / duration_s
+        point = {"factor": factor, "offered_pairs_per_s": round(offered, 1)}
+        point["continuous"] = serve(context, segments, records, arrivals).summary()
+        point["one_at_a_time"] = serve_one_at_a_time(context, segments, records, arrivals).summary()
+        points.append(point)
+    return points
+
+
+def at_equal_p99(points, target_ms: float) -> dict:
+    """The most pairs/s each form carried in the sweep with its p99 latency within
+    `target_ms`, and the offered load it carried it at."""
+    out = {"p99_target_ms": round(target_ms, 3)}
+    for form in ("continuous", "one_at_a_time"):
+        ok = [p for p in points if p[form]["p99_ms"] <= target_ms]
+        best = max(ok, key=lambda p: p[form]["pairs_per_s"]) if ok else None
+        out[form] = ({"pairs_per_s": best[form]["pairs_per_s"], "p99_ms": best[form]["p99_ms"],
+                      "offered_pairs_per_s": best["offered_pairs_per_s"]} if best else None)
+    return out
