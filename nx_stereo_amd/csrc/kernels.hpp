@@ -168,6 +168,38 @@ size_t quality_scratch_bytes(int width, int height);
 hipError_t launch_quality(const float *disparity, float *previous, int width, int height,
                           double *partials, float *sums5, hipStream_t stream);
 
+// The per-pixel tail fused (stages.hip): an ordered run of SpeckleFilter (radius 3),
+// TemporalFilter, Depth, PointCloud (only after Depth in the run) and Quality from the map
+// `in`. `out` takes the run's map where SpeckleFilter or TemporalFilter is in it (in != out
+// with SpeckleFilter). `kept_accum` starts at zero (each finish leaves it at zero);
+// `totals` as launch_quality's partials.
+struct TailArgs {
+    const float *in = nullptr;
+    float *out = nullptr;
+    int W = 0, H = 0;
+    int despeckle = 0;
+    float min_support = 0, max_diff = 0;
+    int temporal = 0;
+    float *t_prev = nullptr, *t_missed = nullptr;
+    float alpha = 0, delta = 0, hold = 0, max_missed = 0;
+    int depth = 0;
+    float focal_baseline = 0;
+    uint16_t *mm = nullptr;
+    float *m = nullptr;
+    int cloud = 0;
+    int dec = 1, gw = 0, gh = 0;
+    float f = 0, cx = 0, cy = 0, nearest = 0, farthest = 0;
+    float *points = nullptr;
+    int32_t *kept_accum = nullptr, *kept_total = nullptr;
+    int quality = 0;
+    float *q_prev = nullptr;
+    double *totals = nullptr;
+    float *sums5 = nullptr;
+};
+hipError_t launch_tail(const TailArgs &a, hipStream_t stream);
+// After launch_tail: the quality sums and the kept count (nothing when neither is in the run).
+hipError_t launch_tail_finish(const TailArgs &a, hipStream_t stream);
+
 // MedianFilter: kernel_size 3 or 5.
 hipError_t launch_median_filter(const float *in, float *out, int width, int height,
                                 int kernel_size, hipStream_t stream);

@@ -564,6 +564,9 @@ struct Segment {
     // (and on the segment's own stream before a run on another follows it).
     hipStream_t last_stream = nullptr;
     hipEvent_t order = nullptr;
+    // Runs of consecutive per-pixel stages in one launch (launch_tail); NXS_FUSE_TAIL=0 at
+    // configure keeps every stage a launch of its own.
+    bool fuse_tail = true;
     bool foreign = false;  // a run has been enqueued on a caller's stream
 
     // device buffers carved from the arena
@@ -571,6 +574,7 @@ struct Segment {
     float *disparity[2]{};
     uint16_t *depth_mm = nullptr;
     float *depth_m = nullptr, *cloud = nullptr, *quality_sums = nullptr;
+    int32_t *tail_kept = nullptr;  // the fused tail's kept-point accumulator (left at zero)
     uint64_t *sharpness_totals = nullptr, *alignment_totals = nullptr;
     float *sharpness_sums = nullptr, *alignment_sums = nullptr;
     double *quality_partials = nullptr;
@@ -700,7 +704,7 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
     const size_t taps_bytes = rows_taps.bytes() + cols_taps.bytes();
     size_t bytes = up256(F) * 2 + up256(P) * 2 + up256(4 * P) * 2 + up256(2 * P) + up256(4 * P) +
                    up256(4 * cloud) + up256(4 * (partials + 1)) +
-                   up256(quality_scratch_bytes(g.width, g.height)) + up256(32) +
+                   up256(quality_scratch_bytes(g.width, g.height)) + up256(32) + up256(32) +
                    up256(sharp) + up256(32) + up256(align) + up256(4 * align_sums) +
                    up256(4 * P) * carries + up256(taps_bytes) + 256;
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -731,6 +735,8 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
     HIP_TRY(launch_fill_f32(reinterpret_cast<float *>(s->quality_partials), 0.0f,
                             quality_scratch_bytes(g.width, g.height) / sizeof(float), s->stream));
     s->quality_sums = reinterpret_cast<float *>(carve(4 * 8));
+    s->tail_kept = reinterpret_cast<int32_t *>(carve(4 * 8));
+    HIP_TRY(launch_fill_f32(reinterpret_cast<float *>(s->tail_kept), 0.0f, 8, s->stream));
     // The sharpness and alignment totals start at zero (each launch leaves them at zero).
     s->sharpness_totals = reinterpret_cast<uint64_t *>(carve(sharp));
     s->sharpness_sums = reinterpret_cast<float *>(carve(4 * 8));
@@ -748,6 +754,7 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
         s->decode_cols = cols_taps.upload(taps + rows_taps.bytes());
     }
     s->workspace_bytes = match;
+    if (const char *fuse = std::getenv("NXS_FUSE_TAIL")) s->fuse_tail = std::atoi(fuse) != 0;
     for (auto &m : s->members) {
         float *a = nullptr, *b = nullptr;
         if (m.kind == Kind::TemporalFilter) {
@@ -836,6 +843,36 @@ struct Enqueued {
     std::map<std::string, Slot> tensors;
     std::vector<const char *> filled;
 };
+
+// The fused per-pixel tail (launch_tail): SpeckleFilter of radius 3, TemporalFilter, Depth,
+// PointCloud and Quality may run as one launch (and one finishing block) where consecutive
+// members take them in this order, PointCloud only after Depth.
+static int tail_rank(const Member &m) {
+    switch (m.kind) {
+        case Kind::SpeckleFilter: return m.radius == 3 ? 0 : -1;
+        case Kind::TemporalFilter: return 1;
+        case Kind::Depth: return 2;
+        case Kind::PointCloud: return 3;
+        case Kind::Quality: return 4;
+        default: return -1;
+    }
+}
+
+// How many members from i on form one fused tail.
+static size_t tail_length(const std::vector<Member> &members, size_t i) {
+    size_t n = 0;
+    int last = -1;
+    bool depth = false;
+    while (i + n < members.size()) {
+        const Member &m = members[i + n];
+        const int rank = tail_rank(m);
+        if (rank <= last || (m.kind == Kind::PointCloud && !depth)) break;
+        depth = depth || m.kind == Kind::Depth;
+        last = rank;
+        n++;
+    }
+    return n;
+}
 
 // Runs of one segment share its arena, carries and workspace, so each waits for the one
 // before it, whichever stream that went on: before a run on `stream`, the stream waits for
@@ -937,9 +974,103 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
 
     std::vector<const char *> &filled = q.filled;
     if (!events.empty()) HIP_TRY(hipEventRecord(events[0], stream));
+    auto note_outputs = [&](const Member &m) {
+        for (const char *name : member_outputs(m.kind)) {
+            bool seen = false;
+            for (const char *f : filled) seen = seen || std::strcmp(f, name) == 0;
+            if (!seen) filled.push_back(name);
+        }
+    };
     for (size_t i = 0; i < s.members.size(); i++) {
         const Member &m = s.members[i];
         const Inputs &in = inputs[i];
+        if (const size_t run = s.fuse_tail ? tail_length(s.members, i) : 0; run >= 2) {
+            // Members i .. i+run-1 in one launch. Each member's inputs are read in member
+            // order, as its own step reads them; the first member's time is the launch's,
+            // the last one's the finish's.
+            TailArgs a;
+            a.W = (int)W;
+            a.H = (int)H;
+            for (size_t k = i; k < i + run; k++) {
+                const Member &mk = s.members[k];
+                const Inputs &ink = inputs[k];
+                switch (mk.kind) {
+                    case Kind::SpeckleFilter: {
+                        auto *d = (const float *)shaped(tensors, "disparity", TYPE_F32, image, "f32 map");
+                        a.despeckle = 1;
+                        a.min_support = scalar(ink, "min_support");
+                        a.max_diff = scalar(ink, "max_diff");
+                        a.in = d;
+                        a.out = s.disparity[0] == d ? s.disparity[1] : s.disparity[0];
+                        tensors["disparity"] = Slot{TYPE_F32, image, a.out};
+                        free_disparity = s.disparity[0] == a.out ? 1 : 0;
+                        break;
+                    }
+                    case Kind::TemporalFilter: {
+                        a.alpha = scalar(ink, "alpha");
+                        a.delta = scalar(ink, "delta");
+                        a.hold = scalar(ink, "hold");
+                        auto *d = (float *)shaped(tensors, "disparity", TYPE_F32, image, "f32 map");
+                        if (!a.in) a.in = a.out = d;
+                        a.temporal = 1;
+                        a.t_prev = s.carry_a[k];
+                        a.t_missed = s.carry_b[k];
+                        a.max_missed = mk.max_missed;
+                        break;
+                    }
+                    case Kind::Depth: {
+                        auto *d = (const float *)shaped(tensors, "disparity", TYPE_F32, image, "f32 map");
+                        const float f = scalar(ink, "focal_length_px"), b = scalar(ink, "baseline_m");
+                        if (!a.in) a.in = d;
+                        a.depth = 1;
+                        a.focal_baseline = f * b;  // host f32 multiply, as depth.rs:20
+                        a.mm = s.depth_mm;
+                        a.m = s.depth_m;
+                        tensors["depth_mm"] = Slot{TYPE_U16, image, s.depth_mm};
+                        tensors["depth_m"] = Slot{TYPE_F32, image, s.depth_m};
+                        break;
+                    }
+                    case Kind::PointCloud: {
+                        a.f = scalar(ink, "focal_length_px");
+                        a.cx = scalar(ink, "cx");
+                        a.cy = scalar(ink, "cy");
+                        (void)shaped(tensors, "depth_m", TYPE_F32, image, "f32 map");
+                        a.cloud = 1;
+                        a.dec = mk.decimation;
+                        a.gw = (int)W / mk.decimation;
+                        a.gh = (int)H / mk.decimation;
+                        a.nearest = mk.nearest;
+                        a.farthest = mk.farthest;
+                        a.points = s.cloud;
+                        a.kept_accum = s.tail_kept;
+                        a.kept_total = s.cloud_kept;
+                        tensors["cloud_grid"] =
+                            Slot{TYPE_F32, {H / mk.decimation, W / mk.decimation, 3}, s.cloud};
+                        tensors["cloud_points"] = Slot{TYPE_S32, {}, s.cloud_kept};
+                        break;
+                    }
+                    case Kind::Quality: {
+                        auto *d = (const float *)shaped(tensors, "disparity", TYPE_F32, image, "f32 map");
+                        if (!a.in) a.in = d;
+                        a.quality = 1;
+                        a.q_prev = s.carry_a[k];
+                        a.totals = s.quality_partials;
+                        a.sums5 = s.quality_sums;
+                        tensors["quality_sums"] = Slot{TYPE_F32, {5}, s.quality_sums};
+                        break;
+                    }
+                    default: throw Error{"a member that cannot join a fused tail"};
+                }
+            }
+            HIP_TRY(launch_tail(a, stream));
+            for (size_t k = i; k + 1 < i + run; k++)
+                if (!events.empty()) HIP_TRY(hipEventRecord(events[k + 1], stream));
+            HIP_TRY(launch_tail_finish(a, stream));
+            if (!events.empty()) HIP_TRY(hipEventRecord(events[i + run], stream));
+            for (size_t k = i; k < i + run; k++) note_outputs(s.members[k]);
+            i += run - 1;
+            continue;
+        }
         switch (m.kind) {
             case Kind::Decode: {
                 auto *lf = (const uint8_t *)shaped(tensors, "left_frame", TYPE_U8, frame, "u8 image");
@@ -1059,11 +1190,7 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
             }
         }
         if (!events.empty()) HIP_TRY(hipEventRecord(events[i + 1], stream));
-        for (const char *name : member_outputs(m.kind)) {
-            bool seen = false;
-            for (const char *f : filled) seen = seen || std::strcmp(f, name) == 0;
-            if (!seen) filled.push_back(name);
-        }
+        note_outputs(m);
     }
 
     return q;

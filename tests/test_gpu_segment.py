@@ -479,3 +479,72 @@ def test_device_entries_run_in_place(context):
     assert [(o[0], o[1], o[2]) for o in outs] == [("disparity", 2, (64, 96))]
     hip.synchronize()
     same_bits(hip.copy_to_host(outs[0][3], (64, 96), np.float32), oracle.sgm(left, right, 64, 8))
+
+
+TAIL_CHAINS = [
+    ["speckle_filter", "temporal_filter", "depth", "point_cloud", "quality"],
+    ["speckle_filter", "depth", "point_cloud", "quality"],
+    ["temporal_filter", "quality"],
+    ["depth", "point_cloud"],
+    ["speckle_filter", "quality"],
+    ["quality", "depth", "point_cloud"],  # quality first: it runs alone, the rest fused
+    ["depth", "quality", "point_cloud"],  # the cloud after quality: not in the fused run
+]
+
+
+def tail_member(name, decimation):
+    return {"speckle_filter": lambda: stages.SpeckleFilter(min_support=9, max_diff=1.5),
+            "temporal_filter": lambda: stages.TemporalFilter(alpha=0.4, delta=2.0, hold=2),
+            "depth": lambda: stages.Depth(),
+            "point_cloud": lambda: stages.PointCloud(decimation=decimation),
+            "quality": lambda: stages.Quality()}[name]()
+
+
+@pytest.mark.parametrize("chain", TAIL_CHAINS, ids=["-".join(c) for c in TAIL_CHAINS])
+@pytest.mark.parametrize("size,decimation", [((200, 77), 3), ((130, 49), 8), ((64, 16), 4)])
+def test_fused_tail_equals_the_stages_one_by_one(context, chain, size, decimation, monkeypatch):
+    """Consecutive per-pixel stages run as one fused launch (and a finishing block); with
+    NXS_FUSE_TAIL=0 at configure each runs alone. Both give the same bits on three pairs in a
+    row (the carries), at sizes off the 64x16 tiles and decimations that put two grid rows in
+    one thread (8) or none (3), and both equal the oracle's chain."""
+    w, h = size
+    geometry, _ = rig_geometry(w, h)
+    rng = np.random.default_rng(w + h + decimation)
+    maps = []
+    for k in range(3):
+        m = np.where(rng.random((h, w)) < 0.25, -1.0, rng.uniform(0.3, 60.0, (h, w))).astype(np.float32)
+        m[h // 4:h // 2, w // 4:w // 2] = 9.5 + 0.25 * k
+        maps.append(m)
+    outs = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("NXS_FUSE_TAIL", fuse)
+        seg = build(context, [(name, tail_member(name, decimation)) for name in chain], geometry)
+        outs[fuse] = [run_ok(seg, {"disparity": m}) for m in maps]
+        seg.release()
+    f, b = geometry["focal_length_px"], geometry["baseline_m"]
+    cx, cy = geometry["principal_point"]
+    temporal, quality = oracle.Temporal((h, w)), oracle.Quality((h, w))
+    for k, m in enumerate(maps):
+        fused, alone = outs["1"][k], outs["0"][k]
+        assert list(fused) == list(alone)
+        for name in fused:
+            same_bits(fused[name], alone[name])
+        d = m.copy()
+        mm = metres = None
+        for name in chain:
+            if name == "speckle_filter":
+                d = oracle.despeckle(d, 9, 1.5)
+            elif name == "temporal_filter":
+                d = temporal.filter(d, 0.4, 2.0, 2)
+            elif name == "depth":
+                mm, metres = oracle.depth(d, f, b)
+            elif name == "point_cloud":
+                points, kept = oracle.cloud(metres, decimation, f, cx, cy)
+                same_bits(fused["cloud_grid"], points)
+                assert int(fused["cloud_points"]) == kept
+            elif name == "quality":
+                np.testing.assert_allclose(fused["quality_sums"], quality.sums(d), rtol=1e-5)
+        if "disparity" in fused:
+            same_bits(fused["disparity"], d)
+        if mm is not None:
+            same_bits(fused["depth_mm"], mm)
