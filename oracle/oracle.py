@@ -83,6 +83,12 @@ def cpu() -> ctypes.CDLL:
         i, f = ctypes.c_int, ctypes.c_float
         _cpu.cpu_sgm.argtypes = [_P, _P, i, i, i, i, i, i, f, _P, i]
         _cpu.cpu_sgm.restype = i
+        _cpu.cpu_rectify_pair.argtypes = [_P, _P, i, i] + [_P] * 6 + [i, i, _P, _P, i]
+        _cpu.cpu_rectify_pair.restype = i
+        _cpu.cpu_despeckle.argtypes = [_P, i, i, f, f, i, _P, i]
+        _cpu.cpu_despeckle.restype = i
+        _cpu.cpu_depth.argtypes = [_P, i, i, f, f, _P, _P, i]
+        _cpu.cpu_depth.restype = i
         _cpu.cpu_max_threads.restype = i
     return _cpu
 
@@ -388,3 +394,55 @@ def cpu_sgm(left, right, disparities=64, paths=8, p1=8, p2=64, uniqueness=1.0526
     if rc:
         raise MemoryError("cpu baseline ran out of memory")
     return out
+
+
+def cpu_rectify_pair(left_frame, right_frame, left_camera, right_camera, size, threads=0):
+    """The multi-threaded baseline's Rectification: both frames through their cameras' maps
+    (each built once and kept while its camera is the same), rows from a queue."""
+    left_frame, right_frame = _c(left_frame, np.uint8), _c(right_frame, np.uint8)
+    fh, fw = left_frame.shape
+    h, w = size
+    outs = np.empty((h, w), np.uint8), np.empty((h, w), np.uint8)
+    cams = [[_c(cam[key], np.float32) for key in ("k", "d", "inverse_pr")]
+            for cam in (left_camera, right_camera)]
+    rc = cpu().cpu_rectify_pair(_ptr(left_frame), _ptr(right_frame), fw, fh,
+                                *[_ptr(a) for a in cams[0] + cams[1]], w, h, _ptr(outs[0]),
+                                _ptr(outs[1]), threads)
+    if rc:
+        raise MemoryError("cpu baseline ran out of memory")
+    return outs
+
+
+def cpu_despeckle(disparity, min_support, max_diff, radius=3, threads=0):
+    """The multi-threaded baseline's SpeckleFilter, rows from a queue."""
+    disparity = _c(disparity, np.float32)
+    h, w = disparity.shape
+    out = np.empty_like(disparity)
+    rc = cpu().cpu_despeckle(_ptr(disparity), w, h, float(min_support), float(max_diff), radius,
+                             _ptr(out), threads)
+    if rc:
+        raise MemoryError("cpu baseline ran out of memory")
+    return out
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), for the baseline's report."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def cpu_depth(disparity, focal_length_px, baseline_m, threads=0):
+    """The multi-threaded baseline's Depth, rows from a queue: (depth_mm, depth_m)."""
+    disparity = _c(disparity, np.float32)
+    h, w = disparity.shape
+    mm, m = np.empty((h, w), np.uint16), np.empty((h, w), np.float32)
+    cpu().cpu_depth(_ptr(disparity), w, h, float(np.float32(focal_length_px)),
+                    float(np.float32(baseline_m)), _ptr(mm), _ptr(m), threads)
+    return mm, m

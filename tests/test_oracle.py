@@ -343,6 +343,47 @@ def test_cpu_baseline_matches_the_oracle_on_a_planar_pair_at_256():
         assert (oracle.cpu_sgm(left, right, 256, 8, threads=threads).view(np.uint32) == want).all()
 
 
+@pytest.mark.parametrize("threads", [1, 2, 7, 16])
+def test_cpu_baseline_ring_and_chunks(threads):
+    """3 paths through a ring of 3 rows (600 x 256 bytes a row: each slot refilled 13 times),
+    and 8 paths over up to 37 column chunks whose neighbours hold each other back."""
+    left, right, _ = scenes.planar_pair(600, 40, seed=4, low=4, high=200)
+    for paths in (3, 8):
+        want = oracle.sgm(left, right, 256, paths).view(np.uint32)
+        got = oracle.cpu_sgm(left, right, 256, paths, threads=threads).view(np.uint32)
+        assert (got == want).all(), (paths, threads)
+
+
+def test_cpu_rectification_despeckle_and_depth_equal_the_oracle():
+    """The baseline's map-based Rectification (the map built once per camera and kept) and
+    its row-parallel SpeckleFilter, against the oracle's, over two calls (the second reusing
+    the maps) and a changed camera (a new map)."""
+    from nx_stereo_amd import stages
+
+    cal = scenes.rig(200, 120)
+    cams = {side: scenes.camera(**cal[side]) for side in ("left", "right")}
+    frames = [scenes.noise(200, 120, 1), scenes.noise(200, 120, 2)]
+    for offset in (0.0, 0.0, 1.5):
+        right = dict(cams["right"], inverse_pr=stages.move_rows(cams["right"]["inverse_pr"], offset))
+        for size in ((120, 200), (60, 100)):
+            got = oracle.cpu_rectify_pair(frames[0], frames[1], cams["left"], right, size, threads=5)
+            for image, frame, cam in zip(got, frames, (cams["left"], right)):
+                want = oracle.rectify(frame, cam["k"], cam["d"], cam["inverse_pr"], size)
+                assert np.array_equal(image, want), (offset, size)
+    rng = np.random.default_rng(3)
+    d = np.where(rng.random((53, 71)) < 0.3, -1.0, rng.uniform(0.5, 20.0, (53, 71))).astype(np.float32)
+    for f, b in ((873.0, 0.2), (400.0, 0.06)):
+        want_mm, want_m = oracle.depth(d, f, b)
+        got_mm, got_m = oracle.cpu_depth(d, f, b, threads=4)
+        assert np.array_equal(got_mm, want_mm) and np.array_equal(got_m.view(np.uint32),
+                                                                    want_m.view(np.uint32))
+    for radius, support, diff in ((3, 12, 1.0), (1, 3, 2.5), (5, 30, 0.5)):
+        want = oracle.despeckle(d, support, diff, radius)
+        for threads in (1, 6):
+            got = oracle.cpu_despeckle(d, support, diff, radius, threads=threads)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (radius, threads)
+
+
 def test_cpu_baseline_other_penalties_and_sizes():
     left, right = scenes.scene(80, 40, 3)
     for p1, p2, u in [(0, 1, 1.0), (4, 40, 1.2), (78, 79, 2.5), (10, 79, 1.0526316)]:
