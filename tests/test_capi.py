@@ -95,3 +95,12 @@ def test_the_no_alloc_matcher_refuses_what_its_scratch_cannot_hold_before_any_de
     assert _native.NOALLOC_SCRATCH_BYTES == 64 << 20
     text = open(os.path.join(ROOT, "include", "nxs_hip.h")).read()
     assert "#define NXS_NOALLOC_SCRATCH_BYTES (64u << 20)" in text
+
+
+def test_the_nif_calls_only_entry_points_the_header_declares():
+    """c_src/nxs_nif.c (compiled where Erlang's headers are) goes through include/nxs_hip.h
+    alone: every nxs_ call it makes is declared there."""
+    text = open(os.path.join(ROOT, "nx_stereo_amd", "elixir", "c_src", "nxs_nif.c")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    called = set(re.findall(r"\b(nxs_[a-z_]+)\s*\(", text))
+    assert called and called <= set(declared_functions()), called - set(declared_functions())
