@@ -327,6 +327,37 @@ def streams_leg(pairs: int = 96) -> dict:
     return out
 
 
+def host_io_leg(pairs: int = 200) -> dict:
+    """The main-line pipeline as a caller with host buffers runs it: each pair's raw frames
+    in a run record through nxs_submit (copied to pinned staging, uploaded), every output
+    (rectified images, disparity, depth_mm, cloud, kept count, quality sums) downloaded into
+    the reply. Pipelined (every pair submitted at once; the segment's three run slots keep
+    up to three in flight), and one at a time (each reply waited for), as a Port's caller
+    sees it."""
+    from nx_stereo_amd import protocol, segment, serving
+
+    geometry, members, left_frame, right_frame = workload()
+    with segment.Context(0) as context:
+        ok, seg = segment.HipSegment.build(members, geometry, context)
+        if context.request(seg.configure_record()) is not True:
+            raise RuntimeError("configure refused")
+        seg.configured_version = seg.version
+        record = seg.run_record({"left_frame": left_frame, "right_frame": right_frame})
+        reply = context.request(record)
+        if not isinstance(reply, protocol.Outputs):
+            raise RuntimeError(f"run refused: {reply}")
+        down = sum(t.nbytes for t in reply.tensors.values())
+        arrivals = [[0.0] * pairs]
+        serving.serve(context, [seg], [[record]], [[0.0] * 8], all_at_once=True)  # warm
+        pipelined = serving.serve(context, [seg], [[record]], arrivals, all_at_once=True).summary()
+        sequential = serving.serve_one_at_a_time(context, [seg], [[record]], [[0.0] * 50],
+                                                 all_at_once=True).summary()
+    return {"workload": "the main-line pipeline with 728x544 host frames in (792 KB a pair) and "
+                        "every output back (%d bytes a pair), through nxs_submit" % down,
+            "pipelined_pairs_per_s": pipelined["pairs_per_s"],
+            "one_at_a_time_pairs_per_s": sequential["pairs_per_s"]}
+
+
 BATCH_CLASSES = [(480, 270, 64, 24), (728, 544, 128, 48), (1280, 720, 192, 24)]
 
 
@@ -551,9 +582,9 @@ def main() -> int:
     parser.add_argument("--steps", type=int, default=200)
     parser.add_argument("--warmup", type=int, default=20)
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    parser.add_argument("--legs", default="large,batched,served,coarse_to_fine,streams",
+    parser.add_argument("--legs", default="large,batched,served,coarse_to_fine,streams,host_io",
                         help="comma-separated extra legs: large, batched, served, "
-                             "coarse_to_fine, streams (or none)")
+                             "coarse_to_fine, streams, host_io (or none)")
     parser.add_argument("--kernels-only", choices=sorted(TRAFFIC_FILES),
                         help="run three matchings and the calibration copy (c2f*: five "
                              "coarse-to-fine matchings), for rocprofv3")
@@ -693,6 +724,8 @@ def main() -> int:
         result["coarse_to_fine"] = coarse_to_fine_leg()
     if "streams" in legs:
         result["streams"] = streams_leg()
+    if "host_io" in legs:
+        result["host_io"] = host_io_leg()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(geometry, members, left_frame, right_frame)
     print(json.dumps(result), flush=True)
