@@ -65,6 +65,9 @@ int slices_fit(int disparities, int slices);
 // Whether `lanes` lanes per line fit D disparities (D / lanes is 2, 4, 8, 12 or 16).
 bool lanes_fit(int disparities, int lanes);
 
+// Whether two waves per line can take 32 lanes a line: D / 32 is 2, 4 or 8.
+bool two_lanes_fit(int disparities);
+
 // The schedule launches at this size use: the autotuner's pick, or the default.
 Schedule schedule_for(int width, int height, int disparities, int paths);
 void remember_schedule(int width, int height, int disparities, int paths, const Schedule &s);

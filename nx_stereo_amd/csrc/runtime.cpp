@@ -71,21 +71,34 @@ static std::map<std::tuple<int, int, int, int>, Schedule> g_schedules;
 // registers cost occupancy. Measured on MI355X at 8 paths: 728x544x128 170 us (prefetch 4:
 // 179), 1280x720x192 637 us (prefetch 8: 666), 1920x1080x256 fastest with 8, 960x540x64
 // even.
-static Schedule default_schedule(int disparities, int paths) {
+static Schedule default_schedule(int width, int height, int disparities, int paths) {
     Schedule s;
     // At 3 paths the rows' walk bounds the kernel (the down-only columns take four lines a
     // wave), and a prefetch of 8 covers it best: 38 us at 364x272x64 (coarse-to-fine's half
-    // images) and 89 us at 728x544x128, against 71 and 156 us with one wave a line.
+    // images) and 92 us at 728x544x128, against 71 and 156 us with one wave a line, and 40
+    // and 110 us with 32 lanes a line.
     s.prefetch = disparities == 192 ? 4 : 8;
     s.lanes = 16;
     s.waves = 2;
+    // At 8 paths and 128 disparities, 32 lanes a line (half the instructions a step in each
+    // wave, twice the waves) walks the lines as fast or faster: 364x272 51-54 against 68 us,
+    // 728x544 159 against 169-172, 1024x768 332 against 348, 640x480, 1280x720 and 1920x1080
+    // the same; a prefetch of 4 is best up to 728x544 and 8 above (1024x768: 368 against
+    // 332). At 64 disparities 16 lanes stay (640x480: 76 against 103-105 us; equal or a
+    // little slower below 400x300, though 480x270 alone walks 53 against 91 us with 32). At
+    // 256 the kernel is bound by its traffic (1920x1080: 1916 against 1981 us with 32).
+    if (paths == 8 && disparities == 128) {
+        s.lanes = 32;
+        s.prefetch = (size_t)width * height <= 500000 ? 4 : 8;
+    }
     return s;
 }
 
 Schedule schedule_for(int width, int height, int disparities, int paths) {
     std::lock_guard<std::mutex> lock(g_schedules_mu);
     auto it = g_schedules.find(std::make_tuple(width, height, disparities, paths));
-    return it == g_schedules.end() ? default_schedule(disparities, paths) : it->second;
+    return it == g_schedules.end() ? default_schedule(width, height, disparities, paths)
+                                   : it->second;
 }
 
 void remember_schedule(int width, int height, int disparities, int paths, const Schedule &s) {
@@ -2253,6 +2266,8 @@ struct nxs_batch {
         for (int waves : {2, 4, 8})
             if (waves == 2 || (c.p.paths == 8 && slices_fit(c.p.disparities, waves / 2) > 1))
                 for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, 16, waves});
+        if (two_lanes_fit(c.p.disparities))
+            for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, 32, 2});
         candidates.push_back(Schedule{4, true, 16, 1});
         for (const Schedule &s : candidates) {
             HIP_TRY(launch_match_batch(c.p, tables + c.first, c.count, s, stream));
@@ -2265,7 +2280,7 @@ struct nxs_batch {
             HIP_TRY(hipEventElapsedTime(&ms, start, stop));
             trials += "  prefetch " + std::to_string(s.prefetch) + ", " +
                       std::to_string(s.lanes) + " lanes, " + std::to_string(s.waves) +
-                      (s.waves == 2 ? " waves" : " wave") + (s.split ? ", split" : "") + ": " +
+                      (s.waves > 1 ? " waves" : " wave") + (s.split ? ", split" : "") + ": " +
                       std::to_string(ms / 3) + " ms\n";
             if (ms < best) {
                 best = ms;
@@ -2402,8 +2417,11 @@ int nxs_schedule_set(int width, int height, int disparities, int paths, const nx
                                          std::to_string(disparities) + " disparities");
     if (s->waves != 1 && s->waves != 2 && s->waves != 4 && s->waves != 8)
         return nxs::fail(NXS_EINVAL, "waves must be 1, 2, 4 or 8");
-    if (s->waves > 1 && s->lanes != 16)
-        return nxs::fail(NXS_EINVAL, "two or more waves per line take 16 lanes per line");
+    if (s->waves > 2 && s->lanes != 16)
+        return nxs::fail(NXS_EINVAL, "four or more waves per line take 16 lanes per line");
+    if (s->waves == 2 && s->lanes != 16 && !(s->lanes == 32 && nxs::two_lanes_fit(disparities)))
+        return nxs::fail(NXS_EINVAL, "two waves per line take 16 lanes per line, or 32 at 64, "
+                                     "128 or 256 disparities");
     if (s->waves > 2 && paths != 8)
         return nxs::fail(NXS_EINVAL, std::to_string(s->waves) + " waves per line take 8 paths");
     if (s->waves > 2 && nxs::slices_fit(disparities, s->waves / 2) == 1)

@@ -643,7 +643,9 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
 
 constexpr int TWO_BLOCK = 128;
 
-template <int K, int PREFETCH>
+// LANES: lanes per line, 16 (four lines a wave) or 32 (two lines a wave, K = D / 32: half
+// the instructions per step in each wave, twice the waves).
+template <int K, int PREFETCH, int LANES>
 __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
     // Steps after which a store has completed (the compiler's own wait for the loads of
     // PREFETCH steps ago already implies it), the window's half-width, and the ring's depth.
@@ -651,13 +653,14 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
     constexpr int X = 2 * PREFETCH + LAG + 2;
     constexpr int RING = X + 1 <= 16 ? 16 : 32;
     static_assert(X <= RING - 1, "the LDS ring must hold the window's late steps");
-    constexpr int D = 16 * K;
-    constexpr int LINES = 4;
+    static_assert(LANES == 16 || LANES == 32, "a line is one or two DPP rows");
+    constexpr int D = LANES * K;
+    constexpr int LINES = 64 / LANES;
     // Own bytes per direction, line, step (mod RING) and lane.
-    __shared__ Bytes<K> ring[2][LINES][RING][16];
+    __shared__ Bytes<K> ring[2][LINES][RING][LANES];
     const int dir = threadIdx.x >> 6;  // wave-uniform: 0 forward, 1 backward
-    const int lane = threadIdx.x & 15;
-    const int row = (threadIdx.x >> 4) & 3;
+    const int lane = threadIdx.x & (LANES - 1);
+    const int row = (threadIdx.x & 63) / LANES;
     const int b = blockIdx.x;
     int o = a.last;
     for (int k = a.first; k < a.last; k++)
@@ -667,9 +670,9 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
         }
     const int W = a.W, H = a.H;
     const PairDesc pd = pair_of(a.single, a.table, blockIdx.y);
-    constexpr uint32_t D_BYTES = 16 * K;
+    constexpr uint32_t D_BYTES = LANES * K;
     if (o == 1 && !a.backward_columns) {
-        // Columns with 3 paths: down only. The block takes eight columns, each wave four of
+        // Columns with 3 paths: down only. The block takes 2 LINES columns, each wave LINES of
         // its own, walked alone (no meeting, no barrier), storing its own path costs.
         const int first_column = (b - a.block_start[o]) * 2 * LINES + dir * LINES;
         const LineGeometry gc = line_geometry(1, first_column + row, W, H);
@@ -685,7 +688,7 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
         const Rsrc paired_c = buffer_of(orientation_volume(pd.b, 1), volume_c);
         const uint32_t p1x2c = (uint32_t)pd.p1 * 0x10001u, p2c = (uint32_t)pd.p2;
         const uint32_t first_c = (uint32_t)gc.x0 * D_BYTES + lane * K;
-        PathState<K, 16> sc;
+        PathState<K, LANES> sc;
         sc.reset(p1x2c, lane);
         Bytes<K> ring_c[PREFETCH];
         Bytes<K> none_c;
@@ -742,7 +745,7 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
     const uint32_t start = dir == 0 ? first : last;
     const uint32_t step = dir == 0 ? stride : 0u - stride;
 
-    PathState<K, 16> s;
+    PathState<K, LANES> s;
     s.reset(p1x2, lane);
     Bytes<K> cost[PREFETCH], other[PREFETCH];
     Bytes<K> none;
@@ -1373,6 +1376,19 @@ static void launch_sliced(const PathArgs &a, int blocks, int count, int prefetch
     }
 }
 
+bool two_lanes_fit(int disparities) {
+    return disparities == 64 || disparities == 128 || disparities == 256;
+}
+
+template <int K, int LANES>
+static void launch_two(const PathArgs &a, int blocks, int count, int prefetch, hipStream_t stream) {
+    switch (prefetch) {
+        case 2: hipLaunchKernelGGL((path2_kernel<K, 2, LANES>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
+        case 8: hipLaunchKernelGGL((path2_kernel<K, 8, LANES>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
+        default: hipLaunchKernelGGL((path2_kernel<K, 4, LANES>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
+    }
+}
+
 template <int K>
 static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &single,
                                           const PairDesc *table, int count, const Schedule &sch,
@@ -1390,24 +1406,26 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
     // take four lines a wave, eight a block); 4 or 8 waves (slices) at 8 paths only.
     const bool two = sch.waves >= 2;
     const int slices = two && p.paths == 8 ? slices_fit(16 * K, sch.waves / 2) : 1;
-    const int per_block = two ? 4 : PATH_BLOCK / (lanes_fit(16 * K, sch.lanes) ? sch.lanes : 16);
+    // Two waves per line take 16 lanes a line, or 32 where D / 32 is 2, 4 or 8.
+    const int two_lanes = two && slices == 1 && sch.lanes == 32 && two_lanes_fit(16 * K) ? 32 : 16;
+    const int per_block = two ? 64 / two_lanes
+                              : PATH_BLOCK / (lanes_fit(16 * K, sch.lanes) ? sch.lanes : 16);
     auto launch = [&](int first, int last) {
         a.first = first;
         a.last = last;
         int blocks = 0;
         for (int o = 0; o < 4; o++) {
             a.block_start[o] = blocks;
-            const int lines_per_block = two && o == 1 && !a.backward_columns ? 8 : per_block;
+            const int lines_per_block = two && o == 1 && !a.backward_columns ? 2 * per_block : per_block;
             if (o >= first && o <= last) blocks += (lines[o] + lines_per_block - 1) / lines_per_block;
         }
         if (two && slices > 1) {
             launch_sliced<16 * K>(a, blocks, count, sch.prefetch, slices, stream);
+        } else if (two && two_lanes == 32) {
+            if constexpr (K % 2 == 0 && (K / 2 == 2 || K / 2 == 4 || K / 2 == 8))
+                launch_two<K / 2, 32>(a, blocks, count, sch.prefetch, stream);
         } else if (two) {
-            switch (sch.prefetch) {
-                case 2: hipLaunchKernelGGL((path2_kernel<K, 2>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
-                case 8: hipLaunchKernelGGL((path2_kernel<K, 8>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
-                default: hipLaunchKernelGGL((path2_kernel<K, 4>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
-            }
+            launch_two<K, 16>(a, blocks, count, sch.prefetch, stream);
         } else {
             launch_paths_lanes<16 * K>(a, blocks, count, sch, stream);
         }

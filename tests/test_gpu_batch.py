@@ -67,7 +67,8 @@ def test_every_schedule_gives_the_same_disparity(disparities, paths):
     kept = _native.get_schedule(w, h, disparities, paths)
     try:
         shapes = [(lanes, 1) for lanes in LANES[disparities]] + \
-            [(16, w) for w in WAVES[disparities] if paths == 8 or w == 2]
+            [(16, w) for w in WAVES[disparities] if paths == 8 or w == 2] + \
+            ([(32, 2)] if disparities in (64, 128, 256) else [])
         for lanes, waves in shapes:
             for prefetch in (2, 4, 8):
                 for split in (False, True):
@@ -86,9 +87,9 @@ def test_every_schedule_gives_the_same_disparity(disparities, paths):
 
 @pytest.mark.parametrize("disparities,paths", [(64, 8), (128, 8), (64, 3), (128, 3)])
 def test_several_waves_on_odd_and_short_lines(disparities, paths):
-    """Two or more waves per line: lines shorter than the prefetch and LDS windows, odd and
-    even lengths (the middle pixel), every diagonal length from 1 up; at 3 paths, column
-    counts off the eight a block takes."""
+    """Two or more waves per line (16 lanes a line, and two waves of 32): lines shorter than
+    the prefetch and LDS windows, odd and even lengths (the middle pixel), every diagonal
+    length from 1 up; at 3 paths, column counts off the eight (or four) a block takes."""
     sizes = [(9, 7), (10, 8), (17, 40), (33, 17)]
     kept = {size: _native.get_schedule(*size, disparities, paths) for size in sizes}
     try:
@@ -96,12 +97,13 @@ def test_several_waves_on_odd_and_short_lines(disparities, paths):
             left, right = scenes.scene(w, h, 3)
             want = oracle.aggregate(oracle.costs(oracle.census(left), oracle.census(right),
                                                  disparities), paths, 8, 64)
-            for waves in WAVES[disparities] if paths == 8 else (2,):
+            shapes = [(16, waves) for waves in (WAVES[disparities] if paths == 8 else (2,))]
+            for lanes, waves in shapes + [(32, 2)]:
                 for prefetch in (2, 4, 8):
-                    _native.set_schedule(w, h, disparities, paths, prefetch, False, 16, waves)
+                    _native.set_schedule(w, h, disparities, paths, prefetch, False, lanes, waves)
                     sums = _native.match_host(left, right, disparities, paths,
                                               intermediates=True)["sums"]
-                    assert np.array_equal(sums, want), (w, h, waves, prefetch)
+                    assert np.array_equal(sums, want), (w, h, lanes, waves, prefetch)
     finally:
         for size, schedule in kept.items():
             _native.set_schedule(*size, disparities, paths, *schedule)
@@ -112,8 +114,10 @@ def test_lanes_that_do_not_fit_are_refused():
         _native.set_schedule(97, 31, 192, 8, 4, False, 32)
     with pytest.raises(_native.NativeError, match="do not fit"):
         _native.set_schedule(97, 31, 64, 8, 4, False, 64)
-    with pytest.raises(_native.NativeError, match="16 lanes"):
-        _native.set_schedule(97, 31, 128, 8, 4, False, 32, 2)
+    with pytest.raises(_native.NativeError, match="16 lanes per line, or 32"):
+        _native.set_schedule(97, 31, 128, 8, 4, False, 64, 2)
+    with pytest.raises(_native.NativeError, match="four or more waves per line take 16 lanes"):
+        _native.set_schedule(97, 31, 128, 8, 4, False, 32, 4)
     with pytest.raises(_native.NativeError, match="8 waves per line do not fit"):
         _native.set_schedule(97, 31, 64, 8, 4, False, 16, 8)
     with pytest.raises(_native.NativeError, match="4 waves per line take 8 paths"):
