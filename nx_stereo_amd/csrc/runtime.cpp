@@ -855,6 +855,7 @@ struct DeviceEntry {
 struct Enqueued {
     std::map<std::string, Slot> tensors;
     std::vector<const char *> filled;
+    int free_disparity = 0;  // which of the segment's two maps the next filter writes
 };
 
 // The fused per-pixel tail (launch_tail): SpeckleFilter of radius 3, TemporalFilter, Depth,
@@ -909,10 +910,11 @@ static void order_after(Segment &s, hipStream_t stream) {
 // a stage only reads are read in place), then each stage in order (segment.rs:206-239).
 // Host entries go through the slot's pinned upload buffer (or, without a slot, a
 // synchronous copy); `workspace` holds workspace_bytes for the matcher, if any.
-static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
-                        const std::vector<std::pair<std::string, HostTensor>> &entries,
-                        const std::vector<DeviceEntry> &device_entries, hipStream_t stream,
-                        RunSlot *run_slot, const std::vector<hipEvent_t> &events, void *workspace) {
+static Enqueued enqueue_entries(Segment &s, const std::vector<Inputs> &inputs,
+                                const std::vector<std::pair<std::string, HostTensor>> &entries,
+                                const std::vector<DeviceEntry> &device_entries,
+                                hipStream_t stream, RunSlot *run_slot,
+                                const std::vector<hipEvent_t> &events) {
     if (inputs.size() != s.members.size())
         throw Error{std::to_string(s.members.size()) + " stages, got the inputs of " +
                     std::to_string(inputs.size())};
@@ -923,7 +925,7 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
                                 frame = {(uint32_t)g.frame_height, (uint32_t)g.frame_width};
     Enqueued q;
     std::map<std::string, Slot> &tensors = q.tensors;
-    int free_disparity = 0;
+    int &free_disparity = q.free_disparity;
 
     // A frame or image packed as u32 words (segment.ex:207-241) holds the u8 image's bytes:
     // it is read as that image.
@@ -985,19 +987,63 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
         tensors[e.name] = slot;
     }
 
-    std::vector<const char *> &filled = q.filled;
     if (!events.empty()) HIP_TRY(hipEventRecord(events[0], stream));
-    auto note_outputs = [&](const Member &m) {
-        for (const char *name : member_outputs(m.kind)) {
-            bool seen = false;
-            for (const char *f : filled) seen = seen || std::strcmp(f, name) == 0;
-            if (!seen) filled.push_back(name);
-        }
-    };
-    for (size_t i = 0; i < s.members.size(); i++) {
+    return q;
+}
+
+// A full-search Disparity member's matching, as its step reads it (inputs checked, images
+// taken, the workspace carved, the output map chosen), and the tensors it leaves: the
+// pair's descriptor for the matcher, launched alone (enqueue_members) or with other
+// segments' pairs of its size class in one launch (the served form's coalescing).
+static PairDesc matcher_pair(Segment &s, const Member &m, const Inputs &in, Enqueued &q,
+                             void *workspace, MatchParams &mp) {
+    const uint32_t W = s.g.width, H = s.g.height;
+    const std::vector<uint32_t> image = {H, W};
+    const int p1 = penalty(in, "p1"), p2 = penalty(in, "p2");
+    const float u = scalar(in, "uniqueness");
+    check_matching(W, H, m.disparities, m.paths, p1, p2, u);
+    PairDesc pd{};
+    pd.left = (const uint8_t *)shaped(q.tensors, "left", TYPE_U8, image, "u8 image");
+    pd.right = (const uint8_t *)shaped(q.tensors, "right", TYPE_U8, image, "u8 image");
+    pd.disparity = s.disparity[q.free_disparity];
+    if (!workspace) throw Error{"no matcher workspace"};
+    match_carve(pd.b, workspace, W, H, m.disparities, m.paths);
+    pd.sums = nullptr;
+    pd.p1 = p1;
+    pd.p2 = p2;
+    pd.uniqueness = u;
+    mp = MatchParams{(int)W, (int)H, m.disparities, m.paths, p1, p2, u};
+    q.tensors["disparity"] = Slot{TYPE_F32, image, pd.disparity};
+    q.free_disparity ^= 1;
+    return pd;
+}
+
+static void note_outputs(Enqueued &q, const Member &m) {
+    for (const char *name : member_outputs(m.kind)) {
+        bool seen = false;
+        for (const char *f : q.filled) seen = seen || std::strcmp(f, name) == 0;
+        if (!seen) q.filled.push_back(name);
+    }
+}
+
+// Enqueues members [from, to) on `stream` (segment.rs:206-239), after enqueue_entries;
+// `workspace` holds workspace_bytes for the matcher, if any.
+static void enqueue_members(Segment &s, const std::vector<Inputs> &inputs, Enqueued &q,
+                            size_t from, size_t to, hipStream_t stream,
+                            const std::vector<hipEvent_t> &events, void *workspace) {
+    const Geometry &g = s.g;
+    const uint32_t W = g.width, H = g.height;
+    const size_t P = (size_t)W * H;
+    const std::vector<uint32_t> image = {H, W},
+                                frame = {(uint32_t)g.frame_height, (uint32_t)g.frame_width};
+    std::map<std::string, Slot> &tensors = q.tensors;
+    int &free_disparity = q.free_disparity;
+    auto note_outputs = [&](const Member &m) { nxs::note_outputs(q, m); };
+    for (size_t i = from; i < to; i++) {
         const Member &m = s.members[i];
         const Inputs &in = inputs[i];
-        if (const size_t run = s.fuse_tail ? tail_length(s.members, i) : 0; run >= 2) {
+        if (const size_t run = s.fuse_tail ? std::min(tail_length(s.members, i), to - i) : 0;
+            run >= 2) {
             // Members i .. i+run-1 in one launch. Each member's inputs are read in member
             // order, as its own step reads them; the first member's time is the launch's,
             // the last one's the finish's.
@@ -1122,14 +1168,13 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
                     BandParams bp = band_params(W, H, 128, m.band, m.paths, m.fine_census, p1, p2, u);
                     bp.positive = true;  // the stage's "> 0 else -1" (stage/disparity.ex:162)
                     HIP_TRY(launch_coarse_to_fine(bp, m.margin, m.coarse_paths, l, r, c, out, stream));
+                    tensors["disparity"] = Slot{TYPE_F32, image, out};
+                    free_disparity ^= 1;
                 } else {
-                    MatchBuffers b;
-                    match_carve(b, workspace, W, H, m.disparities, m.paths);
-                    MatchParams mp{(int)W, (int)H, m.disparities, m.paths, p1, p2, u};
-                    HIP_TRY(launch_match(mp, l, r, b, out, nullptr, stream));
+                    MatchParams mp;
+                    PairDesc pd = matcher_pair(s, m, in, q, workspace, mp);
+                    HIP_TRY(launch_match(mp, pd.left, pd.right, pd.b, pd.disparity, nullptr, stream));
                 }
-                tensors["disparity"] = Slot{TYPE_F32, image, out};
-                free_disparity ^= 1;
                 break;
             }
             case Kind::Sharpness: {
@@ -1205,7 +1250,18 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
         if (!events.empty()) HIP_TRY(hipEventRecord(events[i + 1], stream));
         note_outputs(m);
     }
+}
 
+// Enqueues every member on `stream`: entries go to the segment's buffers (device entries
+// a stage only reads are read in place), then each stage in order (segment.rs:206-239).
+// Host entries go through the slot's pinned upload buffer (or, without a slot, a
+// synchronous copy); `workspace` holds workspace_bytes for the matcher, if any.
+static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
+                        const std::vector<std::pair<std::string, HostTensor>> &entries,
+                        const std::vector<DeviceEntry> &device_entries, hipStream_t stream,
+                        RunSlot *run_slot, const std::vector<hipEvent_t> &events, void *workspace) {
+    Enqueued q = enqueue_entries(s, inputs, entries, device_entries, stream, run_slot, events);
+    enqueue_members(s, inputs, q, 0, s.members.size(), stream, events, workspace);
     return q;
 }
 
