@@ -448,7 +448,15 @@ def served_leg(duration_s: float = 1.5) -> dict:
         warm_records = [r[:warm] for r in records]
         timed = [r[warm:] for r in records]
         serving.serve(ctx, segs, warm_records, [[0.0] * warm for _ in specs], all_at_once=True)
+        before = ctx.coalescing()
         result["continuous"] = serving.serve(ctx, segs, timed, arrivals).summary()
+        after = ctx.coalescing()
+        result["continuous"]["coalesced"] = {"launches": after[0] - before[0],
+                                             "pairs": after[1] - before[1]}
+        # The same arrivals with every pair's matcher its own launch.
+        ctx.coalescing(1)
+        result["continuous_uncoalesced"] = serving.serve(ctx, segs, timed, arrivals).summary()
+        ctx.coalescing(8)
         result["one_at_a_time"] = serving.serve_one_at_a_time(ctx, segs, timed, arrivals).summary()
         # Capacity: every pair of the trace submitted at once.
         result["saturated"] = {

@@ -86,6 +86,19 @@ int nxs_set_hbm_budget(nxs_ctx *ctx, size_t bytes);
 /* The workspace bytes allocated and the runs in flight, for monitoring. */
 int nxs_stats(nxs_ctx *ctx, size_t *workspace_bytes, int *runs_in_flight);
 
+/* Coalescing: runs of other segments queued behind the run being started,
+ * whose first Disparity member searches the full range at the same size,
+ * disparities and paths, share its matcher launch (one descriptor table),
+ * up to `max_pairs` a launch (at most 16; 0 or 1 turns it off; negative
+ * leaves the setting). A run joins only when its run slot and workspace are
+ * free at once, never ahead of an earlier request naming its segment; its
+ * stages before and after the matcher stay on its segment's stream, and its
+ * results are those of a run alone. The default is 8, or NXS_COALESCE.
+ * `launches` and `pairs` (either may be NULL) receive how many coalesced
+ * launches ran and the runs in them. No counterpart in the reference (the
+ * Port computes one request at a time). */
+int nxs_coalescing(nxs_ctx *ctx, int max_pairs, int64_t *launches, int64_t *pairs);
+
 /* ---- The matcher on device buffers (the hot path, without records) ----
  *
  * Stage.Disparity's computation (stage/disparity.ex:135-163, full search;

@@ -19,7 +19,8 @@ NXS_OK, NXS_TIMEOUT, NXS_EINVAL, NXS_EDEVICE, NXS_ENOMEM, NXS_EUNKNOWN = 0, 1, -
 # Every symbol include/nxs_hip.h declares.
 SYMBOLS = (
     "nxs_open", "nxs_close", "nxs_submit", "nxs_wait", "nxs_reply", "nxs_release_reply",
-    "nxs_poll", "nxs_set_hbm_budget", "nxs_stats", "nxs_match_workspace_size", "nxs_match_device",
+    "nxs_poll", "nxs_set_hbm_budget", "nxs_stats", "nxs_coalescing", "nxs_match_workspace_size",
+    "nxs_match_device",
     "nxs_match_device_noalloc", "nxs_match_host",
     "nxs_band_workspace_size", "nxs_match_band_device", "nxs_match_band_host",
     "nxs_coarse_to_fine_workspace_size", "nxs_coarse_to_fine_device", "nxs_coarse_to_fine_host",
@@ -87,6 +88,8 @@ def load() -> ctypes.CDLL:
     lib.nxs_set_hbm_budget.restype = i
     lib.nxs_stats.argtypes = [P, ctypes.POINTER(sz), ctypes.POINTER(i)]
     lib.nxs_stats.restype = i
+    lib.nxs_coalescing.argtypes = [P, i, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+    lib.nxs_coalescing.restype = i
     lib.nxs_match_workspace_size.argtypes = [i, i, i, i]
     lib.nxs_match_workspace_size.restype = sz
     lib.nxs_match_device.argtypes = [P, P, i, i, i, i, i, i, f, P, P, sz, P]

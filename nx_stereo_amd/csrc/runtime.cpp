@@ -568,6 +568,9 @@ struct Segment {
     RunSlot slots[RUN_SLOTS];
     int next_slot = 0;
     size_t workspace_bytes = 0;  // of its largest matcher member, 0 without one
+    // The first Disparity member when it searches the full range, else -1: served runs of
+    // segments whose matchers are of one size class may share that member's launches.
+    int matcher = -1;
     // nxs_run_device's matcher workspace: the segment's own, outside the serving pool and
     // its budget, held until the segment is released (a run on a caller's stream may still
     // use it when nxs_run_device returns).
@@ -767,6 +770,11 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
         s->decode_cols = cols_taps.upload(taps + rows_taps.bytes());
     }
     s->workspace_bytes = match;
+    for (size_t i = 0; i < s->members.size(); i++)
+        if (s->members[i].kind == Kind::Disparity) {
+            s->matcher = s->members[i].coarse_to_fine ? -1 : (int)i;
+            break;
+        }
     if (const char *fuse = std::getenv("NXS_FUSE_TAIL")) s->fuse_tail = std::atoi(fuse) != 0;
     for (auto &m : s->members) {
         float *a = nullptr, *b = nullptr;
@@ -1151,30 +1159,27 @@ static void enqueue_members(Segment &s, const std::vector<Inputs> &inputs, Enque
                 break;
             }
             case Kind::Disparity: {
+                if (!m.coarse_to_fine) {
+                    MatchParams mp;
+                    PairDesc pd = matcher_pair(s, m, in, q, workspace, mp);
+                    HIP_TRY(launch_match(mp, pd.left, pd.right, pd.b, pd.disparity, nullptr, stream));
+                    break;
+                }
                 const int p1 = penalty(in, "p1"), p2 = penalty(in, "p2");
                 const float u = scalar(in, "uniqueness");
-                if (m.coarse_to_fine)
-                    check_coarse_to_fine(W, H, m.band, m.margin, m.paths, m.coarse_paths,
-                                         m.fine_census, p1, p2, u);
-                else
-                    check_matching(W, H, m.disparities, m.paths, p1, p2, u);
+                check_coarse_to_fine(W, H, m.band, m.margin, m.paths, m.coarse_paths,
+                                     m.fine_census, p1, p2, u);
                 auto *l = (const uint8_t *)shaped(tensors, "left", TYPE_U8, image, "u8 image");
                 auto *r = (const uint8_t *)shaped(tensors, "right", TYPE_U8, image, "u8 image");
                 float *out = s.disparity[free_disparity];
                 if (!workspace) throw Error{"no matcher workspace"};
-                if (m.coarse_to_fine) {
-                    CoarseBuffers c;
-                    coarse_to_fine_carve(c, workspace, W, H, m.band, m.paths, m.coarse_paths);
-                    BandParams bp = band_params(W, H, 128, m.band, m.paths, m.fine_census, p1, p2, u);
-                    bp.positive = true;  // the stage's "> 0 else -1" (stage/disparity.ex:162)
-                    HIP_TRY(launch_coarse_to_fine(bp, m.margin, m.coarse_paths, l, r, c, out, stream));
-                    tensors["disparity"] = Slot{TYPE_F32, image, out};
-                    free_disparity ^= 1;
-                } else {
-                    MatchParams mp;
-                    PairDesc pd = matcher_pair(s, m, in, q, workspace, mp);
-                    HIP_TRY(launch_match(mp, pd.left, pd.right, pd.b, pd.disparity, nullptr, stream));
-                }
+                CoarseBuffers c;
+                coarse_to_fine_carve(c, workspace, W, H, m.band, m.paths, m.coarse_paths);
+                BandParams bp = band_params(W, H, 128, m.band, m.paths, m.fine_census, p1, p2, u);
+                bp.positive = true;  // the stage's "> 0 else -1" (stage/disparity.ex:162)
+                HIP_TRY(launch_coarse_to_fine(bp, m.margin, m.coarse_paths, l, r, c, out, stream));
+                tensors["disparity"] = Slot{TYPE_F32, image, out};
+                free_disparity ^= 1;
                 break;
             }
             case Kind::Sharpness: {
@@ -1331,6 +1336,8 @@ struct Job {
     std::vector<uint8_t> record;
     nxs_done_fn done;
     void *user;
+    // The record parsed (its host tensors point into `record`), once the worker has read it.
+    std::unique_ptr<Request> parsed;
 };
 
 struct Kept {
@@ -1365,6 +1372,45 @@ struct nxs_ctx {
     bool stopping = false, worker_done = false;
     std::thread worker, completer;
 
+    // Coalescing: at most `coalesce_max` runs share one matcher launch (1: none), and the
+    // launches and runs so coalesced.
+    static constexpr int MAX_COALESCE = 16, GATHER_SCAN = 64, TABLES = 8;
+    int coalesce_max = 8;
+    int64_t coalesced_launches = 0, coalesced_pairs = 0;
+    // The coalesced launches' descriptor tables: pinned host staging and a device copy
+    // each, taken in turn, a table reused once the launch that read it has finished.
+    struct TableSlot {
+        nxs::PairDesc *host = nullptr, *dev = nullptr;
+        hipEvent_t done = nullptr;
+        bool used = false;
+    };
+    TableSlot tables[TABLES];
+    int next_table_index = 0;
+
+    TableSlot *next_table() {
+        using namespace nxs;
+        TableSlot &t = tables[next_table_index];
+        if (!t.host) {
+            PairDesc *host = nullptr, *dev = nullptr;
+            hipEvent_t done = nullptr;
+            hipError_t e = hipHostMalloc((void **)&host, MAX_COALESCE * sizeof(PairDesc));
+            if (e == hipSuccess) e = hipMalloc((void **)&dev, MAX_COALESCE * sizeof(PairDesc));
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+            if (e != hipSuccess) {
+                if (host) (void)hipHostFree(host);
+                if (dev) (void)hipFree(dev);
+                throw Error{std::string("HIP error: ") + hipGetErrorString(e)};
+            }
+            t.host = host;
+            t.dev = dev;
+            t.done = done;
+        } else if (t.used) {
+            HIP_TRY(hipEventSynchronize(t.done));
+        }
+        next_table_index = (next_table_index + 1) % TABLES;
+        return &t;
+    }
+
     void post(int64_t ticket, nxs_done_fn done, void *user, std::vector<uint8_t> reply) {
         if (done) {
             done(user, ticket, reply.data(), reply.size());
@@ -1377,8 +1423,10 @@ struct nxs_ctx {
         }
     }
 
-    // A free workspace of `bytes`, allocated within the budget, or waited for.
-    nxs::Workspace *acquire_workspace(size_t bytes, std::unique_lock<std::mutex> &lock) {
+    // A free workspace of `bytes`, allocated within the budget, or waited for (with `wait`
+    // false, nullptr where it would wait).
+    nxs::Workspace *acquire_workspace(size_t bytes, std::unique_lock<std::mutex> &lock,
+                                      bool wait = true) {
         using namespace nxs;
         for (;;) {
             for (auto &w : workspaces)
@@ -1427,94 +1475,323 @@ struct nxs_ctx {
             // Admission: a run in flight completes and frees memory. Every busy workspace
             // belongs to such a run (device runs keep workspaces of their own), so the wait
             // ends, unless the context closes first.
+            if (!wait) return nullptr;
             if (stopping) throw Error{"the context is closed"};
             cv_free.wait(lock);
         }
     }
 
-    // Enqueues a run; the reply comes from the completion thread.
-    void start_run(nxs::Request &q, int64_t ticket, nxs_done_fn done, void *user) {
+    // A run taken from the queue with what it holds: a run slot of its segment and, with a
+    // matcher, a workspace from the pool.
+    struct Admitted {
+        nxs::Job job;
+        std::shared_ptr<nxs::Segment> seg;
+        int slot = -1;
+        nxs::Workspace *ws = nullptr;
+    };
+    enum Admission { ADMITTED, REFUSED, BUSY };
+
+    // Takes a run slot of segment `id` and its matcher workspace for `a`, waiting for them;
+    // with `wait` false it takes them only if both are free now, else nothing (BUSY).
+    // Called with `lock` (on mu) held; a refusal's reason goes to `why`.
+    Admission admit(uint32_t id, Admitted &a, std::unique_lock<std::mutex> &lock, bool wait,
+                    std::string &why) {
         using namespace nxs;
-        std::shared_ptr<Segment> seg;
-        int slot_index = -1;
+        auto it = segments.find(id);
+        if (it == segments.end()) {
+            why = "segment " + std::to_string(id) + " is not configured";
+            return REFUSED;
+        }
+        std::shared_ptr<Segment> seg = it->second;
+        int slot = -1;
+        for (;;) {
+            for (int k = 0; k < RUN_SLOTS && slot < 0; k++) {
+                const int s = (seg->next_slot + k) % RUN_SLOTS;
+                if (!seg->slots[s].busy) slot = s;
+            }
+            if (slot >= 0) break;
+            if (!wait) return BUSY;
+            if (stopping) {
+                why = "the context is closed";
+                return REFUSED;
+            }
+            cv_free.wait(lock);
+        }
+        seg->slots[slot].busy = true;
         Workspace *ws = nullptr;
+        try {
+            if (seg->workspace_bytes) {
+                ws = acquire_workspace(seg->workspace_bytes, lock, wait);
+                if (!ws) {
+                    seg->slots[slot].busy = false;
+                    return BUSY;
+                }
+            }
+        } catch (const Error &e) {
+            why = e.reason;
+        } catch (const std::bad_alloc &) {
+            why = "out of host memory";
+        }
+        if (!why.empty()) {
+            seg->slots[slot].busy = false;
+            cv_free.notify_all();
+            return REFUSED;
+        }
+        seg->next_slot = (slot + 1) % RUN_SLOTS;
+        a.seg = std::move(seg);
+        a.slot = slot;
+        a.ws = ws;
+        return ADMITTED;
+    }
+
+    // Gives back what an admitted run holds.
+    void release(Admitted &a) {
+        std::lock_guard<std::mutex> lock(mu);
+        a.seg->slots[a.slot].busy = false;
+        if (a.ws) a.ws->busy = false;
+        cv_free.notify_all();
+    }
+
+    // Refuses an admitted run whose enqueueing failed: the work it already enqueued (on its
+    // segment's stream, and on `also` if not null) may still use its slot and workspace, so
+    // that finishes first.
+    void refuse_admitted(Admitted &a, const std::string &why, hipStream_t also = nullptr) {
+        (void)hipStreamSynchronize(a.seg->stream);
+        if (also) (void)hipStreamSynchronize(also);
+        release(a);
+        post(a.job.ticket, a.job.done, a.job.user, nxs::refused_record(why));
+    }
+
+    static std::string reason_of(std::exception_ptr e) {
+        try {
+            std::rethrow_exception(e);
+        } catch (const nxs::Error &x) {
+            return x.reason;
+        } catch (const std::bad_alloc &) {
+            return "out of host memory";
+        } catch (const std::exception &x) {
+            return std::string("internal error: ") + x.what();
+        } catch (...) {
+            return "internal error";
+        }
+    }
+
+    // Enqueues a run; the reply comes from the completion thread.
+    void start_run(nxs::Job &job) {
+        using namespace nxs;
+        std::vector<Admitted> group(1);
+        group[0].job = std::move(job);
+        std::string why;
+        Admission admitted;
         {
             std::unique_lock<std::mutex> lock(mu);
-            auto it = segments.find(q.segment);
-            if (it == segments.end()) {
-                lock.unlock();
-                post(ticket, done, user,
-                     refused_record("segment " + std::to_string(q.segment) + " is not configured"));
-                return;
-            }
-            seg = it->second;
-            // A free run slot of this segment (its pairs in flight are bounded).
-            for (;;) {
-                for (int k = 0; k < RUN_SLOTS && slot_index < 0; k++) {
-                    const int s = (seg->next_slot + k) % RUN_SLOTS;
-                    if (!seg->slots[s].busy) slot_index = s;
-                }
-                if (slot_index >= 0) break;
-                if (stopping) {
-                    lock.unlock();
-                    post(ticket, done, user, refused_record("the context is closed"));
-                    return;
-                }
-                cv_free.wait(lock);
-            }
-            seg->slots[slot_index].busy = true;
-            seg->next_slot = (slot_index + 1) % RUN_SLOTS;
-            std::string refused;
-            try {
-                if (seg->workspace_bytes) ws = acquire_workspace(seg->workspace_bytes, lock);
-            } catch (const Error &e) {
-                refused = e.reason;
-            } catch (const std::bad_alloc &) {
-                refused = "out of host memory";
-            }
-            if (!refused.empty()) {
-                seg->slots[slot_index].busy = false;
-                cv_free.notify_all();
-                lock.unlock();
-                post(ticket, done, user, refused_record(refused));
-                return;
-            }
+            admitted = admit(group[0].job.parsed->segment, group[0], lock, true, why);
         }
-        RunSlot &slot = seg->slots[slot_index];
-        std::string refused;
-        try {
-            std::lock_guard<std::mutex> guard(seg->mu);
+        if (admitted != ADMITTED) {
+            Job &j = group[0].job;
+            post(j.ticket, j.done, j.user, refused_record(why));
+            return;
+        }
+        if (coalesce_max > 1 && group[0].seg->matcher >= 0) gather(group);
+        if (group.size() == 1)
+            run_one(group[0]);
+        else
+            run_group(group);
+    }
+
+    void run_one(Admitted &a) {
+        using namespace nxs;
+        Segment &seg = *a.seg;
+        RunSlot &slot = seg.slots[a.slot];
+        const Request &q = *a.job.parsed;
+        std::exception_ptr failed;
+        {
+            std::lock_guard<std::mutex> guard(seg.mu);
             try {
-                order_before(*seg, seg->stream);
-                Enqueued e = enqueue(*seg, q.inputs, q.entries, {}, seg->stream, &slot,
-                                     slot.events, ws ? ws->ptr : nullptr);
-                std::vector<Output> outputs = enqueue_downloads(e, slot, seg->stream);
-                HIP_TRY(hipEventRecord(slot.events.back(), seg->stream));
-                order_after(*seg, seg->stream);
+                order_before(seg, seg.stream);
+                Enqueued e = enqueue(seg, q.inputs, q.entries, {}, seg.stream, &slot,
+                                     slot.events, a.ws ? a.ws->ptr : nullptr);
+                std::vector<Output> outputs = enqueue_downloads(e, slot, seg.stream);
+                HIP_TRY(hipEventRecord(slot.events.back(), seg.stream));
+                order_after(seg, seg.stream);
                 std::lock_guard<std::mutex> lock(mu);
-                pending.push_back(Pending{ticket, seg, slot_index, ws, std::move(outputs), done, user});
+                pending.push_back(Pending{a.job.ticket, a.seg, a.slot, a.ws, std::move(outputs),
+                                          a.job.done, a.job.user});
                 cv_pending.notify_all();
                 return;
             } catch (...) {
-                seg->last_stream = seg->stream;  // whatever was enqueued went there
-                throw;
+                seg.last_stream = seg.stream;  // whatever was enqueued went there
+                failed = std::current_exception();
             }
-        } catch (const Error &e) {
-            refused = e.reason;
-        } catch (const std::bad_alloc &) {
-            refused = "out of host memory";
-        } catch (const std::exception &e) {
-            refused = std::string("internal error: ") + e.what();
         }
-        // Work already enqueued may still use the slot and workspace: let it finish.
-        (void)hipStreamSynchronize(seg->stream);
+        refuse_admitted(a, reason_of(failed));
+    }
+
+    // Whether segment `seg` can join a group led by `lead`: its first Disparity member
+    // searches the full range at the lead's size, disparities and paths.
+    static bool same_matcher(const nxs::Segment &seg, const nxs::Segment &lead) {
+        if (seg.matcher < 0 || lead.matcher < 0) return false;
+        const nxs::Member &m = seg.members[seg.matcher], &l = lead.members[lead.matcher];
+        return seg.g.width == lead.g.width && seg.g.height == lead.g.height &&
+               m.disparities == l.disparities && m.paths == l.paths;
+    }
+
+    // Coalescing: runs of other segments queued behind the group's first, whose matchers
+    // are of its size class, join it while their run slot and workspace are free now (they
+    // never wait, so they hold nothing another run waits for). Jobs are taken in queue
+    // order and never ahead of an earlier job naming their own segment.
+    void gather(std::vector<Admitted> &group) {
+        using namespace nxs;
+        std::unique_lock<std::mutex> lock(mu);
+        std::vector<uint32_t> passed = {group[0].job.parsed->segment};
+        auto seen = [&](uint32_t id) {
+            return std::find(passed.begin(), passed.end(), id) != passed.end();
+        };
+        size_t scanned = 0;
+        for (auto it = jobs.begin();
+             it != jobs.end() && group.size() < (size_t)coalesce_max && scanned < GATHER_SCAN;
+             scanned++) {
+            Job &job = *it;
+            if (!job.parsed) {
+                try {
+                    job.parsed = std::make_unique<Request>(parse(job.record.data(), job.record.size()));
+                } catch (...) {
+                    ++it;  // refused in its turn; it names no segment
+                    continue;
+                }
+            }
+            const Request &q = *job.parsed;
+            bool take = false;
+            if (q.kind == RUN && !seen(q.segment)) {
+                auto s = segments.find(q.segment);
+                take = s != segments.end() && same_matcher(*s->second, *group[0].seg);
+            }
+            if (!seen(q.segment)) passed.push_back(q.segment);
+            Admitted a;
+            std::string why;
+            if (!take || admit(q.segment, a, lock, false, why) != ADMITTED) {
+                ++it;
+                continue;
+            }
+            a.job = std::move(job);
+            it = jobs.erase(it);
+            group.push_back(std::move(a));
+        }
+    }
+
+    // One group: each run's stages before its matcher on its own segment's stream, then the
+    // matchers as one launch over a descriptor table on the first run's stream once every
+    // run is ready for it, then each run's remaining stages and downloads on its own
+    // stream after that launch. A run whose enqueueing fails is refused alone.
+    void run_group(std::vector<Admitted> &group) {
+        using namespace nxs;
+        struct Part {
+            Enqueued q;
+            PairDesc pd{};
+            MatchParams mp{};
+            bool live = false;
+        };
+        std::vector<Part> parts(group.size());
+        std::vector<std::pair<size_t, std::string>> refused;
+        TableSlot *table = nullptr;
+        hipStream_t lead_stream = nullptr;
         {
-            std::lock_guard<std::mutex> lock(mu);
-            slot.busy = false;
-            if (ws) ws->busy = false;
-            cv_free.notify_all();
+            // Only this thread holds more than one segment lock at a time.
+            std::vector<std::unique_lock<std::mutex>> locks;
+            for (auto &a : group) locks.emplace_back(a.seg->mu);
+            for (size_t k = 0; k < group.size(); k++) {
+                Admitted &a = group[k];
+                Segment &seg = *a.seg;
+                RunSlot &slot = seg.slots[a.slot];
+                const Request &q = *a.job.parsed;
+                const size_t m = (size_t)seg.matcher;
+                try {
+                    order_before(seg, seg.stream);
+                    parts[k].q = enqueue_entries(seg, q.inputs, q.entries, {}, seg.stream, &slot,
+                                                 slot.events);
+                    enqueue_members(seg, q.inputs, parts[k].q, 0, m, seg.stream, slot.events,
+                                    a.ws ? a.ws->ptr : nullptr);
+                    parts[k].pd = matcher_pair(seg, seg.members[m], q.inputs[m], parts[k].q,
+                                               a.ws ? a.ws->ptr : nullptr, parts[k].mp);
+                    note_outputs(parts[k].q, seg.members[m]);
+                    parts[k].live = true;
+                } catch (...) {
+                    seg.last_stream = seg.stream;
+                    refused.emplace_back(k, reason_of(std::current_exception()));
+                }
+            }
+            // The matchers. A run's stages before its matcher end with its event m (the
+            // event after its uploads when the matcher is its first member).
+            std::vector<size_t> live;
+            for (size_t k = 0; k < group.size(); k++)
+                if (parts[k].live) live.push_back(k);
+            try {
+                if (live.size() == 1) {
+                    Part &p = parts[live[0]];
+                    HIP_TRY(launch_match(p.mp, p.pd.left, p.pd.right, p.pd.b, p.pd.disparity,
+                                         nullptr, group[live[0]].seg->stream));
+                } else if (live.size() > 1) {
+                    Admitted &lead = group[live[0]];
+                    lead_stream = lead.seg->stream;
+                    for (size_t j = 1; j < live.size(); j++) {
+                        const Admitted &a = group[live[j]];
+                        HIP_TRY(hipStreamWaitEvent(
+                            lead_stream, a.seg->slots[a.slot].events[a.seg->matcher], 0));
+                    }
+                    table = next_table();
+                    for (size_t j = 0; j < live.size(); j++) table->host[j] = parts[live[j]].pd;
+                    HIP_TRY(hipMemcpyAsync(table->dev, table->host, live.size() * sizeof(PairDesc),
+                                           hipMemcpyHostToDevice, lead_stream));
+                    const MatchParams &mp = parts[live[0]].mp;
+                    HIP_TRY(launch_match_batch(mp, table->dev, (int)live.size(),
+                                               schedule_for(mp.width, mp.height, mp.disparities,
+                                                            mp.paths),
+                                               lead_stream));
+                    HIP_TRY(hipEventRecord(table->done, lead_stream));
+                    table->used = true;
+                    std::lock_guard<std::mutex> lock(mu);
+                    coalesced_launches++;
+                    coalesced_pairs += (int64_t)live.size();
+                }
+            } catch (...) {
+                const std::string why = reason_of(std::current_exception());
+                for (size_t k : live) {
+                    parts[k].live = false;
+                    group[k].seg->last_stream = group[k].seg->stream;
+                    refused.emplace_back(k, why);
+                }
+            }
+            // Each run's remaining stages and downloads, after the matchers' launch.
+            for (size_t k : live) {
+                if (!parts[k].live) continue;
+                Admitted &a = group[k];
+                Segment &seg = *a.seg;
+                RunSlot &slot = seg.slots[a.slot];
+                const Request &q = *a.job.parsed;
+                const size_t m = (size_t)seg.matcher;
+                try {
+                    if (table && seg.stream != lead_stream)
+                        HIP_TRY(hipStreamWaitEvent(seg.stream, table->done, 0));
+                    HIP_TRY(hipEventRecord(slot.events[m + 1], seg.stream));
+                    enqueue_members(seg, q.inputs, parts[k].q, m + 1, seg.members.size(), seg.stream,
+                                    slot.events, a.ws ? a.ws->ptr : nullptr);
+                    std::vector<Output> outputs = enqueue_downloads(parts[k].q, slot, seg.stream);
+                    HIP_TRY(hipEventRecord(slot.events.back(), seg.stream));
+                    order_after(seg, seg.stream);
+                    std::lock_guard<std::mutex> lock(mu);
+                    pending.push_back(Pending{a.job.ticket, a.seg, a.slot, a.ws, std::move(outputs),
+                                              a.job.done, a.job.user});
+                    cv_pending.notify_all();
+                } catch (...) {
+                    parts[k].live = false;
+                    seg.last_stream = seg.stream;
+                    refused.emplace_back(k, reason_of(std::current_exception()));
+                }
+            }
         }
-        post(ticket, done, user, refused_record(refused));
+        // Refusals after the segment locks are given back (a reply callback may run a
+        // segment's device run).
+        for (auto &r : refused) refuse_admitted(group[r.first], r.second, lead_stream);
     }
 
     // Forgets a segment. Runs in flight keep it alive until they are answered; its own
@@ -1533,15 +1810,19 @@ struct nxs_ctx {
 
     void handle(nxs::Job &job) {
         using namespace nxs;
-        Request q;
         try {
-            q = parse(job.record.data(), job.record.size());
+            if (!job.parsed)
+                job.parsed = std::make_unique<Request>(parse(job.record.data(), job.record.size()));
         } catch (const Error &e) {
             post(job.ticket, job.done, job.user, refused_record(e.reason));
             return;
+        } catch (const std::bad_alloc &) {
+            post(job.ticket, job.done, job.user, refused_record("out of host memory"));
+            return;
         }
+        const Request &q = *job.parsed;
         if (q.kind == RUN) {
-            start_run(q, job.ticket, job.done, job.user);
+            start_run(job);
             return;
         }
         std::vector<uint8_t> reply;
@@ -1671,6 +1952,8 @@ int nxs_open(int device, nxs_ctx **out, char *err, size_t errlen) {
     if (hipMemGetInfo(&free_bytes, &total_bytes) != hipSuccess) free_bytes = (size_t)8 << 30;
     ctx->budget = free_bytes / 2;
     if (const char *mb = std::getenv("NXS_HBM_BUDGET_MB")) ctx->budget = (size_t)std::atoll(mb) << 20;
+    if (const char *c = std::getenv("NXS_COALESCE"))
+        ctx->coalesce_max = std::max(1, std::min(nxs_ctx::MAX_COALESCE, std::atoi(c)));
     ctx->worker = std::thread([ctx] { ctx->work(); });
     ctx->completer = std::thread([ctx] { ctx->complete(); });
     *out = ctx;
@@ -1693,6 +1976,18 @@ int nxs_stats(nxs_ctx *ctx, size_t *workspace_bytes, int *runs_in_flight) {
     return NXS_OK;
 }
 
+int nxs_coalescing(nxs_ctx *ctx, int max_pairs, int64_t *launches, int64_t *pairs) {
+    if (!ctx) return nxs::fail(NXS_EINVAL, "no context");
+    if (max_pairs > nxs_ctx::MAX_COALESCE)
+        return nxs::fail(NXS_EINVAL, "at most " + std::to_string(nxs_ctx::MAX_COALESCE) +
+                                         " pairs share a launch");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (max_pairs >= 0) ctx->coalesce_max = std::max(1, max_pairs);
+    if (launches) *launches = ctx->coalesced_launches;
+    if (pairs) *pairs = ctx->coalesced_pairs;
+    return NXS_OK;
+}
+
 void nxs_close(nxs_ctx *ctx) {
     if (!ctx) return;
     std::deque<nxs::Job> pending;
@@ -1712,6 +2007,14 @@ void nxs_close(nxs_ctx *ctx) {
     ctx->segments.clear();
     for (auto &w : ctx->workspaces) (void)hipFree(w->ptr);
     ctx->workspaces.clear();
+    for (auto &t : ctx->tables) {
+        if (t.done) {
+            (void)hipEventSynchronize(t.done);
+            (void)hipEventDestroy(t.done);
+        }
+        if (t.dev) (void)hipFree(t.dev);
+        if (t.host) (void)hipHostFree(t.host);
+    }
     delete ctx;
 }
 

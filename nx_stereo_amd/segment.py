@@ -80,6 +80,16 @@ class Context:
         self.lib.nxs_release_reply(self.handle, ticket)
         return data
 
+    def coalescing(self, max_pairs: int = -1):
+        """nxs_coalescing: sets how many served runs of one matcher size class may share a
+        launch (0 or 1: none; negative leaves it), and returns (launches, pairs) so far."""
+        launches, pairs = ctypes.c_int64(), ctypes.c_int64()
+        rc = self.lib.nxs_coalescing(self.handle, max_pairs, ctypes.byref(launches),
+                                     ctypes.byref(pairs))
+        if rc != _native.NXS_OK:
+            raise ValueError(_native.last_error())
+        return launches.value, pairs.value
+
     def request(self, record: bytes, timeout_ms: int = -1):
         ticket = self.submit(record)
         if not self.wait(ticket, timeout_ms):

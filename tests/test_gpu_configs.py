@@ -128,6 +128,8 @@ def test_24_served_streams_are_the_same_alone_in_the_mix_and_on_the_cpu():
         records = serving.run_records(segs, frames)
         arrivals = [np.zeros(FRAMES) for _ in specs]
         served = serving.serve(ctx, segs, records, arrivals, keep_replies=True, all_at_once=True)
+        launches, pairs = ctx.coalescing()
+        print(f"coalesced: {pairs} pairs in {launches} matcher launches", flush=True)
     mixed = [[None] * FRAMES for _ in specs]
     for s, k, reply in zip(served.stream, served.frame, served.replies):
         mixed[s][k] = decoded(reply)

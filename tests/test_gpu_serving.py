@@ -58,10 +58,12 @@ def alone(spec, fs):
         return out
 
 
-@pytest.mark.parametrize("budget", [None, 1])
-def test_each_stream_is_the_same_alone_and_in_the_mix(frames, budget):
-    """budget=1 byte: every matcher run waits for the one before to free the device memory."""
+@pytest.mark.parametrize("budget,coalesce", [(None, 8), (None, 1), (1, 8)])
+def test_each_stream_is_the_same_alone_and_in_the_mix(frames, budget, coalesce):
+    """budget=1 byte: every matcher run waits for the one before to free the device memory
+    (and no run joins another's launch). coalesce=1: every run its own matcher launch."""
     with segment.Context(0) as ctx:
+        ctx.coalescing(coalesce)
         if budget is not None:
             assert ctx.lib.nxs_set_hbm_budget(ctx.handle, budget) == _native.NXS_OK
         segs = serving.configure_streams(ctx, SPECS)
@@ -163,3 +165,82 @@ def test_runs_on_a_caller_stream_and_submitted_runs_keep_their_order():
     finally:
         stream.destroy()
         ctx.close()
+
+
+# A 1280x720/192 stream whose fourth pair waits for one of its three run slots: the
+# context's worker is held there while the pairs sent after it queue up.
+HOLDER = serving.StreamSpec(1280, 720, 192, 15.0, 21)
+# Three streams of one size class and one of another, interleaved in the queue.
+GROUPED = [serving.StreamSpec(97, 31, 64, 60.0, 31), serving.StreamSpec(160, 48, 128, 30.0, 32),
+           serving.StreamSpec(97, 31, 64, 60.0, 33), serving.StreamSpec(97, 31, 64, 60.0, 34)]
+
+
+def submit_behind_a_held_worker(ctx, holder, records):
+    """Four pairs of `holder`, then `records` (each (user, record)) in order; the holder's
+    fourth pair waits for a run slot while the records queue. Returns the replies by user."""
+    import ctypes
+    import threading
+
+    replies = {}
+    finished = threading.Semaphore(0)
+
+    @_native.DONE_FN
+    def on_done(user, ticket, reply, length):
+        replies[int(user or 0)] = ctypes.string_at(reply, length)
+        finished.release()
+
+    (left, right), = serving.stream_frames(HOLDER, 1)
+    sends = [(1000 + k, holder.run_record({"left": left, "right": right})) for k in range(4)]
+    sends += records
+    buffers = [(user, (ctypes.c_uint8 * len(r)).from_buffer_copy(r)) for user, r in sends]
+    for user, buf in buffers:
+        assert ctx.lib.nxs_submit(ctx.handle, buf, len(buf), on_done, ctypes.c_void_p(user)) > 0
+    for _ in buffers:
+        assert finished.acquire(timeout=120), "a pair did not complete"
+    return replies
+
+
+def test_queued_runs_of_one_size_class_share_a_matcher_launch():
+    """The three 97x31/64 streams' pairs queued behind the held worker share matcher
+    launches; every stream's replies are still the oracle's chain, in order."""
+    frames = [serving.stream_frames(spec, 2) for spec in GROUPED]
+    with segment.Context(0) as ctx:
+        ctx.coalescing(8)
+        holder, *segs = serving.configure_streams(ctx, [HOLDER] + GROUPED)
+        records = [(10 * s + k, segs[s].run_record({"left": l, "right": r}))
+                   for k in range(2) for s, (l, r) in enumerate(f[k] for f in frames)]
+        print("coalescing: 4 held pairs, then 8 pairs of two size classes", flush=True)
+        replies = submit_behind_a_held_worker(ctx, holder, records)
+        launches, pairs = ctx.coalescing()
+    assert launches >= 1 and pairs >= 2, (launches, pairs)
+    for s, spec in enumerate(GROUPED):
+        want = oracle_stream(spec, frames[s])
+        for k in range(2):
+            got = decoded(replies[10 * s + k])
+            assert same(got[0], want[k][0]) and same(got[1], want[k][1]), (s, k)
+
+
+def test_a_refused_run_leaves_the_rest_of_its_group_computed():
+    """One of the queued same-class pairs carries a penalty the matcher refuses: it alone is
+    refused, with the stage's message; the others of its launch equal the oracle."""
+    frames = [serving.stream_frames(spec, 1)[0] for spec in GROUPED]
+    with segment.Context(0) as ctx:
+        ctx.coalescing(8)
+        holder, *segs = serving.configure_streams(ctx, [HOLDER] + GROUPED)
+        bad = segs[2]
+        bad.inputs = [list(bad.inputs[0]), *bad.inputs[1:]]
+        bad.inputs[0] = [(n, np.array([8.5], np.float32)) if n == "p1" else (n, v)
+                         for n, v in bad.inputs[0]]
+        records = [(s, seg.run_record({"left": l, "right": r}))
+                   for s, (seg, (l, r)) in enumerate(zip(segs, frames))]
+        print("coalescing: a refused pair among 4 queued pairs", flush=True)
+        replies = submit_behind_a_held_worker(ctx, holder, records)
+        launches, pairs = ctx.coalescing()
+    assert launches >= 1 and pairs >= 2, (launches, pairs)
+    refused = protocol.decode(replies[2])
+    assert isinstance(refused, protocol.Refused), refused
+    assert refused.reason == "p1 must be an integer from 0 to 255, got 8.5"
+    for s in (0, 1, 3):
+        want = oracle_stream(GROUPED[s], [frames[s]])[0]
+        got = decoded(replies[s])
+        assert same(got[0], want[0]) and same(got[1], want[1]), s
