@@ -282,18 +282,19 @@ __device__ __forceinline__ Bytes<K> vload_policy(Rsrc r, uint32_t at) {
     return b;
 }
 
-template <int K>
+// A store with a cache policy (AUX 2: non-temporal, streaming).
+template <int K, int AUX = 0>
 __device__ __forceinline__ void vstore(Rsrc r, uint32_t at, const Bytes<K> &b) {
     if constexpr (K == 2) {
-        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)b.w[0], r, at, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)b.w[0], r, at, 0, AUX);
     } else if constexpr (K == 4) {
-        __builtin_amdgcn_raw_buffer_store_b32(b.w[0], r, at, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(b.w[0], r, at, 0, AUX);
     } else if constexpr (K == 8) {
-        __builtin_amdgcn_raw_buffer_store_b64((v2u){b.w[0], b.w[1]}, r, at, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64((v2u){b.w[0], b.w[1]}, r, at, 0, AUX);
     } else if constexpr (K == 12) {
-        __builtin_amdgcn_raw_buffer_store_b96((v3u){b.w[0], b.w[1], b.w[2]}, r, at, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b96((v3u){b.w[0], b.w[1], b.w[2]}, r, at, 0, AUX);
     } else {
-        __builtin_amdgcn_raw_buffer_store_b128((v4u){b.w[0], b.w[1], b.w[2], b.w[3]}, r, at, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128((v4u){b.w[0], b.w[1], b.w[2], b.w[3]}, r, at, 0, AUX);
     }
 }
 
@@ -644,9 +645,14 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
 constexpr int TWO_BLOCK = 128;
 
 // LANES: lanes per line, 16 (four lines a wave) or 32 (two lines a wave, K = D / 32: half
-// the instructions per step in each wave, twice the waves).
-template <int K, int PREFETCH, int LANES>
+// the instructions per step in each wave, twice the waves). STREAM: the launch's volumes
+// overflow the 256 MB Infinity Cache; phase C's pair sums (read next by select) and its
+// reads of the other's bytes (their last use) then go non-temporal, which leaves the cache
+// to the cost volume and the own bytes still to be read (at 1920x1080x256 path + select
+// 2262 -> 2170 us; where the volumes fit, at 728x544x128, it cost 10 us).
+template <int K, int PREFETCH, int LANES, bool STREAM>
 __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
+    constexpr int NT = STREAM && K >= 4 ? 2 : 0;
     // Steps after which a store has completed (the compiler's own wait for the loads of
     // PREFETCH steps ago already implies it), the window's half-width, and the ring's depth.
     constexpr int LAG = PREFETCH == 4 ? 5 : 6;
@@ -819,10 +825,11 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
             Bytes<K> out;
 #pragma unroll
             for (int k = 0; k < (K + 3) / 4; k++) out.w[k] = mine.w[k] + other[p].w[k];
-            vstore<K>(paired, at, out);
+            vstore<K, NT>(paired, at, out);
             __builtin_amdgcn_sched_barrier(0);
             cost[p] = vload<K>(costs, ahead);
-            other[p] = vload<K>(paired, ahead);
+            if constexpr (NT) other[p] = vload_policy<K, NT>(paired, ahead);
+            else other[p] = vload<K>(paired, ahead);
             at += step;
             ahead += step;
         }
@@ -1380,13 +1387,22 @@ bool two_lanes_fit(int disparities) {
     return disparities == 64 || disparities == 128 || disparities == 256;
 }
 
-template <int K, int LANES>
-static void launch_two(const PathArgs &a, int blocks, int count, int prefetch, hipStream_t stream) {
+template <int K, int LANES, bool STREAM>
+static void launch_two_as(const PathArgs &a, int blocks, int count, int prefetch, hipStream_t stream) {
     switch (prefetch) {
-        case 2: hipLaunchKernelGGL((path2_kernel<K, 2, LANES>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
-        case 8: hipLaunchKernelGGL((path2_kernel<K, 8, LANES>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
-        default: hipLaunchKernelGGL((path2_kernel<K, 4, LANES>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
+        case 2: hipLaunchKernelGGL((path2_kernel<K, 2, LANES, STREAM>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
+        case 8: hipLaunchKernelGGL((path2_kernel<K, 8, LANES, STREAM>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
+        default: hipLaunchKernelGGL((path2_kernel<K, 4, LANES, STREAM>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
     }
+}
+
+template <int K, int LANES>
+static void launch_two(const PathArgs &a, int blocks, int count, int prefetch, bool stream_volumes,
+                       hipStream_t stream) {
+    if (stream_volumes)
+        launch_two_as<K, LANES, true>(a, blocks, count, prefetch, stream);
+    else
+        launch_two_as<K, LANES, false>(a, blocks, count, prefetch, stream);
 }
 
 template <int K>
@@ -1410,6 +1426,8 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
     const int two_lanes = two && slices == 1 && sch.lanes == 32 && two_lanes_fit(16 * K) ? 32 : 16;
     const int per_block = two ? 64 / two_lanes
                               : PATH_BLOCK / (lanes_fit(16 * K, sch.lanes) ? sch.lanes : 16);
+    // Volumes the 256 MB Infinity Cache cannot hold are streamed where read for the last time.
+    const bool stream_volumes = (size_t)n * W * H * 16 * K * count > ((size_t)256 << 20);
     auto launch = [&](int first, int last) {
         a.first = first;
         a.last = last;
@@ -1423,9 +1441,9 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
             launch_sliced<16 * K>(a, blocks, count, sch.prefetch, slices, stream);
         } else if (two && two_lanes == 32) {
             if constexpr (K % 2 == 0 && (K / 2 == 2 || K / 2 == 4 || K / 2 == 8))
-                launch_two<K / 2, 32>(a, blocks, count, sch.prefetch, stream);
+                launch_two<K / 2, 32>(a, blocks, count, sch.prefetch, stream_volumes, stream);
         } else if (two) {
-            launch_two<K, 16>(a, blocks, count, sch.prefetch, stream);
+            launch_two<K, 16>(a, blocks, count, sch.prefetch, stream_volumes, stream);
         } else {
             launch_paths_lanes<16 * K>(a, blocks, count, sch, stream);
         }
@@ -1440,8 +1458,6 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
     slot = profile_begin(PK_SELECT, stream);
     const int select_pixels = 16 * SELECT_GROUPS;
     const dim3 select_grid((W + select_pixels - 1) / select_pixels, H, count);
-    // Volumes the 256 MB Infinity Cache cannot hold are streamed.
-    const bool stream_volumes = (size_t)n * W * H * 16 * K * count > ((size_t)256 << 20);
     if (stream_volumes)
         hipLaunchKernelGGL((select_kernel<K, true>), select_grid, dim3(256), 0, stream, single,
                            table, n, W);
