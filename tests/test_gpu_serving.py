@@ -244,3 +244,60 @@ def test_a_refused_run_leaves_the_rest_of_its_group_computed():
         want = oracle_stream(GROUPED[s], [frames[s]])[0]
         got = decoded(replies[s])
         assert same(got[0], want[0]) and same(got[1], want[1]), s
+
+
+def pipeline_members():
+    from nx_stereo_amd import stages
+
+    return [("rectification", stages.Rectification()),
+            ("disparity", stages.Disparity(num_disparities=64, paths=8)),
+            ("speckle_filter", stages.SpeckleFilter()), ("depth", stages.Depth()),
+            ("point_cloud", stages.PointCloud(decimation=4)), ("quality", stages.Quality())]
+
+
+def test_coalesced_pipelines_with_stages_before_the_matcher_equal_each_alone():
+    """Three Rectification -> Disparity -> SpeckleFilter -> Depth -> PointCloud -> Quality
+    segments of one size, two pairs each, queued behind the held worker: their matchers share
+    launches after each rectifies on its own stream, and every output of every pair (the
+    second against the first's carried quality) equals the segment's pairs run alone."""
+    from nx_stereo_amd import scenes
+    from rig import rig_geometry
+
+    geometry, cal = rig_geometry(182, 136)
+    frames = [[scenes.raw_frames(cal, 40 + 2 * s + k, high=56.0)[:2] for k in range(2)]
+              for s in range(3)]
+
+    def segments_of(ctx, count):
+        segs = []
+        for _ in range(count):
+            seg = segment.HipSegment(ctx, pipeline_members(), geometry)
+            assert ctx.request(seg.configure_record()) is True
+            seg.configured_version = seg.version
+            segs.append(seg)
+        return segs
+
+    with segment.Context(0) as ctx:
+        ctx.coalescing(8)
+        (holder,) = serving.configure_streams(ctx, [HOLDER])
+        segs = segments_of(ctx, 3)
+        records = [(10 * s + k, segs[s].run_record({"left_frame": lf, "right_frame": rf}))
+                   for k in range(2) for s, pairs in enumerate(frames) for lf, rf in [pairs[k]]]
+        print("coalescing: 3 pipelines x 2 pairs behind the held worker", flush=True)
+        replies = submit_behind_a_held_worker(ctx, holder, records)
+        launches, pairs = ctx.coalescing()
+    assert launches >= 1 and pairs >= 2, (launches, pairs)
+    for s in range(3):
+        with segment.Context(0) as ctx:
+            ctx.coalescing(1)
+            (seg,) = segments_of(ctx, 1)
+            for k, (lf, rf) in enumerate(frames[s]):
+                ticket = ctx.submit(seg.run_record({"left_frame": lf, "right_frame": rf}))
+                assert ctx.wait(ticket, 60000)
+                alone = protocol.decode(ctx.reply(ticket))
+                mixed = protocol.decode(replies[10 * s + k])
+                assert isinstance(mixed, protocol.Outputs), mixed
+                assert list(mixed.tensors) == list(alone.tensors), (s, k)
+                for name, want in alone.tensors.items():
+                    got, want = np.asarray(mixed.tensors[name]), np.asarray(want)
+                    assert got.shape == want.shape and got.dtype == want.dtype, (s, k, name)
+                    assert got.tobytes() == want.tobytes(), (s, k, name)
