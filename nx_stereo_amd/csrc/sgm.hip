@@ -68,26 +68,32 @@ constexpr int CENSUS_REACH = 4;
 // The LDS slot of right feature f (f = D + x - d, 0 <= f < D + W).
 __host__ __device__ constexpr int right_slot(int f) { return f + (f >> 4); }
 
-// LDS bytes of one block: the two census rows (the right one padded as right_slot says),
-// then 7 padded rows of each image.
+// The census rows in LDS: the left one for every pixel of the cost groups (4 pixels each),
+// the right one for features 0 .. D + W + 2 (the last group's reads past the row end: loaded,
+// used by nothing), each slot placed as right_slot says.
+__host__ __device__ constexpr int left_slots(int W) { return (W + 3) & ~3; }
+__host__ __device__ constexpr int right_slots(int W, int D) { return right_slot(D + W + 3) + 1; }
+
+// LDS bytes of one block: the two census rows, then 7 padded rows of each image.
 __host__ __device__ constexpr size_t census_cost_lds(int W, int D) {
-    return 8 * ((size_t)W + (size_t)right_slot(D + W) + 1) + 14 * (size_t)(W + 2 * CENSUS_REACH);
+    return 8 * ((size_t)left_slots(W) + (size_t)right_slots(W, D)) +
+           14 * (size_t)(W + 2 * CENSUS_REACH);
 }
 
 // Grid y: the row; z: the pair. keep_census: also store both census images. COST_BLOCK
 // threads: 512 where every row of the launch is in flight at once (four blocks a CU), else
 // 1024 (launch_all).
-template <int COST_BLOCK>
+template <int COST_BLOCK, int D>
 __global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single,
                                                                  const PairDesc *table, int W,
-                                                                 int H, int D, int keep_census) {
+                                                                 int H, int keep_census) {
     extern __shared__ __align__(16) uint8_t lds[];
     const PairDesc pd = pair_of(single, table, blockIdx.z);
     const int y = blockIdx.y;
-    uint64_t *cl = reinterpret_cast<uint64_t *>(lds);  // [W]
-    uint64_t *cr = cl + W;                             // [right_slot(D + W) + 1]
+    uint64_t *cl = reinterpret_cast<uint64_t *>(lds);  // [left_slots(W)]
+    uint64_t *cr = cl + left_slots(W);                 // [right_slots(W, D)]
     const int row_pitch = W + 2 * CENSUS_REACH;        // padded row
-    uint8_t *rows = reinterpret_cast<uint8_t *>(cr + right_slot(D + W) + 1);  // [2][7][row_pitch]
+    uint8_t *rows = reinterpret_cast<uint8_t *>(cr + right_slots(W, D));  // [2][7][row_pitch]
     // A thread loads its byte of all 14 rows before it stores any, so that the loads are in
     // flight together (a thread past the padded row loads a byte it does not store). Buffer
     // loads: the rows' offsets are scalars, the column the one vector offset.
@@ -129,7 +135,7 @@ __global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single
         if (keep_census) (side ? pd.b.census_right : pd.b.census_left)[(size_t)y * W + x] = f;
     }
     __syncthreads();
-    const int chunks = D >> 4;
+    constexpr int chunks = D >> 4;
     const int groups = (W + 3) >> 2;
     const Rsrc out = buffer_of(pd.b.costs + (size_t)y * W * D, (uint32_t)W * D);
     for (int q = threadIdx.x; q < groups * chunks; q += COST_BLOCK) {
@@ -137,12 +143,19 @@ __global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single
         const int x0 = 4 * g;
         // Pixel x0 + k at disparity 16c + j reads right feature f0 + k - j, f0 = D + x0 - 16c.
         const int f0 = D + x0 - 16 * c;
-        uint64_t r[19];  // r[i]: feature f0 - 15 + i (past the row's end: never used)
+        // r[i]: feature f0 - 15 + i (past the row's end: never used). f0 is 4 (g mod 4)
+        // past a multiple of 16, so the 19 features cross one multiple of 16, at i = t: their
+        // slots are right_slot(f0 - 15) + i, one more from i = t on. One base and a select
+        // of two addresses per read, each with its constant offset.
+        const int t = 15 - (f0 & 15);
+        const uint64_t *below = cr + right_slot(f0 - 15), *above = below + 1;
+        uint64_t r[19];
 #pragma unroll
-        for (int i = 0; i < 19; i++) r[i] = cr[right_slot(min(f0 - 15 + i, D + W - 1))];
+        for (int i = 0; i < 19; i++) r[i] = (i >= t ? above : below)[i];
+        const uint64_t *ls = cl + x0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const uint64_t l = cl[min(x0 + k, W - 1)];
+            const uint64_t l = ls[k];
             uint32_t words[4];
 #pragma unroll
             for (int w = 0; w < 4; w++) {
@@ -1468,19 +1481,40 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
     return hipGetLastError();
 }
 
+template <int COST_BLOCK>
+static void launch_census_cost(const PairDesc &single, const PairDesc *table, int W, int H, int D,
+                               int count, int keep_census, hipStream_t stream) {
+    const dim3 grid(1, H, count), block(COST_BLOCK);
+    const size_t lds = census_cost_lds(W, D);
+    switch (D) {
+        case 64: hipLaunchKernelGGL((census_cost_kernel<COST_BLOCK, 64>), grid, block, lds, stream, single, table, W, H, keep_census); break;
+        case 128: hipLaunchKernelGGL((census_cost_kernel<COST_BLOCK, 128>), grid, block, lds, stream, single, table, W, H, keep_census); break;
+        case 192: hipLaunchKernelGGL((census_cost_kernel<COST_BLOCK, 192>), grid, block, lds, stream, single, table, W, H, keep_census); break;
+        case 256: hipLaunchKernelGGL((census_cost_kernel<COST_BLOCK, 256>), grid, block, lds, stream, single, table, W, H, keep_census); break;
+        default: break;  // refused by the caller
+    }
+}
+
 static hipError_t launch_all(const MatchParams &p, const PairDesc &single, const PairDesc *table,
                              int count, const Schedule &sch, hipStream_t stream) {
     const int W = p.width, H = p.height, D = p.disparities;
     if (W > MAX_WIDTH || W < 1 || H < 1 || count < 1) return hipErrorInvalidValue;
     // Above 64 KB of LDS a kernel must say so first (up to the CU's 160 KB).
     static const hipError_t lds_ok = [] {
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(census_cost_kernel<512>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)census_cost_lds(MAX_WIDTH, MAX_D));
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void *>(census_cost_kernel<1024>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)census_cost_lds(MAX_WIDTH, MAX_D));
+        const void *kernels[] = {
+            reinterpret_cast<const void *>(census_cost_kernel<512, 64>),
+            reinterpret_cast<const void *>(census_cost_kernel<512, 128>),
+            reinterpret_cast<const void *>(census_cost_kernel<512, 192>),
+            reinterpret_cast<const void *>(census_cost_kernel<512, 256>),
+            reinterpret_cast<const void *>(census_cost_kernel<1024, 64>),
+            reinterpret_cast<const void *>(census_cost_kernel<1024, 128>),
+            reinterpret_cast<const void *>(census_cost_kernel<1024, 192>),
+            reinterpret_cast<const void *>(census_cost_kernel<1024, 256>)};
+        hipError_t e = hipSuccess;
+        for (const void *k : kernels)
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)census_cost_lds(MAX_WIDTH, MAX_D));
         return e;
     }();
     if (lds_ok != hipSuccess) return lds_ok;
@@ -1498,13 +1532,9 @@ static hipError_t launch_all(const MatchParams &p, const PairDesc &single, const
     const bool one_sweep = (size_t)H * count <= 4 * (size_t)compute_units;
     int slot = profile_begin(PK_COST, stream);
     if (one_sweep)
-        hipLaunchKernelGGL(census_cost_kernel<512>, dim3(1, H, count), dim3(512),
-                           census_cost_lds(W, D), stream, single, table, W, H, D,
-                           p.keep_stages ? 1 : 0);
+        launch_census_cost<512>(single, table, W, H, D, count, p.keep_stages ? 1 : 0, stream);
     else
-        hipLaunchKernelGGL(census_cost_kernel<1024>, dim3(1, H, count), dim3(1024),
-                           census_cost_lds(W, D), stream, single, table, W, H, D,
-                           p.keep_stages ? 1 : 0);
+        launch_census_cost<1024>(single, table, W, H, D, count, p.keep_stages ? 1 : 0, stream);
     profile_end(slot, stream);
     hipError_t e;
     switch (D) {
