@@ -545,6 +545,11 @@ struct Slot {
 struct RunSlot {
     uint8_t *upload = nullptr, *download = nullptr;
     size_t upload_bytes = 0, download_bytes = 0;
+    // Device staging of the same sizes: entries land there on the segment's upload stream
+    // and outputs leave from there on its download stream, so that a pair's transfers
+    // overlap the next pair's kernels on the segment's stream.
+    uint8_t *device_upload = nullptr, *device_download = nullptr;
+    hipEvent_t uploaded = nullptr, computed = nullptr, done = nullptr;
     std::vector<hipEvent_t> events;  // before the first member, then after each
     bool busy = false;
 };
@@ -563,6 +568,7 @@ struct Segment {
     Geometry g{};
     std::vector<Member> members;
     hipStream_t stream = nullptr;
+    hipStream_t upload_stream = nullptr, download_stream = nullptr;  // submitted runs' transfers
     void *arena = nullptr;
     std::mutex mu;  // held while work is enqueued on the segment
     RunSlot slots[RUN_SLOTS];
@@ -599,17 +605,25 @@ struct Segment {
     std::vector<float *> carry_a, carry_b;  // per member: temporal previous/missed, quality previous
 
     ~Segment() {
+        if (upload_stream) (void)hipStreamSynchronize(upload_stream);
         if (stream) (void)hipStreamSynchronize(stream);
+        if (download_stream) (void)hipStreamSynchronize(download_stream);
         if (foreign && order) (void)hipEventSynchronize(order);  // the last caller-stream run
         if (order) (void)hipEventDestroy(order);
         if (device_workspace) (void)hipFree(device_workspace);
         for (auto &slot : slots) {
             for (auto e : slot.events) (void)hipEventDestroy(e);
+            for (auto e : {slot.uploaded, slot.computed, slot.done})
+                if (e) (void)hipEventDestroy(e);
             if (slot.upload) (void)hipHostFree(slot.upload);
             if (slot.download) (void)hipHostFree(slot.download);
+            if (slot.device_upload) (void)hipFree(slot.device_upload);
+            if (slot.device_download) (void)hipFree(slot.device_download);
         }
         if (arena) (void)hipFree(arena);
         if (stream) (void)hipStreamDestroy(stream);
+        if (upload_stream) (void)hipStreamDestroy(upload_stream);
+        if (download_stream) (void)hipStreamDestroy(download_stream);
     }
 };
 
@@ -724,6 +738,8 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
                    up256(sharp) + up256(32) + up256(align) + up256(4 * align_sums) +
                    up256(4 * P) * carries + up256(taps_bytes) + 256;
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&s->upload_stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&s->download_stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&s->order, hipEventDisableTiming | hipEventDisableSystemFence));
     HIP_TRY(hipMalloc(&s->arena, bytes));
     uint8_t *p = static_cast<uint8_t *>(s->arena);
@@ -791,21 +807,24 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
         s->carry_b.push_back(b);
     }
     // Pinned staging per pair in flight. Up: the frames, images and maps an entry may bring.
-    // Down: left and right (u8), disparity (f32), depth_mm (u16), the cloud and two small
-    // tensors, each span rounded up to 16 bytes.
+    // Down: the reply record: left and right (u8), disparity (f32), depth_mm (u16), the
+    // cloud and two small tensors, with the record's and each tensor's header.
     for (auto &slot : s->slots) {
         slot.upload_bytes = 2 * F + 2 * P + 8 * P + 256;
-        slot.download_bytes = 2 * P + 4 * P + 2 * P + 4 * cloud + 4 * align_sums + 256;
+        slot.download_bytes = 2 * P + 4 * P + 2 * P + 4 * cloud + 4 * align_sums + 4096;
         HIP_TRY(hipHostMalloc((void **)&slot.upload, slot.upload_bytes));
         HIP_TRY(hipHostMalloc((void **)&slot.download, slot.download_bytes));
+        HIP_TRY(hipMalloc((void **)&slot.device_upload, slot.upload_bytes));
+        HIP_TRY(hipMalloc((void **)&slot.device_download, slot.download_bytes));
         // The events between members only time them: no system-scope fence (each one would
-        // write back and invalidate the caches between the members' kernels). The last one
-        // also tells the host the run is done, downloads included: it keeps the fence.
+        // write back and invalidate the caches between the members' kernels). `done`, after
+        // the downloads, tells the host the run is done: it keeps the fence.
         slot.events.resize(s->members.size() + 1);
         for (size_t k = 0; k < slot.events.size(); k++)
-            HIP_TRY(hipEventCreateWithFlags(&slot.events[k], k + 1 < slot.events.size()
-                                                                  ? hipEventDisableSystemFence
-                                                                  : hipEventDefault));
+            HIP_TRY(hipEventCreateWithFlags(&slot.events[k], hipEventDisableSystemFence));
+        HIP_TRY(hipEventCreateWithFlags(&slot.uploaded, hipEventDisableTiming | hipEventDisableSystemFence));
+        HIP_TRY(hipEventCreateWithFlags(&slot.computed, hipEventDisableTiming | hipEventDisableSystemFence));
+        HIP_TRY(hipEventCreateWithFlags(&slot.done, hipEventDisableTiming));
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     return s;
@@ -959,7 +978,14 @@ static Enqueued enqueue_entries(Segment &s, const std::vector<Inputs> &inputs,
         if (name == "depth_m" && type == TYPE_F32 && shape == image) return s.depth_m;
         return nullptr;
     };
+    // Host entries through the run slot's staging: pinned, then its device staging on the
+    // upload stream, then (once that is done) into place on the run's stream.
     size_t staged = 0;
+    struct Landing {
+        void *target;
+        size_t from, size;
+    };
+    std::vector<Landing> landings;
     for (auto &e : entries) {
         const HostTensor &t = e.second;
         Slot slot{t.type, t.shape, nullptr};
@@ -967,8 +993,9 @@ static Enqueued enqueue_entries(Segment &s, const std::vector<Inputs> &inputs,
         if (void *target = target_of(e.first, slot.type, slot.shape)) {
             if (run_slot && staged + t.size <= run_slot->upload_bytes) {
                 std::memcpy(run_slot->upload + staged, t.bytes, t.size);
-                HIP_TRY(hipMemcpyAsync(target, run_slot->upload + staged, t.size,
-                                       hipMemcpyHostToDevice, stream));
+                HIP_TRY(hipMemcpyAsync(run_slot->device_upload + staged, run_slot->upload + staged,
+                                       t.size, hipMemcpyHostToDevice, s.upload_stream));
+                landings.push_back(Landing{target, staged, t.size});
                 staged += (t.size + 255) & ~(size_t)255;
             } else {
                 HIP_TRY(hipStreamSynchronize(stream));
@@ -978,6 +1005,13 @@ static Enqueued enqueue_entries(Segment &s, const std::vector<Inputs> &inputs,
             if (e.first == "disparity") free_disparity = 1;
         }
         tensors[e.first] = slot;
+    }
+    if (!landings.empty()) {
+        HIP_TRY(hipEventRecord(run_slot->uploaded, s.upload_stream));
+        HIP_TRY(hipStreamWaitEvent(stream, run_slot->uploaded, 0));
+        for (const Landing &l : landings)
+            HIP_TRY(hipMemcpyAsync(l.target, run_slot->device_upload + l.from, l.size,
+                                   hipMemcpyDeviceToDevice, stream));
     }
     for (auto &e : device_entries) {
         Slot slot{e.type, e.shape, nullptr};
@@ -1270,57 +1304,60 @@ static Enqueued enqueue(Segment &s, const std::vector<Inputs> &inputs,
     return q;
 }
 
-// One filled tensor in a run slot's download staging.
-struct Output {
-    const char *name;
-    uint8_t type;
-    std::vector<uint32_t> shape;
-    size_t offset, bytes;
-};
-
-// Enqueues the downloads of every filled tensor into the slot's staging, in first-filled
-// order (segment.rs:220-234), and returns where each lands.
-static std::vector<Output> enqueue_downloads(Enqueued &q, RunSlot &slot, hipStream_t stream) {
-    std::vector<Output> outputs;
-    size_t offset = 0;
+// Lays out a run's reply record in its slot's download staging (protocol.rs:182-202): tag 8,
+// the member count, each member's GPU time in us (written when the run completes), the
+// output count, then each filled tensor in first-filled order (segment.rs:220-234), its
+// name, type and shape written now and its bytes downloaded straight after them. The reply
+// is then the staging itself, with nothing copied on the host. Returns the record's length.
+static size_t enqueue_downloads(Segment &s, Enqueued &q, RunSlot &slot, size_t members,
+                                hipStream_t stream) {
+    size_t at = 3 + 4 * members;
+    if (at > slot.download_bytes || q.filled.size() > 255) throw Error{"output staging too small"};
+    slot.download[0] = TAG_OUTPUTS;
+    slot.download[1] = (uint8_t)members;
+    slot.download[at - 1] = (uint8_t)q.filled.size();
+    std::vector<std::pair<size_t, size_t>> spans;  // each tensor's bytes: offset, length
     for (const char *name : q.filled) {
         const Slot &t = q.tensors[name];
         size_t elements = 1;
         for (uint32_t d : t.shape) elements *= d;
         const size_t bytes = elements * element_bytes(t.type);
-        if (offset + bytes > slot.download_bytes) throw Error{"output staging too small"};
-        HIP_TRY(hipMemcpyAsync(slot.download + offset, t.dev, bytes, hipMemcpyDeviceToHost, stream));
-        outputs.push_back(Output{name, t.type, t.shape, offset, bytes});
-        offset += (bytes + 15) & ~(size_t)15;
+        const size_t length = std::strlen(name);
+        const size_t header = 3 + length + 4 * t.shape.size();
+        if (at + header + bytes > slot.download_bytes) throw Error{"output staging too small"};
+        uint8_t *h = slot.download + at;
+        *h++ = (uint8_t)length;
+        std::memcpy(h, name, length);
+        h += length;
+        *h++ = t.type;
+        *h++ = (uint8_t)t.shape.size();
+        for (uint32_t d : t.shape)
+            for (int b = 0; b < 4; b++) *h++ = (uint8_t)(d >> (8 * b));
+        at += header;
+        spans.push_back({at, bytes});
+        HIP_TRY(hipMemcpyAsync(slot.device_download + at, t.dev, bytes, hipMemcpyDeviceToDevice,
+                               stream));
+        at += bytes;
     }
-    return outputs;
+    // The next run's kernels may now overwrite the segment's buffers: the bytes leave from
+    // the slot's device staging on the download stream.
+    HIP_TRY(hipEventRecord(slot.computed, stream));
+    HIP_TRY(hipStreamWaitEvent(s.download_stream, slot.computed, 0));
+    for (const auto &span : spans)
+        HIP_TRY(hipMemcpyAsync(slot.download + span.first, slot.device_download + span.first,
+                               span.second, hipMemcpyDeviceToHost, s.download_stream));
+    HIP_TRY(hipEventRecord(slot.done, s.download_stream));
+    return at;
 }
 
-// The reply of a completed run: tag 8, each member's GPU time in us, then the tensors
-// (protocol.rs:182-202).
-static std::vector<uint8_t> outputs_record(const RunSlot &slot, size_t members,
-                                           const std::vector<Output> &outputs) {
-    std::vector<uint8_t> reply;
-    reply.push_back(TAG_OUTPUTS);
-    reply.push_back((uint8_t)members);
+// The member times of a completed run, into its reply record.
+static void finish_record(RunSlot &slot, size_t members) {
     for (size_t i = 0; i < members; i++) {
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, slot.events[i], slot.events[i + 1]));
         const uint32_t us = (uint32_t)std::min(4294967295.0, std::floor((double)ms * 1000.0));
-        for (int k = 0; k < 4; k++) reply.push_back((uint8_t)(us >> (8 * k)));
+        for (int k = 0; k < 4; k++) slot.download[2 + 4 * i + k] = (uint8_t)(us >> (8 * k));
     }
-    reply.push_back((uint8_t)outputs.size());
-    for (const Output &o : outputs) {
-        const size_t length = std::strlen(o.name);
-        reply.push_back((uint8_t)length);
-        reply.insert(reply.end(), o.name, o.name + length);
-        reply.push_back(o.type);
-        reply.push_back((uint8_t)o.shape.size());
-        for (uint32_t d : o.shape)
-            for (int b = 0; b < 4; b++) reply.push_back((uint8_t)(d >> (8 * b)));
-        reply.insert(reply.end(), slot.download + o.offset, slot.download + o.offset + o.bytes);
-    }
-    return reply;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1351,7 +1388,7 @@ struct Pending {
     std::shared_ptr<Segment> segment;
     int slot;
     Workspace *workspace;
-    std::vector<Output> outputs;
+    size_t length;  // of the reply record in the slot's download staging
     nxs_done_fn done;
     void *user;
 };
@@ -1409,6 +1446,23 @@ struct nxs_ctx {
         }
         next_table_index = (next_table_index + 1) % TABLES;
         return &t;
+    }
+
+    // A reply that lives in a run slot's staging: handed to the callback as it is, or
+    // copied for nxs_reply.
+    void post_bytes(int64_t ticket, nxs_done_fn done, void *user, const uint8_t *reply,
+                    size_t length) {
+        if (done) {
+            done(user, ticket, reply, length);
+            return;
+        }
+        std::vector<uint8_t> copy;
+        try {
+            copy.assign(reply, reply + length);
+        } catch (const std::bad_alloc &) {
+            copy = nxs::refused_record("out of host memory");
+        }
+        post(ticket, done, user, std::move(copy));
     }
 
     void post(int64_t ticket, nxs_done_fn done, void *user, std::vector<uint8_t> reply) {
@@ -1556,7 +1610,9 @@ struct nxs_ctx {
     // segment's stream, and on `also` if not null) may still use its slot and workspace, so
     // that finishes first.
     void refuse_admitted(Admitted &a, const std::string &why, hipStream_t also = nullptr) {
+        (void)hipStreamSynchronize(a.seg->upload_stream);
         (void)hipStreamSynchronize(a.seg->stream);
+        (void)hipStreamSynchronize(a.seg->download_stream);
         if (also) (void)hipStreamSynchronize(also);
         release(a);
         post(a.job.ticket, a.job.done, a.job.user, nxs::refused_record(why));
@@ -1611,12 +1667,12 @@ struct nxs_ctx {
                 order_before(seg, seg.stream);
                 Enqueued e = enqueue(seg, q.inputs, q.entries, {}, seg.stream, &slot,
                                      slot.events, a.ws ? a.ws->ptr : nullptr);
-                std::vector<Output> outputs = enqueue_downloads(e, slot, seg.stream);
-                HIP_TRY(hipEventRecord(slot.events.back(), seg.stream));
+                const size_t length =
+                    enqueue_downloads(seg, e, slot, seg.members.size(), seg.stream);
                 order_after(seg, seg.stream);
                 std::lock_guard<std::mutex> lock(mu);
-                pending.push_back(Pending{a.job.ticket, a.seg, a.slot, a.ws, std::move(outputs),
-                                          a.job.done, a.job.user});
+                pending.push_back(Pending{a.job.ticket, a.seg, a.slot, a.ws, length, a.job.done,
+                                          a.job.user});
                 cv_pending.notify_all();
                 return;
             } catch (...) {
@@ -1775,12 +1831,12 @@ struct nxs_ctx {
                     HIP_TRY(hipEventRecord(slot.events[m + 1], seg.stream));
                     enqueue_members(seg, q.inputs, parts[k].q, m + 1, seg.members.size(), seg.stream,
                                     slot.events, a.ws ? a.ws->ptr : nullptr);
-                    std::vector<Output> outputs = enqueue_downloads(parts[k].q, slot, seg.stream);
-                    HIP_TRY(hipEventRecord(slot.events.back(), seg.stream));
+                    const size_t length =
+                        enqueue_downloads(seg, parts[k].q, slot, seg.members.size(), seg.stream);
                     order_after(seg, seg.stream);
                     std::lock_guard<std::mutex> lock(mu);
-                    pending.push_back(Pending{a.job.ticket, a.seg, a.slot, a.ws, std::move(outputs),
-                                              a.job.done, a.job.user});
+                    pending.push_back(Pending{a.job.ticket, a.seg, a.slot, a.ws, length, a.job.done,
+                                              a.job.user});
                     cv_pending.notify_all();
                 } catch (...) {
                     parts[k].live = false;
@@ -1881,7 +1937,7 @@ struct nxs_ctx {
                 {
                     std::lock_guard<std::mutex> lock(mu);
                     for (auto it = pending.begin(); it != pending.end(); ++it) {
-                        status = hipEventQuery(it->segment->slots[it->slot].events.back());
+                        status = hipEventQuery(it->segment->slots[it->slot].done);
                         if (status != hipErrorNotReady) {
                             p = std::move(*it);
                             pending.erase(it);
@@ -1892,26 +1948,30 @@ struct nxs_ctx {
                 }
                 if (!found) break;
                 progressed = true;
-                std::vector<uint8_t> reply;
+                std::vector<uint8_t> refusal;
                 RunSlot &slot = p.segment->slots[p.slot];
                 try {
                     if (status != hipSuccess)
                         throw Error{std::string("HIP error: ") + hipGetErrorString(status)};
-                    reply = outputs_record(slot, p.segment->members.size(), p.outputs);
+                    finish_record(slot, p.segment->members.size());
                 } catch (const Error &e) {
-                    reply = refused_record(e.reason);
+                    refusal = refused_record(e.reason);
                 } catch (const std::bad_alloc &) {
-                    reply = refused_record("out of host memory");
+                    refusal = refused_record("out of host memory");
                 } catch (const std::exception &e) {
-                    reply = refused_record(std::string("internal error: ") + e.what());
+                    refusal = refused_record(std::string("internal error: ") + e.what());
                 }
+                // The reply is the slot's staging: the slot is given back after it is posted.
+                if (refusal.empty())
+                    post_bytes(p.ticket, p.done, p.user, slot.download, p.length);
+                else
+                    post(p.ticket, p.done, p.user, std::move(refusal));
                 {
                     std::lock_guard<std::mutex> lock(mu);
                     slot.busy = false;
                     if (p.workspace) p.workspace->busy = false;
                     cv_free.notify_all();
                 }
-                post(p.ticket, p.done, p.user, std::move(reply));
             }
             if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
