@@ -738,8 +738,12 @@ static std::shared_ptr<Segment> configure(uint32_t id, const Geometry &g,
                    up256(sharp) + up256(32) + up256(align) + up256(4 * align_sums) +
                    up256(4 * P) * carries + up256(taps_bytes) + 256;
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-    HIP_TRY(hipStreamCreateWithFlags(&s->upload_stream, hipStreamNonBlocking));
-    HIP_TRY(hipStreamCreateWithFlags(&s->download_stream, hipStreamNonBlocking));
+    // NXS_TRANSFER_STREAMS=0 keeps a submitted run's transfers on the segment's stream.
+    const char *transfer = std::getenv("NXS_TRANSFER_STREAMS");
+    if (!transfer || std::atoi(transfer) != 0) {
+        HIP_TRY(hipStreamCreateWithFlags(&s->upload_stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&s->download_stream, hipStreamNonBlocking));
+    }
     HIP_TRY(hipEventCreateWithFlags(&s->order, hipEventDisableTiming | hipEventDisableSystemFence));
     HIP_TRY(hipMalloc(&s->arena, bytes));
     uint8_t *p = static_cast<uint8_t *>(s->arena);
@@ -980,6 +984,7 @@ static Enqueued enqueue_entries(Segment &s, const std::vector<Inputs> &inputs,
     };
     // Host entries through the run slot's staging: pinned, then its device staging on the
     // upload stream, then (once that is done) into place on the run's stream.
+    hipStream_t up = s.upload_stream ? s.upload_stream : stream;
     size_t staged = 0;
     struct Landing {
         void *target;
@@ -994,7 +999,7 @@ static Enqueued enqueue_entries(Segment &s, const std::vector<Inputs> &inputs,
             if (run_slot && staged + t.size <= run_slot->upload_bytes) {
                 std::memcpy(run_slot->upload + staged, t.bytes, t.size);
                 HIP_TRY(hipMemcpyAsync(run_slot->device_upload + staged, run_slot->upload + staged,
-                                       t.size, hipMemcpyHostToDevice, s.upload_stream));
+                                       t.size, hipMemcpyHostToDevice, up));
                 landings.push_back(Landing{target, staged, t.size});
                 staged += (t.size + 255) & ~(size_t)255;
             } else {
@@ -1006,9 +1011,11 @@ static Enqueued enqueue_entries(Segment &s, const std::vector<Inputs> &inputs,
         }
         tensors[e.first] = slot;
     }
-    if (!landings.empty()) {
-        HIP_TRY(hipEventRecord(run_slot->uploaded, s.upload_stream));
+    if (!landings.empty() && up != stream) {
+        HIP_TRY(hipEventRecord(run_slot->uploaded, up));
         HIP_TRY(hipStreamWaitEvent(stream, run_slot->uploaded, 0));
+    }
+    if (!landings.empty()) {
         for (const Landing &l : landings)
             HIP_TRY(hipMemcpyAsync(l.target, run_slot->device_upload + l.from, l.size,
                                    hipMemcpyDeviceToDevice, stream));
@@ -1341,12 +1348,15 @@ static size_t enqueue_downloads(Segment &s, Enqueued &q, RunSlot &slot, size_t m
     }
     // The next run's kernels may now overwrite the segment's buffers: the bytes leave from
     // the slot's device staging on the download stream.
-    HIP_TRY(hipEventRecord(slot.computed, stream));
-    HIP_TRY(hipStreamWaitEvent(s.download_stream, slot.computed, 0));
+    hipStream_t down = s.download_stream ? s.download_stream : stream;
+    if (down != stream) {
+        HIP_TRY(hipEventRecord(slot.computed, stream));
+        HIP_TRY(hipStreamWaitEvent(down, slot.computed, 0));
+    }
     for (const auto &span : spans)
         HIP_TRY(hipMemcpyAsync(slot.download + span.first, slot.device_download + span.first,
-                               span.second, hipMemcpyDeviceToHost, s.download_stream));
-    HIP_TRY(hipEventRecord(slot.done, s.download_stream));
+                               span.second, hipMemcpyDeviceToHost, down));
+    HIP_TRY(hipEventRecord(slot.done, down));
     return at;
 }
 
@@ -1610,9 +1620,9 @@ struct nxs_ctx {
     // segment's stream, and on `also` if not null) may still use its slot and workspace, so
     // that finishes first.
     void refuse_admitted(Admitted &a, const std::string &why, hipStream_t also = nullptr) {
-        (void)hipStreamSynchronize(a.seg->upload_stream);
+        if (a.seg->upload_stream) (void)hipStreamSynchronize(a.seg->upload_stream);
         (void)hipStreamSynchronize(a.seg->stream);
-        (void)hipStreamSynchronize(a.seg->download_stream);
+        if (a.seg->download_stream) (void)hipStreamSynchronize(a.seg->download_stream);
         if (also) (void)hipStreamSynchronize(also);
         release(a);
         post(a.job.ticket, a.job.done, a.job.user, nxs::refused_record(why));
