@@ -58,10 +58,14 @@ def alone(spec, fs):
         return out
 
 
-@pytest.mark.parametrize("budget,coalesce", [(None, 8), (None, 1), (1, 8)])
-def test_each_stream_is_the_same_alone_and_in_the_mix(frames, budget, coalesce):
+@pytest.mark.parametrize("budget,coalesce,transfer", [(None, 8, "1"), (None, 1, "1"), (1, 8, "1"),
+                                                      (None, 8, "0")])
+def test_each_stream_is_the_same_alone_and_in_the_mix(frames, budget, coalesce, transfer,
+                                                      monkeypatch):
     """budget=1 byte: every matcher run waits for the one before to free the device memory
-    (and no run joins another's launch). coalesce=1: every run its own matcher launch."""
+    (and no run joins another's launch). coalesce=1: every run its own matcher launch.
+    transfer="0": each segment's uploads and downloads on its own stream."""
+    monkeypatch.setenv("NXS_TRANSFER_STREAMS", transfer)
     with segment.Context(0) as ctx:
         ctx.coalescing(coalesce)
         if budget is not None:
