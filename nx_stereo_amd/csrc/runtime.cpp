@@ -2014,18 +2014,35 @@ int nxs_open(int device, nxs_ctx **out, char *err, size_t errlen) {
         report(std::string("hipSetDevice: ") + hipGetErrorString(e));
         return NXS_EDEVICE;
     }
-    auto *ctx = new nxs_ctx();
-    ctx->device = device;
-    // The matcher workspaces may take half the device memory free now, unless
-    // NXS_HBM_BUDGET_MB says otherwise (nxs_set_hbm_budget changes it later).
-    size_t free_bytes = 0, total_bytes = 0;
-    if (hipMemGetInfo(&free_bytes, &total_bytes) != hipSuccess) free_bytes = (size_t)8 << 30;
-    ctx->budget = free_bytes / 2;
-    if (const char *mb = std::getenv("NXS_HBM_BUDGET_MB")) ctx->budget = (size_t)std::atoll(mb) << 20;
-    if (const char *c = std::getenv("NXS_COALESCE"))
-        ctx->coalesce_max = std::max(1, std::min(nxs_ctx::MAX_COALESCE, std::atoi(c)));
-    ctx->worker = std::thread([ctx] { ctx->work(); });
-    ctx->completer = std::thread([ctx] { ctx->complete(); });
+    nxs_ctx *ctx = nullptr;
+    try {
+        ctx = new nxs_ctx();
+        ctx->device = device;
+        // The matcher workspaces may take half the device memory free now, unless
+        // NXS_HBM_BUDGET_MB says otherwise (nxs_set_hbm_budget changes it later).
+        size_t free_bytes = 0, total_bytes = 0;
+        if (hipMemGetInfo(&free_bytes, &total_bytes) != hipSuccess) free_bytes = (size_t)8 << 30;
+        ctx->budget = free_bytes / 2;
+        if (const char *mb = std::getenv("NXS_HBM_BUDGET_MB"))
+            ctx->budget = (size_t)std::atoll(mb) << 20;
+        if (const char *c = std::getenv("NXS_COALESCE"))
+            ctx->coalesce_max = std::max(1, std::min(nxs_ctx::MAX_COALESCE, std::atoi(c)));
+        ctx->worker = std::thread([ctx] { ctx->work(); });
+        ctx->completer = std::thread([ctx] { ctx->complete(); });
+    } catch (const std::exception &x) {
+        // Out of memory or threads: stop what was started and report.
+        if (ctx) {
+            {
+                std::lock_guard<std::mutex> lock(ctx->mu);
+                ctx->stopping = true;
+            }
+            ctx->cv_jobs.notify_all();
+            if (ctx->worker.joinable()) ctx->worker.join();
+            delete ctx;
+        }
+        report(std::string("nxs_open: ") + x.what());
+        return NXS_ENOMEM;
+    }
     *out = ctx;
     return NXS_OK;
 }
@@ -2090,13 +2107,20 @@ void nxs_close(nxs_ctx *ctx) {
 
 int64_t nxs_submit(nxs_ctx *ctx, const uint8_t *record, size_t len, nxs_done_fn done, void *user) {
     if (!ctx || (!record && len)) return nxs::fail(NXS_EINVAL, "no context or record");
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    if (ctx->stopping) return nxs::fail(NXS_EINVAL, "the context is closing");
-    const int64_t ticket = ctx->next_ticket++;
-    ctx->jobs.push_back(nxs::Job{ticket, std::vector<uint8_t>(record, record + len), done, user});
-    if (!done) ctx->kept[ticket];
-    ctx->cv_jobs.notify_one();
-    return ticket;
+    try {
+        // The copy (megabytes with frames) is made before the context's lock is taken.
+        nxs::Job job{0, std::vector<uint8_t>(record, record + len), done, user, nullptr};
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        if (ctx->stopping) return nxs::fail(NXS_EINVAL, "the context is closing");
+        const int64_t ticket = ctx->next_ticket++;
+        job.ticket = ticket;
+        ctx->jobs.push_back(std::move(job));
+        if (!done) ctx->kept[ticket];
+        ctx->cv_jobs.notify_one();
+        return ticket;
+    } catch (const std::bad_alloc &) {
+        return nxs::fail(NXS_ENOMEM, "out of host memory for the record");
+    }
 }
 
 int nxs_wait(nxs_ctx *ctx, int64_t ticket, int timeout_ms) {
@@ -2498,6 +2522,13 @@ int nxs_run_device(nxs_ctx *ctx, const uint8_t *record, size_t len, const nxs_te
 int nxs_profile_start(int capacity) {
     using namespace nxs;
     std::lock_guard<std::mutex> lock(g_profiler.mu);
+    try {
+        g_profiler.begin.reserve(std::max(capacity, 0));
+        g_profiler.end.reserve(std::max(capacity, 0));
+        g_profiler.kernel.reserve(std::max(capacity, 0));
+    } catch (const std::bad_alloc &) {
+        return fail(NXS_ENOMEM, "out of host memory for the profiler");
+    }
     while ((int)g_profiler.begin.size() < capacity) {
         hipEvent_t a, b;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
