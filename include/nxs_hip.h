@@ -190,7 +190,8 @@ int nxs_coarse_to_fine_host(const uint8_t *left, const uint8_t *right, int width
  * over a descriptor table in device memory; the whole batch is captured
  * once into a hipGraph that nxs_batch_run replays. With `autotune`, each
  * class's aggregation schedule (prefetch depth 2/4/8, lanes per line 16/32/64,
- * one or two waves per line, orientations in one launch or one each) is timed on the
+ * one or two waves per line, the two waves meeting through memory or by
+ * recomputation, orientations in one launch or one each) is timed on the
  * batch itself and the fastest kept,
  * also for later single-pair launches of that size. The batch owns one
  * matcher workspace per pair; the pairs' images and outputs stay the
@@ -220,7 +221,14 @@ typedef struct nxs_schedule {
                      well, where disparities / (8 x waves) is 2, 4 or 8. Without a pick:
                      2 waves of 16 lanes, prefetch 8 (4 at 192 disparities); at 128
                      disparities and 8 paths, 2 waves of 32 lanes, prefetch 4 up to
-                     500,000 pixels and 8 above */
+                     500,000 pixels and 8 above; at 8 paths where 4 x width x height x
+                     disparities passes 512 MB, 2 waves of 16 lanes meeting by
+                     recomputation, checkpoints every 8 steps */
+    int meet;     /* with 2 waves: 0, each direction stores its bytes of the line's first
+                     half and the other reads them back; 1, each direction stores its path
+                     every `prefetch` steps of its first half and the other recomputes the
+                     bytes from there (a third of the traffic, one more recurrence step per
+                     pixel of the second half) */
 } nxs_schedule;
 
 int nxs_schedule_set(int width, int height, int disparities, int paths, const nxs_schedule *s);

@@ -55,7 +55,7 @@ class NxsSchedule(ctypes.Structure):
     """struct nxs_schedule: how matcher launches of one size aggregate."""
 
     _fields_ = [("prefetch", ctypes.c_int), ("split", ctypes.c_int), ("lanes", ctypes.c_int),
-                ("waves", ctypes.c_int)]
+                ("waves", ctypes.c_int), ("meet", ctypes.c_int)]
 
 
 class NativeError(RuntimeError):
@@ -303,17 +303,18 @@ def coarse_to_fine_device(left_ptr: int, right_ptr: int, width: int, height: int
     _check(rc, "nxs_coarse_to_fine_device")
 
 
-def set_schedule(width, height, disparities, paths, prefetch, split, lanes=16, waves=1) -> None:
-    s = NxsSchedule(prefetch, int(split), lanes, waves)
+def set_schedule(width, height, disparities, paths, prefetch, split, lanes=16, waves=1,
+                 meet=0) -> None:
+    s = NxsSchedule(prefetch, int(split), lanes, waves, meet)
     _check(load().nxs_schedule_set(width, height, disparities, paths, ctypes.byref(s)),
            "nxs_schedule_set")
 
 
-def get_schedule(width, height, disparities, paths) -> tuple[int, bool, int, int]:
+def get_schedule(width, height, disparities, paths) -> tuple[int, bool, int, int, int]:
     s = NxsSchedule()
     _check(load().nxs_schedule_get(width, height, disparities, paths, ctypes.byref(s)),
            "nxs_schedule_get")
-    return s.prefetch, bool(s.split), s.lanes, s.waves
+    return s.prefetch, bool(s.split), s.lanes, s.waves, s.meet
 
 
 class Batch:

@@ -57,6 +57,8 @@ struct Schedule {
     int waves = 1;  // 16 lanes: 2 = a line's two directions in two waves of one block (at
                     // 3 paths, whose columns go down only, four columns a wave); 8 paths:
                     // 4 or 8 = each direction's D split over 2 or 4 waves as well
+    int meet = 0;   // two waves of one slice: 0 = each direction's first-half bytes stored
+                    // and read back; 1 = recomputed from a checkpoint every `prefetch` steps
 };
 
 // Slices per direction (2 or 4) that fit D disparities, or 1 where they do not.

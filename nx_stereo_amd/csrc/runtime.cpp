@@ -91,6 +91,18 @@ static Schedule default_schedule(int width, int height, int disparities, int pat
         s.lanes = 32;
         s.prefetch = (size_t)width * height <= 500000 ? 4 : 8;
     }
+    // At 8 paths, where the four orientations' byte volumes pass 512 MB, twice the 256 MB
+    // Infinity Cache, the two waves meet by recomputation from a checkpoint every 8 steps: the
+    // path kernel then moves about 13N bytes instead of 20N, for about half again the
+    // recurrence work. Measured on MI355X: 1920x1080 at 128, 192 and 256 disparities 786,
+    // 1041 and 1358 us against 922, 1580 and 1819; 1280x720x192 476 against 605. Where the
+    // volumes stay in the cache the recurrence work costs more than the traffic saves
+    // (728x544x128 188 against 160 us, 1024x768x128 369 against 338).
+    if (paths == 8 && (size_t)4 * width * height * disparities > ((size_t)512 << 20)) {
+        s.lanes = 16;
+        s.prefetch = 8;
+        s.meet = 1;
+    }
     return s;
 }
 
@@ -2728,6 +2740,9 @@ struct nxs_batch {
                 for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, 16, waves});
         if (two_lanes_fit(c.p.disparities))
             for (int prefetch : {2, 4, 8}) candidates.push_back(Schedule{prefetch, false, 32, 2});
+        // The two waves meeting by recomputation, checkpoints every 4 or 8 steps.
+        for (int prefetch : {4, 8}) candidates.push_back(Schedule{prefetch, false, 16, 2, 1});
+        if (two_lanes_fit(c.p.disparities)) candidates.push_back(Schedule{8, false, 32, 2, 1});
         candidates.push_back(Schedule{4, true, 16, 1});
         for (const Schedule &s : candidates) {
             HIP_TRY(launch_match_batch(c.p, tables + c.first, c.count, s, stream));
@@ -2740,8 +2755,8 @@ struct nxs_batch {
             HIP_TRY(hipEventElapsedTime(&ms, start, stop));
             trials += "  prefetch " + std::to_string(s.prefetch) + ", " +
                       std::to_string(s.lanes) + " lanes, " + std::to_string(s.waves) +
-                      (s.waves > 1 ? " waves" : " wave") + (s.split ? ", split" : "") + ": " +
-                      std::to_string(ms / 3) + " ms\n";
+                      (s.waves > 1 ? " waves" : " wave") + (s.meet ? ", recomputed" : "") +
+                      (s.split ? ", split" : "") + ": " + std::to_string(ms / 3) + " ms\n";
             if (ms < best) {
                 best = ms;
                 c.schedule = s;
@@ -2858,6 +2873,7 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
                  << " paths " << c.p.paths << ": prefetch " << c.schedule.prefetch << ", "
                  << c.schedule.lanes << " lanes per line, " << c.schedule.waves
                  << (c.schedule.waves == 2 ? " waves" : " wave") << " per line, "
+                 << (c.schedule.meet ? "meeting by recomputation, " : "")
                  << (c.schedule.split ? "one launch per orientation" : "orientations in one launch")
                  << (autotune ? " (tuned, " + std::to_string(c.tuned_ms) + " ms)" : "") << "\n";
         batch->description = text.str() + batch->trials;
@@ -2887,15 +2903,18 @@ int nxs_schedule_set(int width, int height, int disparities, int paths, const nx
     if (s->waves > 2 && nxs::slices_fit(disparities, s->waves / 2) == 1)
         return nxs::fail(NXS_EINVAL, std::to_string(s->waves) + " waves per line do not fit " +
                                          std::to_string(disparities) + " disparities");
+    if (s->meet != 0 && s->meet != 1) return nxs::fail(NXS_EINVAL, "meet must be 0 or 1");
+    if (s->meet != 0 && s->waves != 2)
+        return nxs::fail(NXS_EINVAL, "meeting by recomputation takes two waves per line");
     nxs::remember_schedule(width, height, disparities, paths,
-                           nxs::Schedule{s->prefetch, s->split != 0, s->lanes, s->waves});
+                           nxs::Schedule{s->prefetch, s->split != 0, s->lanes, s->waves, s->meet});
     return NXS_OK;
 }
 
 int nxs_schedule_get(int width, int height, int disparities, int paths, nxs_schedule *s) {
     if (!s) return nxs::fail(NXS_EINVAL, "no schedule");
     const nxs::Schedule k = nxs::schedule_for(width, height, disparities, paths);
-    *s = nxs_schedule{k.prefetch, k.split ? 1 : 0, k.lanes, k.waves};
+    *s = nxs_schedule{k.prefetch, k.split ? 1 : 0, k.lanes, k.waves, k.meet};
     return NXS_OK;
 }
 

@@ -442,6 +442,27 @@ struct PathState {
         }
         return b;
     }
+
+    // The path whose values bytes() gave: the values, their minimum over the line, the
+    // edges and the next step's part are all functions of the values. All-zero bytes give
+    // the path entering a line (reset()).
+    __device__ __forceinline__ void restore(const Bytes<K> &b, uint32_t p1x2, int lane) {
+        if constexpr (K == 2) {
+            v[0] = lo_pair(b.w[0]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < R / 2; k++) {
+                v[2 * k] = lo_pair(b.w[k]);
+                v[2 * k + 1] = hi_pair(b.w[k]);
+            }
+        }
+        uint32_t mn = 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < R; k++) mn = pk_min(mn, v[k]);
+        PM = LineLanes<LANES>::min(pk_min_halves(mn));
+        left_in = right_in = SENT;
+        prepare(p1x2, lane);
+    }
 };
 
 struct PathArgs {
@@ -657,6 +678,57 @@ __global__ __launch_bounds__(PATH_BLOCK) void path_kernel(PathArgs a) {
 
 constexpr int TWO_BLOCK = 128;
 
+// Columns with 3 paths: down only. The block takes 2 LINES columns, each wave LINES of its
+// own, walked alone (no meeting, no barrier), storing its own path costs.
+template <int K, int PREFETCH, int LANES>
+__device__ __forceinline__ void walk_columns_down(const PathArgs &a, const PairDesc &pd, int b,
+                                                  int dir, int lane, int row) {
+    constexpr int LINES = 64 / LANES;
+    constexpr uint32_t D_BYTES = LANES * K;
+    const int W = a.W, H = a.H, o = 1;
+    const int first_column = (b - a.block_start[o]) * 2 * LINES + dir * LINES;
+    const LineGeometry gc = line_geometry(1, first_column + row, W, H);
+    int longest_column = 0;
+#pragma unroll
+    for (int r = 0; r < LINES; r++)
+        longest_column = max(longest_column, line_geometry(1, first_column + r, W, H).length);
+    if (longest_column == 0) return;  // wave-uniform
+    const int Lc = gc.length;
+    const uint32_t stride_c = (uint32_t)W * D_BYTES;
+    const uint32_t volume_c = (uint32_t)W * H * D_BYTES;
+    const Rsrc costs_c = buffer_of(pd.b.costs, volume_c);
+    const Rsrc paired_c = buffer_of(orientation_volume(pd.b, 1), volume_c);
+    const uint32_t p1x2c = (uint32_t)pd.p1 * 0x10001u, p2c = (uint32_t)pd.p2;
+    const uint32_t first_c = (uint32_t)gc.x0 * D_BYTES + lane * K;
+    PathState<K, LANES> sc;
+    sc.reset(p1x2c, lane);
+    Bytes<K> ring_c[PREFETCH];
+    Bytes<K> none_c;
+#pragma unroll
+    for (int k = 0; k < (K + 3) / 4; k++) none_c.w[k] = 0;
+    // As a step issues: a (masked) store, then the load.
+#pragma unroll
+    for (int p = 0; p < PREFETCH; p++) {
+        __builtin_amdgcn_sched_barrier(0);
+        vstore<K>(paired_c, MASKED, none_c);
+        ring_c[p] = vload<K>(costs_c, first_c + p * stride_c);
+    }
+    uint32_t at_c = first_c, ahead_c = first_c + PREFETCH * stride_c;
+    for (int t0 = 0; t0 < longest_column; t0 += PREFETCH) {
+#pragma unroll
+        for (int p = 0; p < PREFETCH; p++) {
+            const int t = t0 + p;
+            __builtin_amdgcn_sched_barrier(0);
+            sc.step(ring_c[p], p1x2c, p2c, lane);
+            vstore<K>(paired_c, t < Lc ? at_c : MASKED, sc.bytes());
+            ring_c[p] = vload<K>(costs_c, ahead_c);
+            at_c += stride_c;
+            ahead_c += stride_c;
+        }
+    }
+}
+
+
 // LANES: lanes per line, 16 (four lines a wave) or 32 (two lines a wave, K = D / 32: half
 // the instructions per step in each wave, twice the waves). STREAM: the launch's volumes
 // overflow the 256 MB Infinity Cache; phase C's pair sums (read next by select) and its
@@ -689,50 +761,8 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
         }
     const int W = a.W, H = a.H;
     const PairDesc pd = pair_of(a.single, a.table, blockIdx.y);
-    constexpr uint32_t D_BYTES = LANES * K;
     if (o == 1 && !a.backward_columns) {
-        // Columns with 3 paths: down only. The block takes 2 LINES columns, each wave LINES of
-        // its own, walked alone (no meeting, no barrier), storing its own path costs.
-        const int first_column = (b - a.block_start[o]) * 2 * LINES + dir * LINES;
-        const LineGeometry gc = line_geometry(1, first_column + row, W, H);
-        int longest_column = 0;
-#pragma unroll
-        for (int r = 0; r < LINES; r++)
-            longest_column = max(longest_column, line_geometry(1, first_column + r, W, H).length);
-        if (longest_column == 0) return;  // wave-uniform
-        const int Lc = gc.length;
-        const uint32_t stride_c = (uint32_t)W * D_BYTES;
-        const uint32_t volume_c = (uint32_t)W * H * D_BYTES;
-        const Rsrc costs_c = buffer_of(pd.b.costs, volume_c);
-        const Rsrc paired_c = buffer_of(orientation_volume(pd.b, 1), volume_c);
-        const uint32_t p1x2c = (uint32_t)pd.p1 * 0x10001u, p2c = (uint32_t)pd.p2;
-        const uint32_t first_c = (uint32_t)gc.x0 * D_BYTES + lane * K;
-        PathState<K, LANES> sc;
-        sc.reset(p1x2c, lane);
-        Bytes<K> ring_c[PREFETCH];
-        Bytes<K> none_c;
-#pragma unroll
-        for (int k = 0; k < (K + 3) / 4; k++) none_c.w[k] = 0;
-        // As a step issues: a (masked) store, then the load.
-#pragma unroll
-        for (int p = 0; p < PREFETCH; p++) {
-            __builtin_amdgcn_sched_barrier(0);
-            vstore<K>(paired_c, MASKED, none_c);
-            ring_c[p] = vload<K>(costs_c, first_c + p * stride_c);
-        }
-        uint32_t at_c = first_c, ahead_c = first_c + PREFETCH * stride_c;
-        for (int t0 = 0; t0 < longest_column; t0 += PREFETCH) {
-#pragma unroll
-            for (int p = 0; p < PREFETCH; p++) {
-                const int t = t0 + p;
-                __builtin_amdgcn_sched_barrier(0);
-                sc.step(ring_c[p], p1x2c, p2c, lane);
-                vstore<K>(paired_c, t < Lc ? at_c : MASKED, sc.bytes());
-                ring_c[p] = vload<K>(costs_c, ahead_c);
-                at_c += stride_c;
-                ahead_c += stride_c;
-            }
-        }
+        walk_columns_down<K, PREFETCH, LANES>(a, pd, b, dir, lane, row);
         return;
     }
     const int first_line = (b - a.block_start[o]) * LINES;
@@ -857,6 +887,141 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
     for (; i0 < c_start && i0 < longest; i0 += PREFETCH) general(i0);
     for (; i0 < c_end; i0 += PREFETCH) after_window();
     for (; i0 < longest; i0 += PREFETCH) general(i0);
+}
+
+// ------------------------------------------------------------------------------------------
+// The two-wave form meeting by recomputation. The form above stores each direction's bytes
+// of its first half and reads them back in the other's second half: 2N of the 5N an
+// orientation moves. Here a direction stores its path only every S steps of its first half
+// (a checkpoint: the path's D bytes are the whole of its state), and the other direction,
+// walking its second half, takes the first half's pixels S at a time: it loads the checkpoint
+// that enters them, walks the first direction's recurrence over those S pixels again (their
+// costs are the ones it walks itself) and keeps the bytes in registers, then walks its own S
+// steps, adding them. The recurrence is integer arithmetic, so the recomputed bytes are the
+// ones the first direction made. An orientation then moves 2N of costs, 2N/S of checkpoints
+// and N of pair sums, for about one more recurrence step per pixel of the second half.
+//
+// Line positions 0 .. T-1, T = S * ceil(longest / S) over the block's lines: a line of length
+// L has its pixels at 0 .. L-1 and nothing past them (costs load as zeros, stores drop). A
+// direction that walks zero costs from its entering path keeps that path (every value stays
+// 0), so the backward wave may start at T-1. The forward wave takes the first nf = ceil(n/2)
+// of the n segments of S positions, the backward wave the last n - nf: its phase 1. In its
+// phase 2 each wave walks the other's segments, nearest first. Segment j of a wave's phase 1
+// (its steps jS .. jS+S-1) enters with the path after its step jS-1, which the wave stores at
+// that step's pixel; in the other wave's step order that pixel is the one just past the
+// segment, whose pair sum the other stores only one segment later. Segment 0 enters with the
+// line's entering path, which restores from zero bytes (a masked load). The phases meet at
+// one barrier, after every checkpoint store has completed; the waves touch nothing of each
+// other's otherwise, and no LDS.
+template <int K, int S, int LANES, bool STREAM>
+__global__ __launch_bounds__(TWO_BLOCK) void path_rc_kernel(PathArgs a) {
+    constexpr int NT = STREAM && K >= 4 ? 2 : 0;
+    static_assert(LANES == 16 || LANES == 32, "a line is one or two DPP rows");
+    static_assert(S == 2 || S == 4 || S == 8, "segments of 2, 4 or 8 steps");
+    constexpr int D = LANES * K;
+    constexpr int LINES = 64 / LANES;
+    const int dir = threadIdx.x >> 6;  // wave-uniform: 0 forward, 1 backward
+    const int lane = threadIdx.x & (LANES - 1);
+    const int row = (threadIdx.x & 63) / LANES;
+    const int b = blockIdx.x;
+    int o = a.last;
+    for (int k = a.first; k < a.last; k++)
+        if (b < a.block_start[k + 1]) {
+            o = k;
+            break;
+        }
+    const int W = a.W, H = a.H;
+    const PairDesc pd = pair_of(a.single, a.table, blockIdx.y);
+    if (o == 1 && !a.backward_columns) {
+        walk_columns_down<K, S, LANES>(a, pd, b, dir, lane, row);
+        return;
+    }
+    const int first_line = (b - a.block_start[o]) * LINES;
+    const LineGeometry g = line_geometry(o, first_line + row, W, H);
+    int longest = 0;
+#pragma unroll
+    for (int r = 0; r < LINES; r++) longest = max(longest, line_geometry(o, first_line + r, W, H).length);
+    if (longest == 0) return;  // both waves leave
+    const int n = (longest + S - 1) / S;
+    const int T = n * S;
+    const int h1 = (dir == 0 ? (n + 1) / 2 : n / 2) * S;  // this wave's phase 1, in steps
+    const int h2 = T - h1;
+
+    const int dy = o == 0 ? 0 : 1;
+    const int dx = o == 0 ? 1 : (o == 1 ? 0 : (o == 2 ? 1 : -1));
+    const uint32_t stride = (uint32_t)((dy * W + dx) * D);
+    const int L = g.length;
+    const uint32_t first = ((uint32_t)g.y0 * W + (uint32_t)g.x0) * D + lane * K;
+    const uint32_t volume = (uint32_t)W * H * D;
+    const Rsrc costs = buffer_of(pd.b.costs, volume);
+    const Rsrc paired = buffer_of(orientation_volume(pd.b, o), volume);
+    const uint32_t p1x2 = (uint32_t)pd.p1 * 0x10001u;
+    const uint32_t p2 = (uint32_t)pd.p2;
+    // Step t of this wave is at position t (forward) or T-1-t (backward): a pixel of the line
+    // when t - lo < L, with lo the first step on it.
+    const int lo = dir == 0 ? 0 : T - L;
+    const uint32_t base = dir == 0 ? first : first + (uint32_t)(T - 1) * stride;
+    const uint32_t step = dir == 0 ? stride : 0u - stride;
+    auto at = [&](int t) __attribute__((always_inline)) -> uint32_t {
+        return (uint32_t)(t - lo) < (uint32_t)L ? base + (uint32_t)t * step : MASKED;
+    };
+
+    PathState<K, LANES> s;
+    s.reset(p1x2, lane);
+    Bytes<K> ca[S], cb[S];
+#pragma unroll
+    for (int p = 0; p < S; p++) ca[p] = vload<K>(costs, at(p));
+    // Phase 1: the wave's own path, costs S steps ahead, a checkpoint at the end of every
+    // segment but the last.
+    for (int t0 = 0; t0 < h1; t0 += S) {
+#pragma unroll
+        for (int p = 0; p < S; p++) {
+            const int t = t0 + p;
+            __builtin_amdgcn_sched_barrier(0);
+            s.step(ca[p], p1x2, p2, lane);
+            if (p == S - 1) vstore<K>(paired, t + 1 < h1 ? at(t) : MASKED, s.bytes());
+            __builtin_amdgcn_sched_barrier(0);
+            ca[p] = vload<K>(costs, at(t + S));
+        }
+    }
+    // Every checkpoint stored and the other wave's too.
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    if (h2 == 0) return;  // wave-uniform
+    PathState<K, LANES> rec;
+    // Phase 2, segment k: steps h1 + kS .. h1 + kS + S-1, whose costs are in cur; the
+    // next segment's in the other array; the checkpoint entering it (in the other's
+    // order) at step h1 + kS + S.
+    Bytes<K> entering = vload<K>(paired, S < h2 ? at(h1 + S) : MASKED);
+#pragma unroll
+    for (int p = 0; p < S; p++) cb[p] = vload<K>(costs, at(h1 + S + p));
+    auto segment = [&](int k, Bytes<K>(&cur)[S]) __attribute__((always_inline)) {
+        const int t0 = h1 + k * S;
+        rec.restore(entering, p1x2, lane);
+        Bytes<K> theirs[S];
+#pragma unroll
+        for (int r = S - 1; r >= 0; r--) {
+            __builtin_amdgcn_sched_barrier(0);
+            rec.step(cur[r], p1x2, p2, lane);
+            theirs[r] = rec.bytes();
+        }
+        entering = vload<K>(paired, t0 + 2 * S < T ? at(t0 + 2 * S) : MASKED);
+#pragma unroll
+        for (int p = 0; p < S; p++) {
+            __builtin_amdgcn_sched_barrier(0);
+            s.step(cur[p], p1x2, p2, lane);
+            const Bytes<K> mine = s.bytes();
+            Bytes<K> out;
+#pragma unroll
+            for (int w = 0; w < (K + 3) / 4; w++) out.w[w] = mine.w[w] + theirs[p].w[w];
+            vstore<K, NT>(paired, at(t0 + p), out);
+            __builtin_amdgcn_sched_barrier(0);
+            cur[p] = vload<K>(costs, at(t0 + 2 * S + p));
+        }
+    };
+    for (int k = 0; k < h2 / S; k += 2) {
+        segment(k, ca);
+        if ((k + 1) * S < h2) segment(k + 1, cb);
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1409,13 +1574,30 @@ static void launch_two_as(const PathArgs &a, int blocks, int count, int prefetch
     }
 }
 
+template <int K, int LANES, bool STREAM>
+static void launch_rc_as(const PathArgs &a, int blocks, int count, int segment, hipStream_t stream) {
+    switch (segment) {
+        case 2: hipLaunchKernelGGL((path_rc_kernel<K, 2, LANES, STREAM>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
+        case 8: hipLaunchKernelGGL((path_rc_kernel<K, 8, LANES, STREAM>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
+        default: hipLaunchKernelGGL((path_rc_kernel<K, 4, LANES, STREAM>), dim3(blocks, count), dim3(TWO_BLOCK), 0, stream, a); break;
+    }
+}
+
+// Two waves per line, meeting through memory (meet 0) or by recomputation from checkpoints
+// every `prefetch` steps (meet 1).
 template <int K, int LANES>
-static void launch_two(const PathArgs &a, int blocks, int count, int prefetch, bool stream_volumes,
-                       hipStream_t stream) {
-    if (stream_volumes)
+static void launch_two(const PathArgs &a, int blocks, int count, int prefetch, int meet,
+                       bool stream_volumes, hipStream_t stream) {
+    if (meet == 1) {
+        if (stream_volumes)
+            launch_rc_as<K, LANES, true>(a, blocks, count, prefetch, stream);
+        else
+            launch_rc_as<K, LANES, false>(a, blocks, count, prefetch, stream);
+    } else if (stream_volumes) {
         launch_two_as<K, LANES, true>(a, blocks, count, prefetch, stream);
-    else
+    } else {
         launch_two_as<K, LANES, false>(a, blocks, count, prefetch, stream);
+    }
 }
 
 template <int K>
@@ -1454,9 +1636,9 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
             launch_sliced<16 * K>(a, blocks, count, sch.prefetch, slices, stream);
         } else if (two && two_lanes == 32) {
             if constexpr (K % 2 == 0 && (K / 2 == 2 || K / 2 == 4 || K / 2 == 8))
-                launch_two<K / 2, 32>(a, blocks, count, sch.prefetch, stream_volumes, stream);
+                launch_two<K / 2, 32>(a, blocks, count, sch.prefetch, sch.meet, stream_volumes, stream);
         } else if (two) {
-            launch_two<K, 16>(a, blocks, count, sch.prefetch, stream_volumes, stream);
+            launch_two<K, 16>(a, blocks, count, sch.prefetch, sch.meet, stream_volumes, stream);
         } else {
             launch_paths_lanes<16 * K>(a, blocks, count, sch, stream);
         }

@@ -66,18 +66,21 @@ def test_every_schedule_gives_the_same_disparity(disparities, paths):
     ws = hip.Buffer(_native.workspace_size(w, h, disparities, paths))
     kept = _native.get_schedule(w, h, disparities, paths)
     try:
-        shapes = [(lanes, 1) for lanes in LANES[disparities]] + \
-            [(16, w) for w in WAVES[disparities] if paths == 8 or w == 2] + \
-            ([(32, 2)] if disparities in (64, 128, 256) else [])
-        for lanes, waves in shapes:
+        two_lanes = (16, 32) if disparities in (64, 128, 256) else (16,)
+        shapes = [(lanes, 1, 0) for lanes in LANES[disparities]] + \
+            [(16, w, 0) for w in WAVES[disparities] if paths == 8 or w == 2] + \
+            ([(32, 2, 0)] if disparities in (64, 128, 256) else []) + \
+            [(lanes, 2, 1) for lanes in two_lanes]  # meeting by recomputation
+        for lanes, waves, meet in shapes:
             for prefetch in (2, 4, 8):
                 for split in (False, True):
-                    _native.set_schedule(w, h, disparities, paths, prefetch, split, lanes, waves)
+                    _native.set_schedule(w, h, disparities, paths, prefetch, split, lanes, waves,
+                                         meet)
                     _native.match_device(l.ptr, r.ptr, w, h, disparities, paths, 8, 64, 1.0526316,
                                          o.ptr, ws.ptr, ws.nbytes)
                     hip.synchronize()
                     got = o.download((h, w), np.float32)
-                    case = (lanes, waves, prefetch, split)
+                    case = (lanes, waves, meet, prefetch, split)
                     assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), case
                     sums = _native.match_host(left, right, disparities, paths, intermediates=True)["sums"]
                     assert np.array_equal(sums, want_sums), case
@@ -97,13 +100,14 @@ def test_several_waves_on_odd_and_short_lines(disparities, paths):
             left, right = scenes.scene(w, h, 3)
             want = oracle.aggregate(oracle.costs(oracle.census(left), oracle.census(right),
                                                  disparities), paths, 8, 64)
-            shapes = [(16, waves) for waves in (WAVES[disparities] if paths == 8 else (2,))]
-            for lanes, waves in shapes + [(32, 2)]:
+            shapes = [(16, waves, 0) for waves in (WAVES[disparities] if paths == 8 else (2,))]
+            for lanes, waves, meet in shapes + [(32, 2, 0), (16, 2, 1), (32, 2, 1)]:
                 for prefetch in (2, 4, 8):
-                    _native.set_schedule(w, h, disparities, paths, prefetch, False, lanes, waves)
+                    _native.set_schedule(w, h, disparities, paths, prefetch, False, lanes, waves,
+                                         meet)
                     sums = _native.match_host(left, right, disparities, paths,
                                               intermediates=True)["sums"]
-                    assert np.array_equal(sums, want), (w, h, lanes, waves, prefetch)
+                    assert np.array_equal(sums, want), (w, h, lanes, waves, meet, prefetch)
     finally:
         for size, schedule in kept.items():
             _native.set_schedule(*size, disparities, paths, *schedule)
@@ -122,3 +126,7 @@ def test_lanes_that_do_not_fit_are_refused():
         _native.set_schedule(97, 31, 64, 8, 4, False, 16, 8)
     with pytest.raises(_native.NativeError, match="4 waves per line take 8 paths"):
         _native.set_schedule(97, 31, 128, 3, 4, False, 16, 4)
+    with pytest.raises(_native.NativeError, match="recomputation takes two waves per line"):
+        _native.set_schedule(97, 31, 128, 8, 4, False, 16, 1, 1)
+    with pytest.raises(_native.NativeError, match="meet must be 0 or 1"):
+        _native.set_schedule(97, 31, 128, 8, 4, False, 16, 2, 2)
