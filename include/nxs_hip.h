@@ -222,7 +222,7 @@ typedef struct nxs_schedule {
                      2 waves of 16 lanes, prefetch 8 (4 at 192 disparities); at 128
                      disparities and 8 paths, 2 waves of 32 lanes, prefetch 4 up to
                      500,000 pixels and 8 above; at 8 paths where 4 x width x height x
-                     disparities passes 512 MB, 2 waves of 16 lanes meeting by
+                     disparities passes 256 MB, 2 waves of 16 lanes meeting by
                      recomputation, checkpoints every 8 steps */
     int meet;     /* with 2 waves: 0, each direction stores its bytes of the line's first
                      half and the other reads them back; 1, each direction stores its path

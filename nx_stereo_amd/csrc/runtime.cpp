@@ -91,14 +91,15 @@ static Schedule default_schedule(int width, int height, int disparities, int pat
         s.lanes = 32;
         s.prefetch = (size_t)width * height <= 500000 ? 4 : 8;
     }
-    // At 8 paths, where the four orientations' byte volumes pass 512 MB, twice the 256 MB
-    // Infinity Cache, the two waves meet by recomputation from a checkpoint every 8 steps: the
-    // path kernel then moves about 13N bytes instead of 20N, for about half again the
-    // recurrence work. Measured on MI355X: 1920x1080 at 128, 192 and 256 disparities 786,
-    // 1041 and 1358 us against 922, 1580 and 1819; 1280x720x192 476 against 605. Where the
-    // volumes stay in the cache the recurrence work costs more than the traffic saves
-    // (728x544x128 188 against 160 us, 1024x768x128 369 against 338).
-    if (paths == 8 && (size_t)4 * width * height * disparities > ((size_t)512 << 20)) {
+    // At 8 paths, where the four orientations' byte volumes pass the 256 MB Infinity Cache,
+    // the two waves meet by recomputation from a checkpoint every 8 steps, blocks longest
+    // lines first: the path kernel then moves about 13N bytes instead of 20N, for about half
+    // again the recurrence work. Measured on MI355X: 1920x1080 at 64, 128 and 256
+    // disparities 520, 712 and 1207 us against 544, 924 and 1824; 1280x720x192 457 against
+    // 605; 1024x768x128 302 against 346. Where the volumes stay in the cache the recurrence
+    // work costs more than the traffic saves (728x544x128 198 against 160 us, 960x540x64 139
+    // against 123).
+    if (paths == 8 && (size_t)4 * width * height * disparities > ((size_t)256 << 20)) {
         s.lanes = 16;
         s.prefetch = 8;
         s.meet = 1;

@@ -473,7 +473,45 @@ struct PathArgs {
     // The launch covers orientations first..last; block_start[o] is the first block of o.
     int first, last;
     int block_start[4];
+    // Blocks longest lines first (8 paths, all four orientations in the launch): see
+    // block_lines.
+    int longest_first;
 };
+
+// The orientation of block b and the first of its `per_block` lines. By default blocks go
+// orientation by orientation, lines in order. Longest first: the rows and the columns (the
+// longer lines first), then the two diagonal orientations alternately, each from its middle
+// group of lines outward, since a diagonal's length falls with its distance from the middle
+// ones. The hardware starts blocks in index order as room frees up, so the long lines, which
+// bound the launch, start first and the short ones fill in behind them.
+struct BlockLines {
+    int o, first_line;
+};
+
+__device__ __forceinline__ BlockLines block_lines(const PathArgs &a, int b, int per_block) {
+    if (!a.longest_first) {
+        int o = a.last;
+        for (int k = a.first; k < a.last; k++)
+            if (b < a.block_start[k + 1]) {
+                o = k;
+                break;
+            }
+        return {o, (b - a.block_start[o]) * per_block};
+    }
+    const int rows = a.block_start[1] - a.block_start[0];
+    const int columns = a.block_start[2] - a.block_start[1];
+    const int diagonals = a.block_start[3] - a.block_start[2];  // each diagonal orientation
+    const bool rows_first = a.W >= a.H;
+    const int first_count = rows_first ? rows : columns;
+    if (b < first_count) return {rows_first ? 0 : 1, b * per_block};
+    b -= first_count;
+    const int second_count = rows_first ? columns : rows;
+    if (b < second_count) return {rows_first ? 1 : 0, b * per_block};
+    b -= second_count;
+    const int r = b >> 1, center = (diagonals - 1) / 2, k = (r + 1) >> 1;
+    const int group = r == 0 ? center : (r & 1 ? center + k : center - k);
+    return {2 + (b & 1), group * per_block};
+}
 
 // One wave per block, 64 / LANES lines per wave.
 constexpr int PATH_BLOCK = 64;
@@ -753,19 +791,15 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
     const int lane = threadIdx.x & (LANES - 1);
     const int row = (threadIdx.x & 63) / LANES;
     const int b = blockIdx.x;
-    int o = a.last;
-    for (int k = a.first; k < a.last; k++)
-        if (b < a.block_start[k + 1]) {
-            o = k;
-            break;
-        }
+    const BlockLines bl = block_lines(a, b, LINES);
+    const int o = bl.o;
     const int W = a.W, H = a.H;
     const PairDesc pd = pair_of(a.single, a.table, blockIdx.y);
     if (o == 1 && !a.backward_columns) {
         walk_columns_down<K, PREFETCH, LANES>(a, pd, b, dir, lane, row);
         return;
     }
-    const int first_line = (b - a.block_start[o]) * LINES;
+    const int first_line = bl.first_line;
     const LineGeometry g = line_geometry(o, first_line + row, W, H);
     // Block-uniform: the longest line, and the window around the lines' middles (in 2i).
     int longest = 0, mid_lo = 1 << 30, mid_hi = -(1 << 30);
@@ -924,19 +958,15 @@ __global__ __launch_bounds__(TWO_BLOCK) void path_rc_kernel(PathArgs a) {
     const int lane = threadIdx.x & (LANES - 1);
     const int row = (threadIdx.x & 63) / LANES;
     const int b = blockIdx.x;
-    int o = a.last;
-    for (int k = a.first; k < a.last; k++)
-        if (b < a.block_start[k + 1]) {
-            o = k;
-            break;
-        }
+    const BlockLines bl = block_lines(a, b, LINES);
+    const int o = bl.o;
     const int W = a.W, H = a.H;
     const PairDesc pd = pair_of(a.single, a.table, blockIdx.y);
     if (o == 1 && !a.backward_columns) {
         walk_columns_down<K, S, LANES>(a, pd, b, dir, lane, row);
         return;
     }
-    const int first_line = (b - a.block_start[o]) * LINES;
+    const int first_line = bl.first_line;
     const LineGeometry g = line_geometry(o, first_line + row, W, H);
     int longest = 0;
 #pragma unroll
@@ -1613,6 +1643,7 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
     a.W = W;
     a.H = H;
     a.backward_columns = p.paths == 8;
+    a.longest_first = 0;
     // Two waves per line at 8 paths, and at 3 paths for the rows (the down-only columns then
     // take four lines a wave, eight a block); 4 or 8 waves (slices) at 8 paths only.
     const bool two = sch.waves >= 2;
@@ -1626,6 +1657,12 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
     auto launch = [&](int first, int last) {
         a.first = first;
         a.last = last;
+        // The recomputing kernel holds about two waves a SIMD at 256 disparities, so a large
+        // pair's blocks do not all fit at once, and the diagonals' longest lines would start
+        // last: 1920x1080x256 1426 -> 1203 us, 1280x720x192 518 -> 451 us longest first.
+        // Where every block fits at once (728x544x128) the order changes nothing.
+        a.longest_first = two && slices == 1 && sch.meet == 1 && p.paths == 8 && first == 0 &&
+                          last == 3;
         int blocks = 0;
         for (int o = 0; o < 4; o++) {
             a.block_start[o] = blocks;
