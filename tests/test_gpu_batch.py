@@ -130,3 +130,28 @@ def test_lanes_that_do_not_fit_are_refused():
         _native.set_schedule(97, 31, 128, 8, 4, False, 16, 1, 1)
     with pytest.raises(_native.NativeError, match="meet must be 0 or 1"):
         _native.set_schedule(97, 31, 128, 8, 4, False, 16, 2, 2)
+
+
+@pytest.mark.parametrize("w,h,disparities,paths", [
+    (4096, 8, 64, 8),      # the widest rows: 512 segments of 8 a line
+    (16, 700, 64, 8),      # columns and diagonals far longer than the rows
+    (300, 40, 192, 8),     # 12 bytes a lane
+    (4096, 8, 128, 3),     # rows meet, columns go down only
+    (500, 120, 256, 8),
+])
+def test_meeting_by_recomputation_at_the_extremes(w, h, disparities, paths):
+    """The two waves meeting by recomputation on very long and very short lines, every
+    segment length, 16 and 32 lanes a line where they fit: the path sums bit for bit."""
+    left, right, _ = scenes.planar_pair(w, h, seed=w + h, low=4, high=min(disparities - 8, 60))
+    want = oracle.aggregate(oracle.costs(oracle.census(left), oracle.census(right), disparities),
+                            paths, 8, 64)
+    kept = _native.get_schedule(w, h, disparities, paths)
+    lanes = (16, 32) if disparities in (64, 128, 256) else (16,)
+    try:
+        for lane_count in lanes:
+            for prefetch in (2, 8):
+                _native.set_schedule(w, h, disparities, paths, prefetch, False, lane_count, 2, 1)
+                sums = _native.match_host(left, right, disparities, paths, intermediates=True)["sums"]
+                assert np.array_equal(sums, want), (w, h, disparities, paths, lane_count, prefetch)
+    finally:
+        _native.set_schedule(w, h, disparities, paths, *kept)
