@@ -192,8 +192,8 @@ int nxs_coarse_to_fine_host(const uint8_t *left, const uint8_t *right, int width
  * class's aggregation schedule (prefetch depth 2/4/8, lanes per line 16/32/64,
  * one or two waves per line, the two waves meeting through memory or by
  * recomputation, orientations in one launch or one each) is timed on the
- * batch itself and the fastest kept,
- * also for later single-pair launches of that size. The batch owns one
+ * batch itself and the fastest kept for the batch (single-pair launches of
+ * that size keep their own schedule). The batch owns one
  * matcher workspace per pair; the pairs' images and outputs stay the
  * caller's, read and written at every replay. No counterpart in the
  * reference, whose matcher takes one pair at a time. */

@@ -2766,7 +2766,9 @@ struct nxs_batch {
         c.tuned_ms = best / 3;
         (void)hipEventDestroy(start);
         (void)hipEventDestroy(stop);
-        remember_schedule(c.p.width, c.p.height, c.p.disparities, c.p.paths, c.schedule);
+        // The pick stays the batch's own: a schedule timed on a whole class of pairs at once
+        // is not the one a single pair of that size runs fastest with (at 728x544x128 the
+        // batch picks the recomputing form, which a single pair runs 20% slower).
     }
 };
 
