@@ -498,6 +498,24 @@ CALIBRATION_BYTES = 256 << 20
 def kernels_only(size: str) -> int:
     from nx_stereo_amd import _native, hip
 
+    if size == "batch":
+        # The batched leg's 96 pairs: autotuned, then one run and five replays of the graph.
+        pairs, buffers = [], []
+        for k, (w, h, d, count) in enumerate(BATCH_CLASSES):
+            for j in range(count):
+                left, right = match_pair(w, h, d, SEED + 1000 * (k + 1) + j)
+                l, r, o = hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h)
+                buffers += [l, r, o]
+                pairs.append((l.ptr, r.ptr, o.ptr, w, h, d, 8, 8, 64, 1.0526315789473684))
+        batch = _native.Batch(pairs, autotune=True)
+        stream = hip.Stream()
+        for _ in range(6):
+            batch.run(stream.handle)
+        stream.synchronize()
+        print(batch.describe().split("\n")[0], flush=True)
+        batch.close()
+        print("kernels-only batch done", flush=True)
+        return 0
     if size.startswith("c2f"):
         # sgm_coarse_to_fine/3 as the coarse_to_fine leg runs it (band 16, margin 4, 8 paths).
         w, h = (WIDTH, HEIGHT) if size == "c2f728" else (1920, 1080)
@@ -593,9 +611,10 @@ def main() -> int:
     parser.add_argument("--legs", default="large,batched,served,coarse_to_fine,streams,host_io",
                         help="comma-separated extra legs: large, batched, served, "
                              "coarse_to_fine, streams, host_io (or none)")
-    parser.add_argument("--kernels-only", choices=sorted(TRAFFIC_FILES),
+    parser.add_argument("--kernels-only", choices=sorted(TRAFFIC_FILES) + ["batch"],
                         help="run three matchings and the calibration copy (c2f*: five "
-                             "coarse-to-fine matchings), for rocprofv3")
+                             "coarse-to-fine matchings; batch: the batched leg's graph, "
+                             "autotuned, then six runs), for rocprofv3")
     parser.add_argument("--traffic", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT"),
                         help="HBM bytes per launch from two rocprofv3 --pmc runs into OUT")
     args = parser.parse_args()
