@@ -74,14 +74,88 @@ __host__ __device__ constexpr int right_slot(int f) { return f + (f >> 4); }
 __host__ __device__ constexpr int left_slots(int W) { return (W + 3) & ~3; }
 __host__ __device__ constexpr int right_slots(int W, int D) { return right_slot(D + W + 3) + 1; }
 
+// An image row in LDS: CENSUS_REACH replicated pixels, the row, the right edge replicated
+// on to a whole dword past the last census group's reads (bytes x0 .. x0 + 11 of the group
+// at x0 = 4 ((W + 3) / 4 - 1)).
+__host__ __device__ constexpr int census_row_pitch(int W) { return (W + 2 * CENSUS_REACH + 7) & ~3; }
+
 // LDS bytes of one block: the two census rows, then 7 padded rows of each image.
 __host__ __device__ constexpr size_t census_cost_lds(int W, int D) {
     return 8 * ((size_t)left_slots(W) + (size_t)right_slots(W, D)) +
-           14 * (size_t)(W + 2 * CENSUS_REACH);
+           14 * (size_t)census_row_pitch(W);
+}
+
+// The census of four neighbouring pixels x0 .. x0 + 3 of one image row, from the dwords of
+// the 7 padded rows (row r = dy + 3, `pitch_dwords` apart) that hold columns x0 - 4 .. x0 + 7.
+// Neighbours are compared two at a time, in the halves of a packed u16: with the
+// neighbours a, b as (a, b) and the centre c as (0x7FFE + c) in both halves, bit 15 of a
+// half of the difference is a + 1 < c. Comparison k (0..23: row rank 0..5 over dy = -3..3
+// but 0, then the column pairs (-4, -3), (-2, -1), (1, 2), (3, 4)) leaves its two bits at
+// bits k and 16 + k of the low word (k < 16), or k - 16 and k of the high word: an order of
+// the 48 bits of its own, the same for both images, which the Hamming costs do not see.
+// census_bits_in_order puts them in the window's row-major order.
+struct CensusQuad {
+    uint32_t lo[4], hi[4];
+};
+
+// Two pixels at a time, row by row, so that few registers are live (the rows' dwords are
+// read once for each two).
+__device__ __forceinline__ CensusQuad census_quad(const uint32_t *img, int pitch_dwords) {
+    CensusQuad q;
+    const uint32_t centres = img[3 * pitch_dwords + 1];
+#pragma unroll
+    for (int k0 = 0; k0 < 4; k0 += 2) {
+    uint32_t Y[4];
+#pragma unroll
+    for (int k = k0; k < k0 + 2; k++) {
+        const uint32_t half = ((centres >> (8 * k)) & 0xFFu) + 0x7FFEu;
+        Y[k] = half | (half << 16);
+        q.lo[k] = q.hi[k] = 0;
+    }
+#pragma unroll
+    for (int rank = 0; rank < 6; rank++) {
+        const int r = rank < 3 ? rank : rank + 1;
+        const uint32_t w0 = img[r * pitch_dwords], w1 = img[r * pitch_dwords + 1],
+                       w2 = img[r * pitch_dwords + 2];
+#pragma unroll
+        for (int k = k0; k < k0 + 2; k++) {
+#pragma unroll
+            for (int pair = 0; pair < 4; pair++) {
+                // Bytes first and first + 1 of the 12 bytes w0 w1 w2.
+                const int first = k + (pair < 2 ? 2 * pair : 2 * pair + 1);
+                const uint32_t X = first + 1 < 8
+                    ? __builtin_amdgcn_perm(w1, w0, (uint32_t)first | 0x0C00u | (uint32_t)(first + 1) << 16 | 0x0C000000u)
+                    : __builtin_amdgcn_perm(w2, w1, (uint32_t)(first - 4) | 0x0C00u | (uint32_t)(first - 3) << 16 | 0x0C000000u);
+                const uint32_t z = pk_sub(Y[k], X);
+                const int c = rank * 4 + pair;
+                if (c < 16) q.lo[k] |= (z >> (15 - c)) & (0x00010001u << c);
+                else q.hi[k] |= (z >> (31 - c)) & (0x00010001u << (c - 16));
+            }
+        }
+    }
+    }
+    return q;
+}
+
+// Bits 0..15 of x to the even bits 0..30.
+__device__ __forceinline__ uint32_t spread_even(uint32_t x) {
+    x &= 0xFFFFu;
+    x = (x | x << 8) & 0x00FF00FFu;
+    x = (x | x << 4) & 0x0F0F0F0Fu;
+    x = (x | x << 2) & 0x33333333u;
+    return (x | x << 1) & 0x55555555u;
+}
+
+// A census in census_quad's order to the window's row-major order (bit 2k and 2k + 1 are
+// comparison k's two).
+__device__ __forceinline__ uint64_t census_bits_in_order(uint32_t lo, uint32_t hi) {
+    const uint32_t low = spread_even(lo) | spread_even(lo >> 16) << 1;
+    const uint32_t high = spread_even(hi & 0xFFu) | spread_even((hi >> 16) & 0xFFu) << 1;
+    return (uint64_t)low | (uint64_t)high << 32;
 }
 
 // Grid y: the row; z: the pair. keep_census: also store both census images. COST_BLOCK
-// threads: 512 where every row of the launch is in flight at once (four blocks a CU), else
+// threads: 512 where every row of the launch is in flight at once (three blocks a CU), else
 // 1024 (launch_all).
 template <int COST_BLOCK, int D>
 __global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single,
@@ -92,7 +166,7 @@ __global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single
     const int y = blockIdx.y;
     uint64_t *cl = reinterpret_cast<uint64_t *>(lds);  // [left_slots(W)]
     uint64_t *cr = cl + left_slots(W);                 // [right_slots(W, D)]
-    const int row_pitch = W + 2 * CENSUS_REACH;        // padded row
+    const int row_pitch = census_row_pitch(W);         // padded row
     uint8_t *rows = reinterpret_cast<uint8_t *>(cr + right_slots(W, D));  // [2][7][row_pitch]
     // A thread loads its byte of all 14 rows before it stores any, so that the loads are in
     // flight together (a thread past the padded row loads a byte it does not store). Buffer
@@ -113,26 +187,23 @@ __global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single
     }
     for (int f = threadIdx.x; f < D; f += COST_BLOCK) cr[right_slot(f)] = 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < 2 * W; i += COST_BLOCK) {
-        const int side = i >= W ? 1 : 0, x = i - side * W;
-        const uint8_t *img = rows + side * 7 * row_pitch + CENSUS_REACH + x;  // column x
-        const int centre = img[3 * row_pitch];
-        uint64_t f = 0;
-        int bit = 0;
+    // Four pixels a thread (census_quad); pixels past the row's end land in slots that only
+    // pixels past it read.
+    const int quads = (W + 3) >> 2;
+    for (int i = threadIdx.x; i < 2 * quads; i += COST_BLOCK) {
+        const int side = i >= quads ? 1 : 0, x0 = 4 * (i - side * quads);
+        // Columns x0 - 4 .. x0 + 7: bytes x0 .. x0 + 11 of each padded row, dword aligned.
+        const uint32_t *img = reinterpret_cast<const uint32_t *>(rows + side * 7 * row_pitch + x0);
+        const CensusQuad q = census_quad(img, row_pitch >> 2);
 #pragma unroll
-        for (int dy = -3; dy <= 3; dy++) {
-            if (dy == 0) continue;
-            const uint8_t *row = img + (3 + dy) * row_pitch;
-#pragma unroll
-            for (int dx = -4; dx <= 4; dx++) {
-                if (dx == 0) continue;
-                f |= (uint64_t)((int)row[dx] + 1 < centre) << bit;
-                bit++;
-            }
+        for (int k = 0; k < 4; k++) {
+            const uint64_t f = (uint64_t)q.lo[k] | (uint64_t)q.hi[k] << 32;
+            if (side) cr[right_slot(D + x0 + k)] = f;
+            else cl[x0 + k] = f;
+            if (keep_census && x0 + k < W)
+                (side ? pd.b.census_right : pd.b.census_left)[(size_t)y * W + x0 + k] =
+                    census_bits_in_order(q.lo[k], q.hi[k]);
         }
-        if (side) cr[right_slot(D + x)] = f;
-        else cl[x] = f;
-        if (keep_census) (side ? pd.b.census_right : pd.b.census_left)[(size_t)y * W + x] = f;
     }
     __syncthreads();
     constexpr int chunks = D >> 4;
@@ -1737,10 +1808,11 @@ static hipError_t launch_all(const MatchParams &p, const PairDesc &single, const
         return e;
     }();
     if (lds_ok != hipSuccess) return lds_ok;
-    // 512-thread blocks when every row of the launch runs at once (four blocks a CU): the
-    // cost volume is then written in one sweep, and at 728x544x128, whose volumes fill the
-    // 256 MB Infinity Cache, the path kernel after it runs 8% faster (180 -> 165 us).
-    // Otherwise 1024 (at 1920x1080x256 the census and costs take 201 us so, 243 with 512).
+    // 512-thread blocks when every row of the launch runs at once (three blocks a CU at the
+    // kernel's 74 VGPRs): the cost volume is then written in one sweep, and at 728x544x128,
+    // whose volumes fill the 256 MB Infinity Cache, the path kernel after it runs 8% faster
+    // (180 -> 165 us). Otherwise 1024 (at 1920x1080x256 either takes about 150 us: the
+    // rows' blocks run in rounds, each round's stores draining before the next starts).
     static const int compute_units = [] {
         int device = 0, n = 0;
         if (hipGetDevice(&device) != hipSuccess ||
@@ -1748,7 +1820,7 @@ static hipError_t launch_all(const MatchParams &p, const PairDesc &single, const
             n = 0;
         return n > 0 ? n : 256;
     }();
-    const bool one_sweep = (size_t)H * count <= 4 * (size_t)compute_units;
+    const bool one_sweep = (size_t)H * count <= 3 * (size_t)compute_units;
     int slot = profile_begin(PK_COST, stream);
     if (one_sweep)
         launch_census_cost<512>(single, table, W, H, D, count, p.keep_stages ? 1 : 0, stream);
