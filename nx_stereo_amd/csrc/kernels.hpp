@@ -176,9 +176,8 @@ hipError_t launch_quality(const float *disparity, float *previous, int width, in
 // The per-pixel tail fused (stages.hip): an ordered run of SpeckleFilter (radius 3),
 // TemporalFilter, Depth, PointCloud (only after Depth in the run) and Quality from the map
 // `in`. `out` takes the run's map where SpeckleFilter or TemporalFilter is in it (in != out
-// with SpeckleFilter). `kept_accum` and `counter` start at zero (each launch leaves them
-// at zero); `totals` as launch_quality's partials. The launch's last block makes the
-// quality sums and the kept count.
+// with SpeckleFilter). `kept_accum` starts at zero (each finish leaves it at zero);
+// `totals` as launch_quality's partials.
 struct TailArgs {
     const float *in = nullptr;
     float *out = nullptr;
@@ -197,13 +196,14 @@ struct TailArgs {
     float f = 0, cx = 0, cy = 0, nearest = 0, farthest = 0;
     float *points = nullptr;
     int32_t *kept_accum = nullptr, *kept_total = nullptr;
-    unsigned *counter = nullptr;  // the blocks that have finished (with Quality or PointCloud)
     int quality = 0;
     float *q_prev = nullptr;
     double *totals = nullptr;
     float *sums5 = nullptr;
 };
 hipError_t launch_tail(const TailArgs &a, hipStream_t stream);
+// After launch_tail: the quality sums and the kept count (nothing when neither is in the run).
+hipError_t launch_tail_finish(const TailArgs &a, hipStream_t stream);
 
 // MedianFilter: kernel_size 3 or 5.
 hipError_t launch_median_filter(const float *in, float *out, int width, int height,

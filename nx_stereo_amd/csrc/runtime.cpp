@@ -609,8 +609,7 @@ struct Segment {
     float *disparity[2]{};
     uint16_t *depth_mm = nullptr;
     float *depth_m = nullptr, *cloud = nullptr, *quality_sums = nullptr;
-    // The fused tail's kept-point accumulator, then its finished-block counter (left at zero).
-    int32_t *tail_kept = nullptr;
+    int32_t *tail_kept = nullptr;  // the fused tail's kept-point accumulator (left at zero)
     uint64_t *sharpness_totals = nullptr, *alignment_totals = nullptr;
     float *sharpness_sums = nullptr, *alignment_sums = nullptr;
     double *quality_partials = nullptr;
@@ -1184,10 +1183,11 @@ static void enqueue_members(Segment &s, const std::vector<Inputs> &inputs, Enque
                     default: throw Error{"a member that cannot join a fused tail"};
                 }
             }
-            a.counter = reinterpret_cast<unsigned *>(s.tail_kept + 1);
             HIP_TRY(launch_tail(a, stream));
-            for (size_t k = i; k < i + run; k++)
+            for (size_t k = i; k + 1 < i + run; k++)
                 if (!events.empty()) HIP_TRY(hipEventRecord(events[k + 1], stream));
+            HIP_TRY(launch_tail_finish(a, stream));
+            if (!events.empty()) HIP_TRY(hipEventRecord(events[i + run], stream));
             for (size_t k = i; k < i + run; k++) note_outputs(s.members[k]);
             i += run - 1;
             continue;

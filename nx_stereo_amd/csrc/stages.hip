@@ -392,7 +392,7 @@ __global__ __launch_bounds__(QUALITY_COLS * QUALITY_PHASES) void quality_chunks_
 
 constexpr int QUALITY_FINISH_GROUPS = 64;  // groups of QUALITY_COLS: up to 4096 columns
 
-// One block (any whole number of waves): the column totals into the five sums.
+// One block of QUALITY_FINISH threads: the column totals into the five sums.
 __device__ void finish_quality(double *__restrict__ totals, int W, float *__restrict__ sums5) {
     static_assert(QUALITY_COLS == 64, "a group of columns is one wave's lanes");
     __shared__ double group_sum[5][QUALITY_FINISH_GROUPS];
@@ -400,7 +400,7 @@ __device__ void finish_quality(double *__restrict__ totals, int W, float *__rest
     const int wave = threadIdx.x / QUALITY_COLS, lane = threadIdx.x % QUALITY_COLS;
     // A wave per group: each column's total cast to f32 (a column past W is 0) and set back to
     // zero for the next launch, then the pairwise tree, lane i + half into lane i.
-    for (int g = wave; g < groups; g += (int)blockDim.x / QUALITY_COLS) {
+    for (int g = wave; g < groups; g += QUALITY_FINISH / QUALITY_COLS) {
         const int x = g * QUALITY_COLS + lane;
         double v[5];
 #pragma unroll
@@ -449,15 +449,13 @@ hipError_t launch_quality(const float *disparity, float *previous, int W, int H,
 // ------------------------------------------------------------------------------------------
 // The per-pixel tail, fused: an ordered run of SpeckleFilter (radius 3), TemporalFilter,
 // Depth, PointCloud and Quality in one launch over 64x16 tiles, each pixel through the run's
-// stages in turn with the very operations of the kernels above; the last block to finish
-// then does what needs every tile (the quality sums, the kept count). A tile's 512 threads
+// stages in turn with the very operations of the kernels above, then one block that
+// finishes what needs every tile (the quality sums, the kept count). A tile's 512 threads
 // take two rows each, 8 apart, as quality_chunks_kernel does, so each column's quality
 // sums gather the tile's 16 rows before one f64 atomic add per column and sum (exact in
-// any order, as there). At 728x544 the launch took 12.5 us against 22 for the four
-// launches it replaced (64x8 tiles, twice the blocks and the atomics: 15 us), plus 4.9 us
-// for a finishing launch of its own before the finish moved into its last block. The kept
-// count gathers by an integer atomic add into `kept_accum`, which the finishing block moves
-// into `kept_total` and sets back to zero.
+// any order, as there). At 728x544 the launch takes 12.5 us against 22 for the four
+// launches it replaces (64x8 tiles, twice the blocks and the atomics: 15 us). The kept count gathers by an integer atomic add into `kept_accum`,
+// which the finishing block moves into `kept_total` and sets back to zero.
 
 constexpr int TAIL_TX = 64, TAIL_PHASES = 8, TAIL_TY = 16, TAIL_R = 3;
 
@@ -573,15 +571,16 @@ __global__ __launch_bounds__(TAIL_TX * TAIL_PHASES) void tail_kernel(TailArgs a)
             atomicAdd(&a.totals[(size_t)k * W + x], sum);
         }
     }
-    if (!a.quality && !a.cloud) return;  // launch-uniform
-    // The finish, in the last block to get here: the quality sums (as quality_finish_kernel)
-    // and the kept count. Every wave's atomics are done before the block counts itself in.
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    if (!last_block(a.counter, gridDim.x * gridDim.y)) return;
-    if (a.quality) finish_quality(a.totals, W, a.sums5);
-    if (a.cloud && threadIdx.x == 0) {
-        *a.kept_total = *a.kept_accum;
-        *a.kept_accum = 0;
+}
+
+// The tail's finish: the quality sums (as quality_finish_kernel) and the kept count.
+__global__ __launch_bounds__(QUALITY_FINISH) void tail_finish_kernel(
+    double *__restrict__ totals, int W, float *__restrict__ sums5,
+    int32_t *__restrict__ kept_accum, int32_t *__restrict__ kept_total) {
+    if (totals) finish_quality(totals, W, sums5);
+    if (kept_accum && threadIdx.x == 0) {
+        *kept_total = *kept_accum;
+        *kept_accum = 0;
     }
 }
 
@@ -592,6 +591,13 @@ hipError_t launch_tail(const TailArgs &a, hipStream_t stream) {
     return hipGetLastError();
 }
 
+hipError_t launch_tail_finish(const TailArgs &a, hipStream_t stream) {
+    if (!a.quality && !a.cloud) return hipSuccess;
+    hipLaunchKernelGGL(tail_finish_kernel, dim3(1), dim3(QUALITY_FINISH), 0, stream,
+                       a.quality ? a.totals : nullptr, a.W, a.sums5,
+                       a.cloud ? a.kept_accum : nullptr, a.kept_total);
+    return hipGetLastError();
+}
 
 // ------------------------------------------------------------------------------------------
 // MedianFilter (filters.rs:134-216; image.ex:479-505): the middle of each pixel's k x k

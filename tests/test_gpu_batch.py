@@ -155,3 +155,24 @@ def test_meeting_by_recomputation_at_the_extremes(w, h, disparities, paths):
                 assert np.array_equal(sums, want), (w, h, disparities, paths, lane_count, prefetch)
     finally:
         _native.set_schedule(w, h, disparities, paths, *kept)
+
+
+def test_many_short_pairs_in_one_batch():
+    """Two classes of many pairs only 7 or 8 rows tall (the least the matcher takes): rows
+    clamped at both edges of every pair, and census launches of 630 and 640 rows over 90 and
+    80 pairs: every pair's disparity bit for bit."""
+    pairs, bufs = [], []
+    for k, (w, h) in enumerate([(41, 7)] * 90 + [(33, 8)] * 80):
+        left, right = scenes.scene(w, h, 100 + k)
+        pairs.append((left, right))
+        bufs.append((hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h)))
+    batch = _native.Batch([(l.ptr, r.ptr, o.ptr, left.shape[1], left.shape[0], 64, 8, 8, 64, 1.0526316)
+                           for (l, r, o), (left, _) in zip(bufs, pairs)], autotune=False)
+    stream = hip.Stream()
+    batch.run(stream.handle)
+    stream.synchronize()
+    for (left, right), (_l, _r, o) in zip(pairs, bufs):
+        got = o.download(left.shape, np.float32)
+        want = oracle.sgm(left, right, 64, 8, 8, 64, 1.0526316)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), left.shape
+    batch.close()

@@ -79,3 +79,19 @@ def test_maximum_width():
     left, right, _ = scenes.planar_pair(4096, 8, seed=9, low=4, high=50)
     got = _native.match_host(left, right, 64, 8)["disparity"]
     same_bits(got, oracle.sgm(left, right, 64, 8))
+
+
+@pytest.mark.parametrize("size,disparities", [((20, 600), 64), ((37, 530), 128), ((13, 1100), 192),
+                                              ((70, 520), 256)])
+def test_tall_narrow_images(size, disparities):
+    """Five hundred rows and more of narrow images (1024-thread census blocks, widths off a
+    multiple of four, fewer pixels than disparities): census, costs and the disparity bit for
+    bit."""
+    w, h = size
+    left, right = scenes.scene(w, h, 11)
+    got = _native.match_host(left, right, disparities, 8, intermediates=True)
+    fl, fr = oracle.census(left), oracle.census(right)
+    same_bits(got["census_left"], fl)
+    same_bits(got["census_right"], fr)
+    same_bits(got["costs"], oracle.costs(fl, fr, disparities))
+    same_bits(got["disparity"], oracle.sgm(left, right, disparities, 8))
