@@ -393,22 +393,30 @@ __global__ __launch_bounds__(BAND_PATH_BLOCK) void band_path_kernel(BandPathArgs
                 constexpr bool WHOLE = decltype(whole)::value;
                 const int bases_next = block_bases(t0 + 16);
                 const int changes = changes_of(t0);
-                uint64_t moves = __builtin_amdgcn_ballot_w64(changes != 0 && lane < 16);
-                int here = position(t0), there = position(min(t0 + PREFETCH, n - 1));
+                const uint64_t moves = __builtin_amdgcn_ballot_w64(changes != 0 && lane < 16);
+                // A block of steps where no line of the wave changes base (most of them) runs
+                // without the per-step test.
+                auto steps = [&](auto any) __attribute__((always_inline)) {
+                    constexpr bool ANY = decltype(any)::value;
+                    uint64_t m = moves;
+                    int here = position(t0), there = position(min(t0 + PREFETCH, n - 1));
 #pragma unroll
-                for (int p = 0; p < 16; p++) {
-                    const int t = t0 + p;
-                    __builtin_amdgcn_sched_barrier(0);
-                    const uint32_t c = cost_ring[p];
-                    cost_ring[p] = __builtin_amdgcn_raw_buffer_load_b8(costs, lane_cost, (uint32_t)there * B, 0);
-                    const bool moving_here = (moves & LINE_BITS) != 0;
-                    moves >>= 1;
-                    const uint32_t v = line.step(c, lane, group, p1, p2, moving_here, delta_at(changes, p));
-                    if (WHOLE || t < n)
-                        __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, out, lane_cost, (uint32_t)here * B, 0);
-                    here += move;
-                    there = WHOLE ? there + move : position(min(t + 1 + PREFETCH, n - 1));
-                }
+                    for (int p = 0; p < 16; p++) {
+                        const int t = t0 + p;
+                        __builtin_amdgcn_sched_barrier(0);
+                        const uint32_t c = cost_ring[p];
+                        cost_ring[p] = __builtin_amdgcn_raw_buffer_load_b8(costs, lane_cost, (uint32_t)there * B, 0);
+                        const bool moving_here = ANY && (m & LINE_BITS) != 0;
+                        m >>= 1;
+                        const uint32_t v = line.step(c, lane, group, p1, p2, moving_here, delta_at(changes, p));
+                        if (WHOLE || t < n)
+                            __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, out, lane_cost, (uint32_t)here * B, 0);
+                        here += move;
+                        there = WHOLE ? there + move : position(min(t + 1 + PREFETCH, n - 1));
+                    }
+                };
+                if (moves) steps(std::true_type{});
+                else steps(std::false_type{});
                 previous = bases_now;
                 bases_now = searched_of(bases_next);
             };
@@ -430,20 +438,26 @@ __global__ __launch_bounds__(BAND_PATH_BLOCK) void band_path_kernel(BandPathArgs
         for (int t0 = 0; t0 < longest; t0 += 16) {
             const int bases_next = block_bases(t0 + 16);
             const int changes = changes_of(t0);
-            uint64_t moves = __builtin_amdgcn_ballot_w64(changes != 0 && lane < 16);
+            const uint64_t moves = __builtin_amdgcn_ballot_w64(changes != 0 && lane < 16);
+            auto steps = [&](auto any) __attribute__((always_inline)) {
+                constexpr bool ANY = decltype(any)::value;
+                uint64_t m = moves;
 #pragma unroll
-            for (int p = 0; p < 16; p++) {
-                const int t = t0 + p;
-                __builtin_amdgcn_sched_barrier(0);
-                const uint32_t c = cost_ring[p];
-                cost_ring[p] = __builtin_amdgcn_raw_buffer_load_b8(costs, t + PREFETCH < L ? ahead : MASKED, 0, 0);
-                const bool moving_here = (moves & LINE_BITS) != 0;
-                moves >>= 1;
-                const uint32_t v = line.step(c, lane, group, p1, p2, moving_here, delta_at(changes, p));
-                __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, out, t < L ? at : MASKED, 0, 0);
-                at += step * B;
-                ahead += step * B;
-            }
+                for (int p = 0; p < 16; p++) {
+                    const int t = t0 + p;
+                    __builtin_amdgcn_sched_barrier(0);
+                    const uint32_t c = cost_ring[p];
+                    cost_ring[p] = __builtin_amdgcn_raw_buffer_load_b8(costs, t + PREFETCH < L ? ahead : MASKED, 0, 0);
+                    const bool moving_here = ANY && (m & LINE_BITS) != 0;
+                    m >>= 1;
+                    const uint32_t v = line.step(c, lane, group, p1, p2, moving_here, delta_at(changes, p));
+                    __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, out, t < L ? at : MASKED, 0, 0);
+                    at += step * B;
+                    ahead += step * B;
+                }
+            };
+            if (moves) steps(std::true_type{});
+            else steps(std::false_type{});
             previous = bases_now;
             bases_now = searched_of(bases_next);
         }
