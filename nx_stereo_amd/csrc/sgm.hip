@@ -105,34 +105,39 @@ __device__ __forceinline__ CensusQuad census_quad(const uint32_t *img, int pitch
     const uint32_t centres = img[3 * pitch_dwords + 1];
 #pragma unroll
     for (int k0 = 0; k0 < 4; k0 += 2) {
-    uint32_t Y[4];
-#pragma unroll
-    for (int k = k0; k < k0 + 2; k++) {
-        const uint32_t half = ((centres >> (8 * k)) & 0xFFu) + 0x7FFEu;
-        Y[k] = half | (half << 16);
-        q.lo[k] = q.hi[k] = 0;
-    }
-#pragma unroll
-    for (int rank = 0; rank < 6; rank++) {
-        const int r = rank < 3 ? rank : rank + 1;
-        const uint32_t w0 = img[r * pitch_dwords], w1 = img[r * pitch_dwords + 1],
-                       w2 = img[r * pitch_dwords + 2];
+        uint32_t Y[4];
 #pragma unroll
         for (int k = k0; k < k0 + 2; k++) {
+            const uint32_t half = ((centres >> (8 * k)) & 0xFFu) + 0x7FFEu;
+            Y[k] = half | (half << 16);
+            q.lo[k] = q.hi[k] = 0;
+        }
 #pragma unroll
-            for (int pair = 0; pair < 4; pair++) {
-                // Bytes first and first + 1 of the 12 bytes w0 w1 w2.
-                const int first = k + (pair < 2 ? 2 * pair : 2 * pair + 1);
-                const uint32_t X = first + 1 < 8
-                    ? __builtin_amdgcn_perm(w1, w0, (uint32_t)first | 0x0C00u | (uint32_t)(first + 1) << 16 | 0x0C000000u)
-                    : __builtin_amdgcn_perm(w2, w1, (uint32_t)(first - 4) | 0x0C00u | (uint32_t)(first - 3) << 16 | 0x0C000000u);
-                const uint32_t z = pk_sub(Y[k], X);
-                const int c = rank * 4 + pair;
-                if (c < 16) q.lo[k] |= (z >> (15 - c)) & (0x00010001u << c);
-                else q.hi[k] |= (z >> (31 - c)) & (0x00010001u << (c - 16));
+        for (int rank = 0; rank < 6; rank++) {
+            const int r = rank < 3 ? rank : rank + 1;
+            const uint32_t w0 = img[r * pitch_dwords], w1 = img[r * pitch_dwords + 1],
+                           w2 = img[r * pitch_dwords + 2];
+#pragma unroll
+            for (int k = k0; k < k0 + 2; k++) {
+#pragma unroll
+                for (int pair = 0; pair < 4; pair++) {
+                    // Bytes first and first + 1 of the 12 bytes w0 w1 w2.
+                    const int first = k + (pair < 2 ? 2 * pair : 2 * pair + 1);
+                    const uint32_t X =
+                        first + 1 < 8
+                            ? __builtin_amdgcn_perm(w1, w0, (uint32_t)first | 0x0C00u |
+                                                                (uint32_t)(first + 1) << 16 |
+                                                                0x0C000000u)
+                            : __builtin_amdgcn_perm(w2, w1, (uint32_t)(first - 4) | 0x0C00u |
+                                                                (uint32_t)(first - 3) << 16 |
+                                                                0x0C000000u);
+                    const uint32_t z = pk_sub(Y[k], X);
+                    const int c = rank * 4 + pair;
+                    if (c < 16) q.lo[k] |= (z >> (15 - c)) & (0x00010001u << c);
+                    else q.hi[k] |= (z >> (31 - c)) & (0x00010001u << (c - 16));
+                }
             }
         }
-    }
     }
     return q;
 }
