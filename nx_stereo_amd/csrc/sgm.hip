@@ -1301,7 +1301,8 @@ __global__ __launch_bounds__(128 * G) void path_sliced_kernel(PathArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Selection (reference.rs:148-186; nx_stereo.ex:608-641), 16 lanes per pixel.
+// Selection (reference.rs:148-186; nx_stereo.ex:608-641), a pixel's disparities over 4, 8 or
+// 16 lanes.
 //
 // V[d] = S[d] + 0x8000 for d > x. The winner is the lowest d of the least V; d = 0 is never
 // past the left edge and S <= 1016, so the winner is never a penalised d, and (V << 4 | j)
@@ -1310,7 +1311,7 @@ __global__ __launch_bounds__(128 * G) void path_sliced_kernel(PathArgs a) {
 // least V outside [below, above]: those entries are lifted to 0xFFFD..0xFFFF, above every
 // V (at most 0x83F8). The winner's two neighbours come from the lanes holding them.
 
-// A block takes SELECT_GROUPS groups of 16 pixels of a row, one after another, loading the
+// A block takes SELECT_GROUPS groups of 256 / PL pixels of a row, one after another, loading the
 // next group's four volumes while it selects in the current one (so that each wave keeps
 // loads in flight).
 constexpr int SELECT_GROUPS = 2;
@@ -1318,10 +1319,43 @@ constexpr int SELECT_GROUPS = 2;
 // STREAM: the volumes are read with non-temporal loads, which at sizes whose volumes
 // overflow the Infinity Cache keep them from displacing what later launches reuse
 // (1920x1080x256: 391 -> 347 us; at 728x544x128, whose volumes it holds, plain loads).
-template <int K, bool STREAM>
+// The minimum over the PL lanes of a pixel (PL = 4, 8 or 16 consecutive lanes of a DPP
+// row), left in each of them: quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror, then the
+// row's two halves.
+template <int PL>
+__device__ __forceinline__ uint32_t pixel_min_u32(uint32_t v) {
+    if constexpr (PL == 16) {
+        return row_min_u32(v);
+    } else {
+        v = min(v, dpp<0xB1>(v));
+        v = min(v, dpp<0x4E>(v));
+        if constexpr (PL == 8) v = min(v, dpp<0x141>(v));
+        return v;
+    }
+}
+template <int PL>
+__device__ __forceinline__ uint32_t pixel_min_pk(uint32_t v) {
+    if constexpr (PL == 16) {
+        return row_min_pk(v);
+    } else {
+        v = pk_min(v, dpp<0xB1>(v));
+        v = pk_min(v, dpp<0x4E>(v));
+        if constexpr (PL == 8) v = pk_min(v, dpp<0x141>(v));
+        return v;
+    }
+}
+
+// K disparities a lane, PL lanes a pixel (D = PL K): 16 a lane where D allows (64, 128 and
+// 256 disparities on 4, 8 and 16 lanes), so that a wave selects more pixels at once and the
+// per-pixel work (the reductions over the pixel's lanes, the subpixel step) is shared by
+// fewer lanes; 192 disparities take 12 a lane on 16 lanes.
+template <int K, int PL, bool STREAM>
 __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const PairDesc *table,
                                                      int orientations, int W) {
-    constexpr int D = 16 * K, R = K / 2;
+    static_assert(K <= 16, "a key keeps the lane's index in 4 bits");
+    static_assert(PL == 4 || PL == 8 || PL == 16, "a pixel's lanes lie in one DPP row");
+    constexpr int D = PL * K, R = K / 2;
+    constexpr int PIXELS = 256 / PL;  // a block's pixels per group
     const PairDesc pd = pair_of(single, table, blockIdx.z);
     const uint32_t volume = (uint32_t)W * gridDim.y * D;
     const Rsrc vol[4] = {buffer_of(pd.b.paired[0], volume), buffer_of(pd.b.paired[1], volume),
@@ -1330,13 +1364,14 @@ __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const Pair
     float *__restrict__ selected = pd.b.selected;
     uint32_t *__restrict__ keys = pd.b.keys;
     uint16_t *__restrict__ sums_out = pd.sums;
-    const int lane16 = threadIdx.x & 15;
-    const int x_first = blockIdx.x * 16 * SELECT_GROUPS + (threadIdx.x >> 4), y = blockIdx.y;
-    const int d0 = lane16 * K;
+    const int lane = threadIdx.x & (PL - 1);  // within the pixel's lanes
+    const int x_first = blockIdx.x * PIXELS * SELECT_GROUPS + (int)(threadIdx.x / PL),
+              y = blockIdx.y;
+    const int d0 = lane * K;
     // This lane's bytes of group g's pixel (the last pixel of the row past its end).
     auto offset_of = [&](int g) {
-        const uint32_t pix = (uint32_t)y * W + min(x_first + 16 * g, W - 1);
-        return pix * D + (uint32_t)lane16 * K;  // the volume is under 2^31 bytes
+        const uint32_t pix = (uint32_t)y * W + min(x_first + PIXELS * g, W - 1);
+        return pix * D + (uint32_t)lane * K;  // the volume is under 2^31 bytes
     };
     Bytes<K> cur[4], next[4];
     auto load = [&](Bytes<K>(&b)[4], int g) __attribute__((always_inline)) {
@@ -1349,7 +1384,7 @@ __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const Pair
 #pragma unroll
     for (int g = 0; g < SELECT_GROUPS; g++) {
         if (g + 1 < SELECT_GROUPS) load(next, g + 1);
-        const int x = x_first + 16 * g;
+        const int x = x_first + PIXELS * g;
         const bool valid = x < W;
         const uint32_t at = offset_of(g), pix = (uint32_t)y * W + min(x, W - 1);
         uint32_t sum[R];
@@ -1397,7 +1432,7 @@ __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const Pair
 #pragma unroll
         for (int r = 1; r < R; r++) kmin = pk_min(kmin, key[r]);
         const uint32_t k16 = pk_min_halves(kmin) & 0xFFFFu;
-        const uint32_t best = row_min_u32((k16 >> 4) << 8 | (uint32_t)(d0 + (int)(k16 & 15u)));
+        const uint32_t best = pixel_min_u32<PL>((k16 >> 4) << 8 | (uint32_t)(d0 + (int)(k16 & 15u)));
         const uint32_t minimum = best >> 8;
         const int winner = (int)(best & 0xFFu);
         const int below = max(winner - 1, 0), above = min(winner + 1, D - 1);
@@ -1412,7 +1447,7 @@ __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const Pair
             const uint32_t lifted = pk_sub(pk_min(pk_sub(pair_index(r), T2), S2), S2);
             smin = pk_min(smin, pk_max(value[r], lifted));
         }
-        const uint32_t second = row_min_u32(pk_min_halves(smin)) & 0xFFFFu;
+        const uint32_t second = pixel_min_u32<PL>(pk_min_halves(smin)) & 0xFFFFu;
         // The winner's two neighbours' sums, from the lanes that hold them: the owner of
         // d = below gives S[below] in the low half (others 0xFFFF), of above in the high.
         const int jb = below - d0, ja = above - d0;
@@ -1424,9 +1459,9 @@ __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const Pair
         }
         const uint32_t sb = (jb & 1 ? wb >> 16 : wb) & 0xFFFFu;
         const uint32_t sa = (ja & 1 ? wa >> 16 : wa) & 0xFFFFu;
-        const uint32_t near = row_min_pk(((uint32_t)jb < (uint32_t)K ? sb : 0xFFFFu) |
+        const uint32_t near = pixel_min_pk<PL>(((uint32_t)jb < (uint32_t)K ? sb : 0xFFFFu) |
                                          ((uint32_t)ja < (uint32_t)K ? sa : 0xFFFFu) << 16);
-        if (lane16 == 0 && valid) {
+        if (lane == 0 && valid) {
             const uint32_t lower = (near & 0xFFFFu) + (x < below ? 0x8000u : 0u);
             const uint32_t upper = (near >> 16) + (x < above ? 0x8000u : 0u);
             const float lower_f = (float)lower;
@@ -1764,14 +1799,16 @@ static hipError_t launch_paths_and_select(const MatchParams &p, const PairDesc &
     }
     profile_end(slot, stream);
     slot = profile_begin(PK_SELECT, stream);
-    const int select_pixels = 16 * SELECT_GROUPS;
+    // 16 disparities a select lane where D allows (K = D / 16 lanes a pixel), else 12.
+    constexpr int SK = K == 12 ? 12 : 16, SPL = 16 * K / SK;
+    const int select_pixels = 256 / SPL * SELECT_GROUPS;
     const dim3 select_grid((W + select_pixels - 1) / select_pixels, H, count);
     if (stream_volumes)
-        hipLaunchKernelGGL((select_kernel<K, true>), select_grid, dim3(256), 0, stream, single,
-                           table, n, W);
+        hipLaunchKernelGGL((select_kernel<SK, SPL, true>), select_grid, dim3(256), 0, stream,
+                           single, table, n, W);
     else
-        hipLaunchKernelGGL((select_kernel<K, false>), select_grid, dim3(256), 0, stream, single,
-                           table, n, W);
+        hipLaunchKernelGGL((select_kernel<SK, SPL, false>), select_grid, dim3(256), 0, stream,
+                           single, table, n, W);
     profile_end(slot, stream);
     return hipGetLastError();
 }
