@@ -84,12 +84,14 @@ __global__ __launch_bounds__(256) void band_base_kernel(const float *coarse, int
 
 constexpr int BAND_COST_BLOCK = 512;
 
-// The image rows in LDS carry BAND_REACH replicated pixels on each side, so the census reads
-// its 9 columns without clamping each one.
-constexpr int BAND_REACH = 4;
+// The image rows in LDS carry CENSUS_REACH replicated pixels on each side, so the census
+// reads its 9 columns without clamping each one, padded as the full search's census rows
+// (census_row_pitch), so that sparse_9x7 takes its four-pixel packed form (census_quad).
 
+// LDS bytes: the two census rows (a whole number of four-pixel groups each), then 7 padded
+// rows of each image.
 __host__ __device__ constexpr size_t band_cost_lds(int W) {
-    return 16 * (size_t)W + 14 * (size_t)(W + 2 * BAND_REACH);
+    return 16 * (size_t)((W + 3) & ~3) + 14 * (size_t)census_row_pitch(W);
 }
 
 // Census of the pixel at `img`, in the middle of 7 padded rows `pitch` apart. WINDOW 0:
@@ -133,10 +135,11 @@ __global__ __launch_bounds__(BAND_COST_BLOCK) void band_cost_kernel(
     const uint8_t *left, const uint8_t *right, const int32_t *base, int W, int H, uint8_t *costs,
     uint64_t *census_left, uint64_t *census_right) {
     extern __shared__ __align__(16) uint8_t band_lds[];
-    uint64_t *cl = reinterpret_cast<uint64_t *>(band_lds);  // [W]
-    uint64_t *cr = cl + W;                                    // [W]
-    const int pitch = W + 2 * BAND_REACH;
-    uint8_t *rows = reinterpret_cast<uint8_t *>(cr + W);      // [2][7][pitch]
+    const int W4 = (W + 3) & ~3;
+    uint64_t *cl = reinterpret_cast<uint64_t *>(band_lds);  // [W4]
+    uint64_t *cr = cl + W4;                                   // [W4]
+    const int pitch = census_row_pitch(W);
+    uint8_t *rows = reinterpret_cast<uint8_t *>(cr + W4);     // [2][7][pitch]
     const int y = blockIdx.x;
     // A thread loads its byte of all 14 rows before it stores any, so that the loads are in
     // flight together (a thread past the padded row loads a byte it does not store); the
@@ -144,7 +147,7 @@ __global__ __launch_bounds__(BAND_COST_BLOCK) void band_cost_kernel(
     const Rsrc images[2] = {buffer_of(left, (uint32_t)W * H), buffer_of(right, (uint32_t)W * H)};
     for (int i0 = 0; i0 < pitch; i0 += BAND_COST_BLOCK) {
         const int i = i0 + (int)threadIdx.x;
-        const int column = min(max(i - BAND_REACH, 0), W - 1);
+        const int column = min(max(i - CENSUS_REACH, 0), W - 1);
         uint32_t v[14];
 #pragma unroll
         for (int k = 0; k < 14; k++)
@@ -156,11 +159,29 @@ __global__ __launch_bounds__(BAND_COST_BLOCK) void band_cost_kernel(
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < 2 * W; i += BAND_COST_BLOCK) {
-        const int side = i >= W ? 1 : 0, x = i - side * W;
-        const uint64_t f = census_at<WINDOW>(rows + side * 7 * pitch + BAND_REACH + x, pitch);
-        (side ? cr : cl)[x] = f;
-        if (census_left) (side ? census_right : census_left)[(size_t)y * W + x] = f;
+    if constexpr (WINDOW == 0) {
+        // Four pixels a thread in packed compares; pixels past the row's end land in the
+        // groups' spare slots, which nothing reads.
+        const int quads = W4 >> 2;
+        for (int i = threadIdx.x; i < 2 * quads; i += BAND_COST_BLOCK) {
+            const int side = i >= quads ? 1 : 0, x0 = 4 * (i - side * quads);
+            const uint32_t *img = reinterpret_cast<const uint32_t *>(rows + side * 7 * pitch + x0);
+            const CensusQuad q = census_quad(img, pitch >> 2);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                (side ? cr : cl)[x0 + k] = (uint64_t)q.lo[k] | (uint64_t)q.hi[k] << 32;
+                if (census_left && x0 + k < W)
+                    (side ? census_right : census_left)[(size_t)y * W + x0 + k] =
+                        census_bits_in_order(q.lo[k], q.hi[k]);
+            }
+        }
+    } else {
+        for (int i = threadIdx.x; i < 2 * W; i += BAND_COST_BLOCK) {
+            const int side = i >= W ? 1 : 0, x = i - side * W;
+            const uint64_t f = census_at<WINDOW>(rows + side * 7 * pitch + CENSUS_REACH + x, pitch);
+            (side ? cr : cl)[x] = f;
+            if (census_left) (side ? census_right : census_left)[(size_t)y * W + x] = f;
+        }
     }
     __syncthreads();
     constexpr int CHUNKS = B >= 16 ? B / 16 : 1;

@@ -1,5 +1,5 @@
 // Device-side helpers shared by the gfx950 kernels: packed u16 arithmetic,
-// DPP row exchanges and reductions over a 16-lane DPP row.
+// DPP row exchanges and reductions over a 16-lane DPP row, and the 9x7 census.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -139,6 +139,94 @@ __device__ __forceinline__ LineGeometry line_geometry(int o, int i, int W, int H
         g.length = i < W + H - 1 ? min(H - g.y0, g.x0 + 1) : 0;
     }
     return g;
+}
+
+
+// ------------------------------------------------------------------------------------------
+// The 9x7 census (sparse_9x7: dy in -3..3, dx in -4..4, both non-zero, 48 bits), shared by
+// the full search's census and cost kernel and the band search's.
+
+// The image rows in LDS carry CENSUS_REACH replicated pixels on each side, so the census
+// reads its 9 columns without clamping each one.
+constexpr int CENSUS_REACH = 4;
+
+// An image row in LDS: CENSUS_REACH replicated pixels, the row, the right edge replicated
+// on to a whole dword past the last census group's reads (bytes x0 .. x0 + 11 of the group
+// at x0 = 4 ((W + 3) / 4 - 1)).
+__host__ __device__ constexpr int census_row_pitch(int W) { return (W + 2 * CENSUS_REACH + 7) & ~3; }
+
+// The census of four neighbouring pixels x0 .. x0 + 3 of one image row, from the dwords of
+// the 7 padded rows (row r = dy + 3, `pitch_dwords` apart) that hold columns x0 - 4 .. x0 + 7.
+// Neighbours are compared two at a time, in the halves of a packed u16: with the
+// neighbours a, b as (a, b) and the centre c as (0x7FFE + c) in both halves, bit 15 of a
+// half of the difference is a + 1 < c. Comparison k (0..23: row rank 0..5 over dy = -3..3
+// but 0, then the column pairs (-4, -3), (-2, -1), (1, 2), (3, 4)) leaves its two bits at
+// bits k and 16 + k of the low word (k < 16), or k - 16 and k of the high word: an order of
+// the 48 bits of its own, the same for both images, which the Hamming costs do not see.
+// census_bits_in_order puts them in the window's row-major order.
+struct CensusQuad {
+    uint32_t lo[4], hi[4];
+};
+
+// Two pixels at a time, row by row, so that few registers are live (the rows' dwords are
+// read once for each two).
+__device__ __forceinline__ CensusQuad census_quad(const uint32_t *img, int pitch_dwords) {
+    CensusQuad q;
+    const uint32_t centres = img[3 * pitch_dwords + 1];
+#pragma unroll
+    for (int k0 = 0; k0 < 4; k0 += 2) {
+        uint32_t Y[4];
+#pragma unroll
+        for (int k = k0; k < k0 + 2; k++) {
+            const uint32_t half = ((centres >> (8 * k)) & 0xFFu) + 0x7FFEu;
+            Y[k] = half | (half << 16);
+            q.lo[k] = q.hi[k] = 0;
+        }
+#pragma unroll
+        for (int rank = 0; rank < 6; rank++) {
+            const int r = rank < 3 ? rank : rank + 1;
+            const uint32_t w0 = img[r * pitch_dwords], w1 = img[r * pitch_dwords + 1],
+                           w2 = img[r * pitch_dwords + 2];
+#pragma unroll
+            for (int k = k0; k < k0 + 2; k++) {
+#pragma unroll
+                for (int pair = 0; pair < 4; pair++) {
+                    // Bytes first and first + 1 of the 12 bytes w0 w1 w2.
+                    const int first = k + (pair < 2 ? 2 * pair : 2 * pair + 1);
+                    const uint32_t X =
+                        first + 1 < 8
+                            ? __builtin_amdgcn_perm(w1, w0, (uint32_t)first | 0x0C00u |
+                                                                (uint32_t)(first + 1) << 16 |
+                                                                0x0C000000u)
+                            : __builtin_amdgcn_perm(w2, w1, (uint32_t)(first - 4) | 0x0C00u |
+                                                                (uint32_t)(first - 3) << 16 |
+                                                                0x0C000000u);
+                    const uint32_t z = pk_sub(Y[k], X);
+                    const int c = rank * 4 + pair;
+                    if (c < 16) q.lo[k] |= (z >> (15 - c)) & (0x00010001u << c);
+                    else q.hi[k] |= (z >> (31 - c)) & (0x00010001u << (c - 16));
+                }
+            }
+        }
+    }
+    return q;
+}
+
+// Bits 0..15 of x to the even bits 0..30.
+__device__ __forceinline__ uint32_t spread_even(uint32_t x) {
+    x &= 0xFFFFu;
+    x = (x | x << 8) & 0x00FF00FFu;
+    x = (x | x << 4) & 0x0F0F0F0Fu;
+    x = (x | x << 2) & 0x33333333u;
+    return (x | x << 1) & 0x55555555u;
+}
+
+// A census in census_quad's order to the window's row-major order (bit 2k and 2k + 1 are
+// comparison k's two).
+__device__ __forceinline__ uint64_t census_bits_in_order(uint32_t lo, uint32_t hi) {
+    const uint32_t low = spread_even(lo) | spread_even(lo >> 16) << 1;
+    const uint32_t high = spread_even(hi & 0xFFu) | spread_even((hi >> 16) & 0xFFu) << 1;
+    return (uint64_t)low | (uint64_t)high << 32;
 }
 
 }  // namespace nxs
