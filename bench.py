@@ -374,19 +374,24 @@ def batched_leg(replays: int = 5) -> dict:
             l, r, o = hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h)
             buffers += [l, r, o]
             pairs.append((l.ptr, r.ptr, o.ptr, w, h, d, 8, 8, 64, 1.0526315789473684))
-    batch = _native.Batch(pairs, autotune=True)
     stream = hip.Stream()
-    batch.run(stream.handle)
-    stream.synchronize()
     e0, e1 = hip.Event(), hip.Event()
-    e0.record(stream)
-    for _ in range(replays):
+
+    def timed(batch):
         batch.run(stream.handle)
-    e1.record(stream)
-    e1.synchronize()
-    batched_ms = e0.elapsed_ms(e1) / replays
-    describe = batch.describe()
-    batch.close()
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(replays):
+            batch.run(stream.handle)
+        e1.record(stream)
+        e1.synchronize()
+        describe = batch.describe()
+        batch.close()
+        return e0.elapsed_ms(e1) / replays, describe
+
+    batched_ms, describe = timed(_native.Batch(pairs, autotune=True))
+    # Every class in one launch per kernel, untuned (the tuned plans above include it).
+    fused_ms, _ = timed(_native.Batch(pairs, autotune=False, fused=True))
 
     # Plain: one nxs_match_device per pair, one workspace per size reused pair after pair.
     spaces = {}
@@ -418,6 +423,7 @@ def batched_leg(replays: int = 5) -> dict:
         "batched_ms": round(batched_ms, 3), "batched_pairs_per_s": round(len(pairs) / batched_ms * 1e3, 2),
         "plain_ms": round(plain_ms, 3), "plain_pairs_per_s": round(len(pairs) / plain_ms * 1e3, 2),
         "speedup": round(plain_ms / batched_ms, 3),
+        "fused_ms": round(fused_ms, 3), "fused_pairs_per_s": round(len(pairs) / fused_ms * 1e3, 2),
         "schedules": describe,
     }
 

@@ -186,17 +186,31 @@ int nxs_coarse_to_fine_host(const uint8_t *left, const uint8_t *right, int width
  *
  * Many pairs of uneven sizes and disparity ranges through the matcher
  * (as nxs_match_device) in one graph: pairs are grouped into size classes
- * (width, height, disparities, paths); each kernel is one launch per class
- * over a descriptor table in device memory; the whole batch is captured
- * once into a hipGraph that nxs_batch_run replays. With `autotune`, each
- * class's aggregation schedule (prefetch depth 2/4/8, lanes per line 16/32/64,
- * one or two waves per line, the two waves meeting through memory or by
- * recomputation, orientations in one launch or one each) is timed on the
- * batch itself and the fastest kept for the batch (single-pair launches of
- * that size keep their own schedule). The batch owns one
- * matcher workspace per pair; the pairs' images and outputs stay the
- * caller's, read and written at every replay. No counterpart in the
- * reference, whose matcher takes one pair at a time. */
+ * (width, height, disparities, paths) and read their settings from a
+ * descriptor table in device memory; the whole batch is captured once into
+ * a hipGraph that nxs_batch_run replays. `mode`:
+ *   NXS_BATCH_DEFAULT: each kernel one launch per class, with the class
+ *     size's own aggregation schedule, the classes on two parallel branches
+ *     of the graph;
+ *   NXS_BATCH_TUNED: each class's aggregation schedule (prefetch depth
+ *     2/4/8, lanes per line 16/32/64, one or two waves per line, the two
+ *     waves meeting through memory or by recomputation, orientations in one
+ *     launch or one each) is timed on the batch itself and the fastest kept
+ *     for the batch (single-pair launches of that size keep their own
+ *     schedule); then the plans (one, two or four branches, each class in
+ *     one launch or in launches of 8 or 2 pairs, and the fused plan below
+ *     with checkpoints every 4 or 8 steps) are timed and the fastest kept;
+ *   NXS_BATCH_FUSED: every class in one launch per kernel (census and
+ *     costs, aggregation, selection, check and median: four launches for the
+ *     batch), the aggregation as two waves of 16 lanes a line meeting by
+ *     recomputation from checkpoints every 8 steps; up to 8 size classes.
+ * The results are the same in every mode. The batch owns one matcher
+ * workspace per pair; the pairs' images and outputs stay the caller's, read
+ * and written at every replay. No counterpart in the reference, whose
+ * matcher takes one pair at a time. */
+#define NXS_BATCH_DEFAULT 0
+#define NXS_BATCH_TUNED 1
+#define NXS_BATCH_FUSED 2
 typedef struct nxs_pair {
     const uint8_t *left, *right; /* device, u8 {height, width} */
     float *disparity;            /* device, f32 {height, width} */
@@ -234,7 +248,7 @@ typedef struct nxs_schedule {
 int nxs_schedule_set(int width, int height, int disparities, int paths, const nxs_schedule *s);
 int nxs_schedule_get(int width, int height, int disparities, int paths, nxs_schedule *s);
 
-int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch **out);
+int nxs_batch_create(const nxs_pair *pairs, int count, int mode, nxs_batch **out);
 int nxs_batch_run(nxs_batch *batch, void *stream);
 /* The classes and their schedules, one line each; valid while the batch lives. */
 const char *nxs_batch_describe(nxs_batch *batch);

@@ -317,17 +317,23 @@ def get_schedule(width, height, disparities, paths) -> tuple[int, bool, int, int
     return s.prefetch, bool(s.split), s.lanes, s.waves, s.meet
 
 
+BATCH_DEFAULT, BATCH_TUNED, BATCH_FUSED = 0, 1, 2  # nxs_batch_create's modes
+
+
 class Batch:
     """nxs_batch: pairs (left, right, out device addresses; width, height, disparities,
-    paths, p1, p2, uniqueness) captured into one graph; run() replays it on a stream."""
+    paths, p1, p2, uniqueness) captured into one graph; run() replays it on a stream.
+    autotune: the class schedules and plans timed on the batch (NXS_BATCH_TUNED); fused:
+    every class in one launch per kernel (NXS_BATCH_FUSED)."""
 
-    def __init__(self, pairs, autotune: bool = True):
+    def __init__(self, pairs, autotune: bool = True, fused: bool = False):
         lib = load()
         arr = (NxsPair * len(pairs))()
         for k, (left, right, out, w, h, d, paths, p1, p2, u) in enumerate(pairs):
             arr[k] = NxsPair(left, right, out, w, h, d, paths, p1, p2, float(np.float32(u)))
         handle = ctypes.c_void_p()
-        _check(lib.nxs_batch_create(arr, len(pairs), int(autotune), ctypes.byref(handle)),
+        mode = BATCH_FUSED if fused else (BATCH_TUNED if autotune else BATCH_DEFAULT)
+        _check(lib.nxs_batch_create(arr, len(pairs), mode, ctypes.byref(handle)),
                "nxs_batch_create")
         self.lib, self.handle = lib, handle
 

@@ -84,6 +84,21 @@ hipError_t launch_match(const MatchParams &p, const uint8_t *left, const uint8_t
 hipError_t launch_match_batch(const MatchParams &p, const PairDesc *table, int count,
                               const Schedule &schedule, hipStream_t stream);
 
+// One size class of a fused batch launch: its settings and its pairs table[first ..
+// first+count).
+struct MixedMember {
+    MatchParams p;
+    int first, count;
+};
+constexpr int MAX_MIXED_CLASSES = 8;
+
+// Every class of a batch in one launch per kernel (census and costs, the aggregation,
+// selection, check and median), the aggregation as two waves of 16 lanes a line meeting by
+// recomputation every `segment` (4 or 8) steps: up to MAX_MIXED_CLASSES classes, whose
+// keep_stages and positive are the first class's.
+hipError_t launch_match_mixed(const MixedMember *classes, int n, const PairDesc *table,
+                              int segment, hipStream_t stream);
+
 // The left-right check and 3x3 median of a selected map and its keys (the last kernel of
 // every matching); `checked` receives the checked map when keep_checked.
 hipError_t launch_check_median(const float *selected, const uint32_t *keys, float *checked,

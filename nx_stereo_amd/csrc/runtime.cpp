@@ -2598,10 +2598,13 @@ struct BatchClass {
 
 // How the batch spreads over the device: each class's pairs go in launches of `chunk` pairs
 // (0: the whole class in one), and the launches are dealt to `branches` concurrent streams
-// (parallel branches of the captured graph), the largest first to the least loaded.
+// (parallel branches of the captured graph), the largest first to the least loaded. Fused
+// (4 or 8): every class in one launch per kernel (launch_match_mixed), the aggregation
+// meeting by recomputation every `fused` steps, whatever the classes' own schedules.
 struct BatchPlan {
     int branches = 1;
     int chunk = 0;
+    int fused = 0;
 };
 
 // One launch chain of the batch: pairs tables[first .. first+count) of class `cls`.
@@ -2610,6 +2613,14 @@ struct BatchLaunch {
 };
 
 constexpr int MAX_BRANCHES = 4;
+
+inline std::string plan_text(const BatchPlan &p) {
+    if (p.fused)
+        return "every class in one launch per kernel, checkpoints every " +
+               std::to_string(p.fused) + " steps";
+    return std::to_string(p.branches) + " branches, " +
+           (p.chunk ? std::to_string(p.chunk) + " pairs a launch" : std::string("a class a launch"));
+}
 
 }  // namespace nxs
 
@@ -2675,6 +2686,12 @@ struct nxs_batch {
     void enqueue_all(hipStream_t s, const nxs::BatchPlan &p,
                      const std::vector<nxs::BatchLaunch> &ls) {
         using namespace nxs;
+        if (p.fused) {
+            std::vector<MixedMember> members;
+            for (const auto &c : classes) members.push_back(MixedMember{c.p, c.first, c.count});
+            HIP_TRY(launch_match_mixed(members.data(), (int)members.size(), tables, p.fused, s));
+            return;
+        }
         if (p.branches <= 1) {
             for (const auto &l : ls)
                 HIP_TRY(launch_match_batch(classes[l.cls].p, tables + l.first, l.count,
@@ -2699,25 +2716,26 @@ struct nxs_batch {
         HIP_TRY(hipEventCreate(&start));
         HIP_TRY(hipEventCreate(&stop));
         float best = 1e30f;
+        std::vector<BatchPlan> plans;
         for (int branches : {1, 2, 4})
-            for (int chunk : {0, 8, 2}) {
-                const BatchPlan p{branches, chunk};
-                const std::vector<BatchLaunch> ls = deal(p);
-                enqueue_all(stream, p, ls);
-                HIP_TRY(hipEventRecord(start, stream));
-                for (int r = 0; r < 3; r++) enqueue_all(stream, p, ls);
-                HIP_TRY(hipEventRecord(stop, stream));
-                HIP_TRY(hipEventSynchronize(stop));
-                float ms = 0;
-                HIP_TRY(hipEventElapsedTime(&ms, start, stop));
-                trials += "  batch: " + std::to_string(branches) + " branches, " +
-                          (chunk ? std::to_string(chunk) + " pairs a launch" : std::string("a class a launch")) +
-                          ": " + std::to_string(ms / 3) + " ms\n";
-                if (ms < best) {
-                    best = ms;
-                    plan = p;
-                }
+            for (int chunk : {0, 8, 2}) plans.push_back(BatchPlan{branches, chunk, 0});
+        if (classes.size() <= (size_t)MAX_MIXED_CLASSES)
+            for (int segment : {4, 8}) plans.push_back(BatchPlan{1, 0, segment});
+        for (const BatchPlan &p : plans) {
+            const std::vector<BatchLaunch> ls = deal(p);
+            enqueue_all(stream, p, ls);
+            HIP_TRY(hipEventRecord(start, stream));
+            for (int r = 0; r < 3; r++) enqueue_all(stream, p, ls);
+            HIP_TRY(hipEventRecord(stop, stream));
+            HIP_TRY(hipEventSynchronize(stop));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, start, stop));
+            trials += "  batch: " + plan_text(p) + ": " + std::to_string(ms / 3) + " ms\n";
+            if (ms < best) {
+                best = ms;
+                plan = p;
             }
+        }
         (void)hipEventDestroy(start);
         (void)hipEventDestroy(stop);
     }
@@ -2774,9 +2792,12 @@ struct nxs_batch {
 
 extern "C" {
 
-int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch **out) {
+int nxs_batch_create(const nxs_pair *pairs, int count, int mode, nxs_batch **out) {
     using namespace nxs;
     if (!pairs || count < 1 || !out) return fail(NXS_EINVAL, "no pairs or output pointer");
+    if (mode < NXS_BATCH_DEFAULT || mode > NXS_BATCH_FUSED)
+        return fail(NXS_EINVAL, "mode must be NXS_BATCH_DEFAULT, NXS_BATCH_TUNED or NXS_BATCH_FUSED");
+    const bool autotune = mode == NXS_BATCH_TUNED;
     auto batch = std::make_unique<nxs_batch>();
     try {
         // Size classes in order of first appearance; each pair's workspace in one allocation.
@@ -2801,6 +2822,9 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
             }
             members[c].push_back(i);
         }
+        if (mode == NXS_BATCH_FUSED && batch->classes.size() > (size_t)MAX_MIXED_CLASSES)
+            throw Error{"a fused batch takes at most " + std::to_string(MAX_MIXED_CLASSES) +
+                        " size classes, got " + std::to_string(batch->classes.size())};
         size_t total = 0;
         for (size_t c = 0; c < batch->classes.size(); c++) {
             const MatchParams &p = batch->classes[c].p;
@@ -2844,8 +2868,11 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
                                  ":\n";
                 batch->tune(c);
             }
-        // Without tuning: the classes in two branches, a class a launch.
-        batch->plan = BatchPlan{std::min(2, (int)batch->classes.size()), 0};
+        // Without tuning: the classes in two branches, a class a launch; or fused, checkpoints
+        // every 8 steps.
+        batch->plan = mode == NXS_BATCH_FUSED
+                          ? BatchPlan{1, 0, 8}
+                          : BatchPlan{std::min(2, (int)batch->classes.size()), 0, 0};
         if (autotune) batch->tune_plan();
         batch->launches = batch->deal(batch->plan);
         // One graph for the whole batch.
@@ -2867,18 +2894,28 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int autotune, nxs_batch *
         HIP_TRY(e);
         HIP_TRY(hipGraphInstantiate(&batch->exec, batch->graph, nullptr, nullptr, 0));
         std::ostringstream text;
-        text << "plan: " << batch->plan.branches << " branches, "
-             << (batch->plan.chunk ? std::to_string(batch->plan.chunk) + " pairs a launch"
-                                   : std::string("a class a launch"))
-             << ", " << batch->launches.size() << " launch chains\n";
-        for (auto &c : batch->classes)
+        text << "plan: " << plan_text(batch->plan) << ", "
+             << (batch->plan.fused ? std::string("4 launches")
+                                   : std::to_string(batch->launches.size()) + " launch chains")
+             << "\n";
+        for (auto &c : batch->classes) {
             text << c.count << " x " << c.p.width << "x" << c.p.height << "x" << c.p.disparities
-                 << " paths " << c.p.paths << ": prefetch " << c.schedule.prefetch << ", "
-                 << c.schedule.lanes << " lanes per line, " << c.schedule.waves
+                 << " paths " << c.p.paths << ": ";
+            if (batch->plan.fused) {
+                // The fused launch's aggregation, whatever the class's own pick.
+                text << "16 lanes per line, 2 waves per line, meeting by recomputation"
+                     << (autotune ? " (its own pick took " + std::to_string(c.tuned_ms) + " ms)"
+                                  : "")
+                     << "\n";
+                continue;
+            }
+            text << "prefetch " << c.schedule.prefetch << ", " << c.schedule.lanes
+                 << " lanes per line, " << c.schedule.waves
                  << (c.schedule.waves == 2 ? " waves" : " wave") << " per line, "
                  << (c.schedule.meet ? "meeting by recomputation, " : "")
                  << (c.schedule.split ? "one launch per orientation" : "orientations in one launch")
                  << (autotune ? " (tuned, " + std::to_string(c.tuned_ms) + " ms)" : "") << "\n";
+        }
         batch->description = text.str() + batch->trials;
     } catch (const Error &e) {
         return fail(NXS_EDEVICE, e.reason);

@@ -76,16 +76,11 @@ __host__ __device__ constexpr size_t census_cost_lds(int W, int D) {
            14 * (size_t)census_row_pitch(W);
 }
 
-// Grid y: the row; z: the pair. keep_census: also store both census images. COST_BLOCK
-// threads: 512 where every row of the launch is in flight at once (three blocks a CU), else
-// 1024 (launch_all).
+// Row y of pair pd, by one block of COST_BLOCK threads with census_cost_lds(W, D) bytes of
+// LDS at `lds`. keep_census: also store both census images.
 template <int COST_BLOCK, int D>
-__global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single,
-                                                                 const PairDesc *table, int W,
-                                                                 int H, int keep_census) {
-    extern __shared__ __align__(16) uint8_t lds[];
-    const PairDesc pd = pair_of(single, table, blockIdx.z);
-    const int y = blockIdx.y;
+__device__ __forceinline__ void census_cost_row(const PairDesc &pd, int W, int H, int y,
+                                                int keep_census, uint8_t *lds) {
     uint64_t *cl = reinterpret_cast<uint64_t *>(lds);  // [left_slots(W)]
     uint64_t *cr = cl + left_slots(W);                 // [right_slots(W, D)]
     const int row_pitch = census_row_pitch(W);         // padded row
@@ -164,6 +159,17 @@ __global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single
                                                    out, at, 0, 0);
         }
     }
+}
+
+// Grid y: the row; z: the pair. COST_BLOCK threads: 512 where every row of the launch is in
+// flight at once (three blocks a CU), else 1024 (launch_all).
+template <int COST_BLOCK, int D>
+__global__ __launch_bounds__(COST_BLOCK) void census_cost_kernel(PairDesc single,
+                                                                 const PairDesc *table, int W,
+                                                                 int H, int keep_census) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    census_cost_row<COST_BLOCK, D>(pair_of(single, table, blockIdx.z), W, H, blockIdx.y,
+                                   keep_census, lds);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -940,8 +946,11 @@ __global__ __launch_bounds__(TWO_BLOCK) void path2_kernel(PathArgs a) {
 // line's entering path, which restores from zero bytes (a masked load). The phases meet at
 // one barrier, after every checkpoint store has completed; the waves touch nothing of each
 // other's otherwise, and no LDS.
+// Block b of launch geometry `a` (TWO_BLOCK threads), on the pair pair_of(a.single, table,
+// index).
 template <int K, int S, int LANES, bool STREAM>
-__global__ __launch_bounds__(TWO_BLOCK) void path_rc_kernel(PathArgs a) {
+__device__ __forceinline__ void path_rc_block(const PathArgs &a, const PairDesc *table, int index,
+                                              int b) {
     constexpr int NT = STREAM && K >= 4 ? 2 : 0;
     static_assert(LANES == 16 || LANES == 32, "a line is one or two DPP rows");
     static_assert(S == 2 || S == 4 || S == 8, "segments of 2, 4 or 8 steps");
@@ -950,11 +959,10 @@ __global__ __launch_bounds__(TWO_BLOCK) void path_rc_kernel(PathArgs a) {
     const int dir = threadIdx.x >> 6;  // wave-uniform: 0 forward, 1 backward
     const int lane = threadIdx.x & (LANES - 1);
     const int row = (threadIdx.x & 63) / LANES;
-    const int b = blockIdx.x;
     const BlockLines bl = block_lines(a, b, LINES);
     const int o = bl.o;
     const int W = a.W, H = a.H;
-    const PairDesc pd = pair_of(a.single, a.table, blockIdx.y);
+    const PairDesc pd = pair_of(a.single, table, index);
     if (o == 1 && !a.backward_columns) {
         walk_columns_down<K, S, LANES>(a, pd, b, dir, lane, row);
         return;
@@ -1045,6 +1053,11 @@ __global__ __launch_bounds__(TWO_BLOCK) void path_rc_kernel(PathArgs a) {
         segment(k, ca);
         if ((k + 1) * S < h2) segment(k + 1, cb);
     }
+}
+
+template <int K, int S, int LANES, bool STREAM>
+__global__ __launch_bounds__(TWO_BLOCK) void path_rc_kernel(PathArgs a) {
+    path_rc_block<K, S, LANES, STREAM>(a, a.table, blockIdx.y, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1266,15 +1279,15 @@ __device__ __forceinline__ uint32_t pixel_min_pk(uint32_t v) {
 // 256 disparities on 4, 8 and 16 lanes), so that a wave selects more pixels at once and the
 // per-pixel work (the reductions over the pixel's lanes, the subpixel step) is shared by
 // fewer lanes; 192 disparities take 12 a lane on 16 lanes.
+// Block bx of row y of pair pd (256 threads).
 template <int K, int PL, bool STREAM>
-__global__ __launch_bounds__(256) void select_kernel(PairDesc single, const PairDesc *table,
-                                                     int orientations, int W) {
+__device__ __forceinline__ void select_block(const PairDesc &pd, int orientations, int W, int H,
+                                             int bx, int y) {
     static_assert(K <= 16, "a key keeps the lane's index in 4 bits");
     static_assert(PL == 4 || PL == 8 || PL == 16, "a pixel's lanes lie in one DPP row");
     constexpr int D = PL * K, R = K / 2;
     constexpr int PIXELS = 256 / PL;  // a block's pixels per group
-    const PairDesc pd = pair_of(single, table, blockIdx.z);
-    const uint32_t volume = (uint32_t)W * gridDim.y * D;
+    const uint32_t volume = (uint32_t)W * H * D;
     const Rsrc vol[4] = {buffer_of(pd.b.paired[0], volume), buffer_of(pd.b.paired[1], volume),
                          buffer_of(pd.b.paired[2], volume), buffer_of(pd.b.paired[3], volume)};
     const float uniqueness = pd.uniqueness;
@@ -1282,8 +1295,7 @@ __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const Pair
     uint32_t *__restrict__ keys = pd.b.keys;
     uint16_t *__restrict__ sums_out = pd.sums;
     const int lane = threadIdx.x & (PL - 1);  // within the pixel's lanes
-    const int x_first = blockIdx.x * PIXELS * SELECT_GROUPS + (int)(threadIdx.x / PL),
-              y = blockIdx.y;
+    const int x_first = bx * PIXELS * SELECT_GROUPS + (int)(threadIdx.x / PL);
     const int d0 = lane * K;
     // This lane's bytes of group g's pixel (the last pixel of the row past its end).
     auto offset_of = [&](int g) {
@@ -1404,6 +1416,14 @@ __global__ __launch_bounds__(256) void select_kernel(PairDesc single, const Pair
     }
 }
 
+// Grid x: groups of pixels of a row; y: the row; z: the pair.
+template <int K, int PL, bool STREAM>
+__global__ __launch_bounds__(256) void select_kernel(PairDesc single, const PairDesc *table,
+                                                     int orientations, int W) {
+    select_block<K, PL, STREAM>(pair_of(single, table, blockIdx.z), orientations, W, gridDim.y,
+                                blockIdx.x, blockIdx.y);
+}
+
 // ------------------------------------------------------------------------------------------
 // Left-right check (reference.rs:188-235; check.rs:7-47), per row: each right pixel keeps
 // the cheapest left match projected onto it, ties to the larger disparity, via an LDS
@@ -1447,15 +1467,13 @@ __host__ __device__ constexpr size_t check_median_lds(int W) { return 6 * 4 * (s
 // Grid x: the row; y: the pair.
 constexpr int CHECK_BLOCK = 1024;
 
-__global__ __launch_bounds__(CHECK_BLOCK) void check_median_kernel(PairDesc single,
-                                                                   const PairDesc *table, int W,
-                                                                   int H, int keep_checked,
-                                                                   int positive) {
-    extern __shared__ __align__(16) uint32_t check_lds[];
+// Row y of pair pd, by one block of CHECK_BLOCK threads with check_median_lds(W) bytes of
+// LDS at `check_lds`.
+__device__ __forceinline__ void check_median_row(const PairDesc &pd, int W, int H, int y,
+                                                 int keep_checked, int positive,
+                                                 uint32_t *check_lds) {
     uint32_t *right = check_lds;                                    // [3][W]
     float *checked = reinterpret_cast<float *>(check_lds + 3 * W);  // [3][W]
-    const PairDesc pd = pair_of(single, table, blockIdx.y);
-    const int y = blockIdx.x;
     int rows[3];
 #pragma unroll
     for (int r = 0; r < 3; r++) rows[r] = min(max(y - 1 + r, 0), H - 1);
@@ -1513,6 +1531,15 @@ __global__ __launch_bounds__(CHECK_BLOCK) void check_median_kernel(PairDesc sing
         const float m = __int_as_float(v[4]);
         out[x] = !positive || m > 0.0f ? m : -1.0f;
     }
+}
+
+__global__ __launch_bounds__(CHECK_BLOCK) void check_median_kernel(PairDesc single,
+                                                                   const PairDesc *table, int W,
+                                                                   int H, int keep_checked,
+                                                                   int positive) {
+    extern __shared__ __align__(16) uint32_t check_lds[];
+    check_median_row(pair_of(single, table, blockIdx.y), W, H, blockIdx.x, keep_checked,
+                     positive, check_lds);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1744,6 +1771,18 @@ static void launch_census_cost(const PairDesc &single, const PairDesc *table, in
     }
 }
 
+// The device's CUs (256 on MI355X), read once.
+static int compute_unit_count() {
+    static const int compute_units = [] {
+        int device = 0, n = 0;
+        if (hipGetDevice(&device) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+            n = 0;
+        return n > 0 ? n : 256;
+    }();
+    return compute_units;
+}
+
 static hipError_t launch_all(const MatchParams &p, const PairDesc &single, const PairDesc *table,
                              int count, const Schedule &sch, hipStream_t stream) {
     const int W = p.width, H = p.height, D = p.disparities;
@@ -1772,14 +1811,7 @@ static hipError_t launch_all(const MatchParams &p, const PairDesc &single, const
     // whose volumes fill the 256 MB Infinity Cache, the path kernel after it runs 8% faster
     // (180 -> 165 us). Otherwise 1024 (at 1920x1080x256 either takes about 150 us: the
     // rows' blocks run in rounds, each round's stores draining before the next starts).
-    static const int compute_units = [] {
-        int device = 0, n = 0;
-        if (hipGetDevice(&device) != hipSuccess ||
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
-            n = 0;
-        return n > 0 ? n : 256;
-    }();
-    const bool one_sweep = (size_t)H * count <= 3 * (size_t)compute_units;
+    const bool one_sweep = (size_t)H * count <= 3 * (size_t)compute_unit_count();
     int slot = profile_begin(PK_COST, stream);
     if (one_sweep)
         launch_census_cost<512>(single, table, W, H, D, count, p.keep_stages ? 1 : 0, stream);
@@ -1845,6 +1877,230 @@ hipError_t launch_match_batch(const MatchParams &p, const PairDesc *table, int c
                               const Schedule &schedule, hipStream_t stream) {
     PairDesc unused{};
     return launch_all(p, unused, table, count, schedule, stream);
+}
+
+// ------------------------------------------------------------------------------------------
+// The batched form's fused launches: every size class of a batch in one launch per kernel.
+// The grid of each launch is the classes' blocks one after the other (the class with the
+// most work a pair first), each class its pairs in table order, each pair its blocks as the
+// class's own launch orders them. A block finds its class from the classes' first blocks,
+// then runs the very body of the per-class kernel at the class's disparity count: results
+// are the per-class launches' bit for bit. One kernel holds every body up to the batch's
+// largest disparity count, so its registers are that body's: the classes share one
+// occupancy, the largest class's.
+
+struct MixedClass {
+    int W, H, D, orientations;
+    int first;     // the class's first table entry
+    int start[4];  // its first block in the census, path, select and check launches
+    int path_blocks, select_x;  // a pair's blocks in the path launch; select blocks a row
+    PathArgs path;  // one pair's path launch geometry (its pair fields unused)
+};
+
+struct MixedArgs {
+    const PairDesc *table;
+    int classes;
+    int keep_stages, positive;
+    MixedClass c[MAX_MIXED_CLASSES];
+};
+static_assert(sizeof(MixedArgs) <= 4096, "a kernel's arguments take at most 4 KB");
+
+enum { MIX_COST = 0, MIX_PATH = 1, MIX_SELECT = 2, MIX_CHECK = 3 };
+
+// The class of block b of launch `which`: the last whose first block is at or before b.
+__device__ __forceinline__ int mixed_class(const MixedArgs &m, int which, int b) {
+    int k = 0;
+    for (int c = 1; c < m.classes; c++) k = b >= m.c[c].start[which] ? c : k;
+    return k;
+}
+
+template <int COST_BLOCK>
+__global__ __launch_bounds__(COST_BLOCK) void census_cost_mixed_kernel(MixedArgs m) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const MixedClass &c = m.c[mixed_class(m, MIX_COST, blockIdx.x)];
+    const int local = blockIdx.x - c.start[MIX_COST];
+    const int pair = local / c.H, y = local - pair * c.H;
+    const PairDesc pd = m.table[c.first + pair];
+    switch (c.D) {
+        case 64: census_cost_row<COST_BLOCK, 64>(pd, c.W, c.H, y, m.keep_stages, lds); break;
+        case 128: census_cost_row<COST_BLOCK, 128>(pd, c.W, c.H, y, m.keep_stages, lds); break;
+        case 192: census_cost_row<COST_BLOCK, 192>(pd, c.W, c.H, y, m.keep_stages, lds); break;
+        default: census_cost_row<COST_BLOCK, 256>(pd, c.W, c.H, y, m.keep_stages, lds); break;
+    }
+}
+
+// Two waves of 16 lanes a line meeting by recomputation (path_rc_kernel), disparity counts up
+// to 16 KMAX.
+template <int KMAX, int S, bool STREAM>
+__global__ __launch_bounds__(TWO_BLOCK) void path_rc_mixed_kernel(MixedArgs m) {
+    const MixedClass &c = m.c[mixed_class(m, MIX_PATH, blockIdx.x)];
+    const int local = blockIdx.x - c.start[MIX_PATH];
+    const int pair = local / c.path_blocks, b = local - pair * c.path_blocks;
+    const PathArgs &a = c.path;
+    const int index = c.first + pair;
+    switch (c.D) {
+        case 64: path_rc_block<4, S, 16, STREAM>(a, m.table, index, b); break;
+        case 128: if constexpr (KMAX >= 8) path_rc_block<8, S, 16, STREAM>(a, m.table, index, b); break;
+        case 192: if constexpr (KMAX >= 12) path_rc_block<12, S, 16, STREAM>(a, m.table, index, b); break;
+        default: if constexpr (KMAX >= 16) path_rc_block<16, S, 16, STREAM>(a, m.table, index, b); break;
+    }
+}
+
+// As select_kernel at each class's disparity count (the lanes a pixel launch_paths_and_select
+// gives it).
+template <int KMAX, bool STREAM>
+__global__ __launch_bounds__(256) void select_mixed_kernel(MixedArgs m) {
+    const MixedClass &c = m.c[mixed_class(m, MIX_SELECT, blockIdx.x)];
+    const int local = blockIdx.x - c.start[MIX_SELECT];
+    const int per_pair = c.select_x * c.H;
+    const int pair = local / per_pair, r = local - pair * per_pair;
+    const int y = r / c.select_x, bx = r - y * c.select_x;
+    const PairDesc pd = m.table[c.first + pair];
+    switch (c.D) {
+        case 64: select_block<16, 4, STREAM>(pd, c.orientations, c.W, c.H, bx, y); break;
+        case 128: if constexpr (KMAX >= 8) select_block<16, 8, STREAM>(pd, c.orientations, c.W, c.H, bx, y); break;
+        case 192: if constexpr (KMAX >= 12) select_block<12, 16, STREAM>(pd, c.orientations, c.W, c.H, bx, y); break;
+        default: if constexpr (KMAX >= 16) select_block<16, 16, STREAM>(pd, c.orientations, c.W, c.H, bx, y); break;
+    }
+}
+
+__global__ __launch_bounds__(CHECK_BLOCK) void check_median_mixed_kernel(MixedArgs m) {
+    extern __shared__ __align__(16) uint32_t check_lds[];
+    const MixedClass &c = m.c[mixed_class(m, MIX_CHECK, blockIdx.x)];
+    const int local = blockIdx.x - c.start[MIX_CHECK];
+    const int pair = local / c.H, y = local - pair * c.H;
+    check_median_row(m.table[c.first + pair], c.W, c.H, y, m.keep_stages, m.positive, check_lds);
+}
+
+template <int KMAX, bool STREAM>
+static void launch_mixed_paths(const MixedArgs &m, int blocks, int segment, hipStream_t stream) {
+    if (segment == 4)
+        hipLaunchKernelGGL((path_rc_mixed_kernel<KMAX, 4, STREAM>), dim3(blocks), dim3(TWO_BLOCK), 0, stream, m);
+    else
+        hipLaunchKernelGGL((path_rc_mixed_kernel<KMAX, 8, STREAM>), dim3(blocks), dim3(TWO_BLOCK), 0, stream, m);
+}
+
+template <int KMAX>
+static void launch_mixed_aggregation(const MixedArgs &m, int path_blocks, int select_blocks,
+                                     int segment, bool stream_volumes, hipStream_t stream) {
+    int slot = profile_begin(PK_PATH, stream);
+    if (stream_volumes)
+        launch_mixed_paths<KMAX, true>(m, path_blocks, segment, stream);
+    else
+        launch_mixed_paths<KMAX, false>(m, path_blocks, segment, stream);
+    profile_end(slot, stream);
+    slot = profile_begin(PK_SELECT, stream);
+    if (stream_volumes)
+        hipLaunchKernelGGL((select_mixed_kernel<KMAX, true>), dim3(select_blocks), dim3(256), 0, stream, m);
+    else
+        hipLaunchKernelGGL((select_mixed_kernel<KMAX, false>), dim3(select_blocks), dim3(256), 0, stream, m);
+    profile_end(slot, stream);
+}
+
+hipError_t launch_match_mixed(const MixedMember *members, int n, const PairDesc *table,
+                              int segment, hipStream_t stream) {
+    if (n < 1 || n > MAX_MIXED_CLASSES || !table || (segment != 4 && segment != 8))
+        return hipErrorInvalidValue;
+    for (int i = 0; i < n; i++) {
+        const MatchParams &p = members[i].p;
+        if (p.width < 1 || p.width > MAX_WIDTH || p.height < 1 || members[i].count < 1 ||
+            (p.disparities != 64 && p.disparities != 128 && p.disparities != 192 &&
+             p.disparities != 256))
+            return hipErrorInvalidValue;
+    }
+    // The classes with the most work a pair first: their long lines start first.
+    int order[MAX_MIXED_CLASSES];
+    for (int i = 0; i < n; i++) order[i] = i;
+    auto work = [&](int i) {
+        const MatchParams &p = members[i].p;
+        return (double)p.width * p.height * p.disparities;
+    };
+    std::stable_sort(order, order + n, [&](int a, int b) { return work(a) > work(b); });
+    MixedArgs m{};
+    m.table = table;
+    m.classes = n;
+    m.keep_stages = members[0].p.keep_stages ? 1 : 0;
+    m.positive = members[0].p.positive ? 1 : 0;
+    int starts[4] = {0, 0, 0, 0};
+    int kmax = 0, rows = 0;
+    size_t volumes = 0, cost_lds = 0, check_lds = 0;
+    for (int k = 0; k < n; k++) {
+        const MixedMember &mm = members[order[k]];
+        const MatchParams &p = mm.p;
+        MixedClass &c = m.c[k];
+        const int W = p.width, H = p.height;
+        c.W = W;
+        c.H = H;
+        c.D = p.disparities;
+        c.orientations = orientations_of(p.paths);
+        c.first = mm.first;
+        PathArgs &a = c.path;
+        a.table = nullptr;
+        a.W = W;
+        a.H = H;
+        a.backward_columns = p.paths == 8;
+        a.first = 0;
+        a.last = c.orientations - 1;
+        a.longest_first = p.paths == 8;
+        // As launch_paths_and_select lays out a pair: two waves of 16 lanes take four lines
+        // a block (eight down-only columns at 3 paths).
+        const int lines[4] = {H, W, W + H - 1, W + H - 1};
+        int blocks = 0;
+        for (int o = 0; o < 4; o++) {
+            a.block_start[o] = blocks;
+            const int per_block = o == 1 && !a.backward_columns ? 8 : 4;
+            if (o < c.orientations) blocks += (lines[o] + per_block - 1) / per_block;
+        }
+        c.path_blocks = blocks;
+        const int K = p.disparities / 16;
+        const int spl = K == 12 ? 16 : K;  // 16 disparities a select lane, 12 at 192
+        c.select_x = (W + 256 / spl * SELECT_GROUPS - 1) / (256 / spl * SELECT_GROUPS);
+        const int per_pair[4] = {H, c.path_blocks, c.select_x * H, H};
+        for (int w = 0; w < 4; w++) {
+            c.start[w] = starts[w];
+            starts[w] += per_pair[w] * mm.count;
+        }
+        kmax = std::max(kmax, K);
+        rows += H * mm.count;
+        volumes += (size_t)c.orientations * W * H * p.disparities * mm.count;
+        cost_lds = std::max(cost_lds, census_cost_lds(W, p.disparities));
+        check_lds = std::max(check_lds, check_median_lds(W));
+    }
+    static const hipError_t lds_ok = [] {
+        const void *kernels[] = {reinterpret_cast<const void *>(census_cost_mixed_kernel<512>),
+                                 reinterpret_cast<const void *>(census_cost_mixed_kernel<1024>)};
+        hipError_t e = hipSuccess;
+        for (const void *k : kernels)
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)census_cost_lds(MAX_WIDTH, MAX_D));
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void *>(check_median_mixed_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)check_median_lds(MAX_WIDTH));
+        return e;
+    }();
+    if (lds_ok != hipSuccess) return lds_ok;
+    int slot = profile_begin(PK_COST, stream);
+    if (rows <= 3 * compute_unit_count())
+        hipLaunchKernelGGL(census_cost_mixed_kernel<512>, dim3(starts[MIX_COST]), dim3(512),
+                           cost_lds, stream, m);
+    else
+        hipLaunchKernelGGL(census_cost_mixed_kernel<1024>, dim3(starts[MIX_COST]), dim3(1024),
+                           cost_lds, stream, m);
+    profile_end(slot, stream);
+    const bool stream_volumes = volumes > ((size_t)256 << 20);
+    switch (kmax) {
+        case 4: launch_mixed_aggregation<4>(m, starts[MIX_PATH], starts[MIX_SELECT], segment, stream_volumes, stream); break;
+        case 8: launch_mixed_aggregation<8>(m, starts[MIX_PATH], starts[MIX_SELECT], segment, stream_volumes, stream); break;
+        case 12: launch_mixed_aggregation<12>(m, starts[MIX_PATH], starts[MIX_SELECT], segment, stream_volumes, stream); break;
+        default: launch_mixed_aggregation<16>(m, starts[MIX_PATH], starts[MIX_SELECT], segment, stream_volumes, stream); break;
+    }
+    slot = profile_begin(PK_CHECK_MEDIAN, stream);
+    hipLaunchKernelGGL(check_median_mixed_kernel, dim3(starts[MIX_CHECK]), dim3(CHECK_BLOCK),
+                       check_lds, stream, m);
+    profile_end(slot, stream);
+    return hipGetLastError();
 }
 
 }  // namespace nxs

@@ -97,6 +97,27 @@ def test_the_no_alloc_matcher_refuses_what_its_scratch_cannot_hold_before_any_de
     assert "#define NXS_NOALLOC_SCRATCH_BYTES (64u << 20)" in text
 
 
+def test_a_batch_refuses_an_unknown_mode_and_a_fused_batch_of_nine_classes_before_any_device_work():
+    """nxs_batch_create checks its mode, and a fused batch's class count (one launch per
+    kernel holds at most 8 size classes), before it allocates or launches anything (the
+    pointers here are never read)."""
+    lib = _native.load()
+    fake = 0x1000
+    text = open(os.path.join(ROOT, "include", "nxs_hip.h")).read()
+    for name, value in (("DEFAULT", 0), ("TUNED", 1), ("FUSED", 2)):
+        assert f"#define NXS_BATCH_{name} {value}" in text
+        assert getattr(_native, f"BATCH_{name}") == value
+    pairs = (_native.NxsPair * 9)()
+    for k in range(9):
+        pairs[k] = _native.NxsPair(fake, fake, fake, 64 + 8 * k, 48, 64, 8, 8, 64, 1.0526316)
+    handle = ctypes.c_void_p()
+    assert lib.nxs_batch_create(pairs, 9, 3, ctypes.byref(handle)) == _native.NXS_EINVAL
+    assert "mode must be" in _native.last_error()
+    assert lib.nxs_batch_create(pairs, 9, _native.BATCH_FUSED, ctypes.byref(handle)) != _native.NXS_OK
+    assert "at most 8 size classes, got 9" in _native.last_error()
+    assert not handle.value
+
+
 def test_the_nif_calls_only_entry_points_the_header_declares():
     """c_src/nxs_nif.c (compiled where Erlang's headers are) goes through include/nxs_hip.h
     alone: every nxs_ call it makes is declared there."""

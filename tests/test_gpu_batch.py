@@ -24,14 +24,19 @@ def make_pairs(seed):
     return pairs
 
 
-def test_batch_matches_the_oracle_on_every_replay():
+@pytest.mark.parametrize("fused", [False, True], ids=["tuned", "fused"])
+def test_batch_matches_the_oracle_on_every_replay(fused):
+    """Tuned: each class's schedule and the plan timed on the batch. Fused: the four classes
+    (64 to 192 disparities, one at 3 paths) in one launch per kernel."""
     pairs = make_pairs(1)
     bufs = []
     for left, right, (w, h, d, paths, p1, p2, u) in pairs:
         bufs.append((hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h)))
     batch = _native.Batch([(l.ptr, r.ptr, o.ptr, *cfg) for (l, r, o), (_, _, cfg) in zip(bufs, pairs)],
-                          autotune=True)
+                          autotune=not fused, fused=fused)
     print(batch.describe(), flush=True)
+    if fused:  # tuning may pick the fused plan as well
+        assert batch.describe().startswith("plan: every class in one launch per kernel")
     stream = hip.Stream()
     for replay in range(3):
         if replay:

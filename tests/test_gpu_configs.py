@@ -66,15 +66,19 @@ def batch_images(variant):
     return pairs
 
 
-def test_batch_of_96_pairs_at_the_benchmark_classes():
+@pytest.mark.parametrize("fused", [False, True], ids=["per_class", "fused"])
+def test_batch_of_96_pairs_at_the_benchmark_classes(fused):
     """nxs_batch over the batched leg's classes, every pair checked after a replay, then
-    again with new images uploaded before the next replay."""
+    again with new images uploaded before the next replay: launches per class, and every
+    class in one launch per kernel."""
     pairs = batch_images(0)
     bufs = [(hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h))
             for left, right, (w, h, d) in pairs]
-    print("batch: 96 pairs, 24 x 480x270/64, 48 x 728x544/128, 24 x 1280x720/192", flush=True)
+    print(f"batch: 96 pairs, 24 x 480x270/64, 48 x 728x544/128, 24 x 1280x720/192, "
+          f"fused {fused}", flush=True)
     batch = _native.Batch([(l.ptr, r.ptr, o.ptr, w, h, d, 8, 8, 64, UNIQUENESS)
-                           for (l, r, o), (_, _, (w, h, d)) in zip(bufs, pairs)], autotune=False)
+                           for (l, r, o), (_, _, (w, h, d)) in zip(bufs, pairs)], autotune=False,
+                          fused=fused)
     stream = hip.Stream()
     try:
         for variant in range(2):
