@@ -504,8 +504,9 @@ CALIBRATION_BYTES = 256 << 20
 def kernels_only(size: str) -> int:
     from nx_stereo_amd import _native, hip
 
-    if size == "batch":
-        # The batched leg's 96 pairs: autotuned, then one run and five replays of the graph.
+    if size in ("batch", "batch_fused"):
+        # The batched leg's 96 pairs: autotuned (or every class in one launch per kernel),
+        # then one run and five replays of the graph.
         pairs, buffers = [], []
         for k, (w, h, d, count) in enumerate(BATCH_CLASSES):
             for j in range(count):
@@ -513,14 +514,14 @@ def kernels_only(size: str) -> int:
                 l, r, o = hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h)
                 buffers += [l, r, o]
                 pairs.append((l.ptr, r.ptr, o.ptr, w, h, d, 8, 8, 64, 1.0526315789473684))
-        batch = _native.Batch(pairs, autotune=True)
+        batch = _native.Batch(pairs, autotune=size == "batch", fused=size == "batch_fused")
         stream = hip.Stream()
         for _ in range(6):
             batch.run(stream.handle)
         stream.synchronize()
         print(batch.describe().split("\n")[0], flush=True)
         batch.close()
-        print("kernels-only batch done", flush=True)
+        print(f"kernels-only {size} done", flush=True)
         return 0
     if size.startswith("c2f"):
         # sgm_coarse_to_fine/3 as the coarse_to_fine leg runs it (band 16, margin 4, 8 paths).
@@ -617,7 +618,7 @@ def main() -> int:
     parser.add_argument("--legs", default="large,batched,served,coarse_to_fine,streams,host_io",
                         help="comma-separated extra legs: large, batched, served, "
                              "coarse_to_fine, streams, host_io (or none)")
-    parser.add_argument("--kernels-only", choices=sorted(TRAFFIC_FILES) + ["batch"],
+    parser.add_argument("--kernels-only", choices=sorted(TRAFFIC_FILES) + ["batch", "batch_fused"],
                         help="run three matchings and the calibration copy (c2f*: five "
                              "coarse-to-fine matchings; batch: the batched leg's graph, "
                              "autotuned, then six runs), for rocprofv3")
