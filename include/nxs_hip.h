@@ -249,6 +249,22 @@ int nxs_schedule_set(int width, int height, int disparities, int paths, const nx
 int nxs_schedule_get(int width, int height, int disparities, int paths, nxs_schedule *s);
 
 int nxs_batch_create(const nxs_pair *pairs, int count, int mode, nxs_batch **out);
+
+/* A batch's plan, as nxs_batch_describe's first line names it. */
+typedef struct nxs_batch_plan {
+    int branches; /* 1, 2 or 4 parallel branches of the graph */
+    int chunk;    /* pairs a launch, 0: each class in one launch (unused when fused) */
+    int fused;    /* 0; or 4 or 8: each branch's classes (whole classes, the largest first to
+                     the least loaded branch; at most 8 classes) in one launch per kernel,
+                     checkpoints every `fused` steps */
+} nxs_batch_plan;
+
+/* nxs_batch_create with the plan pinned, untuned, and each class's schedule its
+ * size's (nxs_schedule_get): for a caller that has tuned once and keeps the plan it
+ * picked, so that a later batch of the same mix skips tuning. No counterpart in the
+ * reference. */
+int nxs_batch_create_planned(const nxs_pair *pairs, int count, const nxs_batch_plan *plan,
+                             nxs_batch **out);
 int nxs_batch_run(nxs_batch *batch, void *stream);
 /* The classes and their schedules, one line each; valid while the batch lives. */
 const char *nxs_batch_describe(nxs_batch *batch);

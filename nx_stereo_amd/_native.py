@@ -25,7 +25,7 @@ SYMBOLS = (
     "nxs_band_workspace_size", "nxs_match_band_device", "nxs_match_band_host",
     "nxs_coarse_to_fine_workspace_size", "nxs_coarse_to_fine_device", "nxs_coarse_to_fine_host",
     "nxs_run_device", "nxs_profile_start", "nxs_profile_stop", "nxs_schedule_set",
-    "nxs_schedule_get", "nxs_batch_create",
+    "nxs_schedule_get", "nxs_batch_create", "nxs_batch_create_planned",
     "nxs_batch_run", "nxs_batch_describe", "nxs_batch_destroy", "nxs_last_error",
 )
 
@@ -56,6 +56,12 @@ class NxsSchedule(ctypes.Structure):
 
     _fields_ = [("prefetch", ctypes.c_int), ("split", ctypes.c_int), ("lanes", ctypes.c_int),
                 ("waves", ctypes.c_int), ("meet", ctypes.c_int)]
+
+
+class NxsBatchPlan(ctypes.Structure):
+    """struct nxs_batch_plan: a batch's branches, pairs a launch and fused checkpoints."""
+
+    _fields_ = [("branches", ctypes.c_int), ("chunk", ctypes.c_int), ("fused", ctypes.c_int)]
 
 
 class NativeError(RuntimeError):
@@ -123,6 +129,9 @@ def load() -> ctypes.CDLL:
     lib.nxs_schedule_get.restype = i
     lib.nxs_batch_create.argtypes = [ctypes.POINTER(NxsPair), i, i, ctypes.POINTER(P)]
     lib.nxs_batch_create.restype = i
+    lib.nxs_batch_create_planned.argtypes = [ctypes.POINTER(NxsPair), i,
+                                             ctypes.POINTER(NxsBatchPlan), ctypes.POINTER(P)]
+    lib.nxs_batch_create_planned.restype = i
     lib.nxs_batch_run.argtypes = [P, P]
     lib.nxs_batch_run.restype = i
     lib.nxs_batch_describe.argtypes = [P]
@@ -324,17 +333,23 @@ class Batch:
     """nxs_batch: pairs (left, right, out device addresses; width, height, disparities,
     paths, p1, p2, uniqueness) captured into one graph; run() replays it on a stream.
     autotune: the class schedules and plans timed on the batch (NXS_BATCH_TUNED); fused:
-    every class in one launch per kernel (NXS_BATCH_FUSED)."""
+    every class in one launch per kernel (NXS_BATCH_FUSED); plan: (branches, chunk, fused)
+    pinned, untuned (nxs_batch_create_planned)."""
 
-    def __init__(self, pairs, autotune: bool = True, fused: bool = False):
+    def __init__(self, pairs, autotune: bool = True, fused: bool = False, plan=None):
         lib = load()
         arr = (NxsPair * len(pairs))()
         for k, (left, right, out, w, h, d, paths, p1, p2, u) in enumerate(pairs):
             arr[k] = NxsPair(left, right, out, w, h, d, paths, p1, p2, float(np.float32(u)))
         handle = ctypes.c_void_p()
-        mode = BATCH_FUSED if fused else (BATCH_TUNED if autotune else BATCH_DEFAULT)
-        _check(lib.nxs_batch_create(arr, len(pairs), mode, ctypes.byref(handle)),
-               "nxs_batch_create")
+        if plan is not None:
+            pinned = NxsBatchPlan(*plan)
+            _check(lib.nxs_batch_create_planned(arr, len(pairs), ctypes.byref(pinned),
+                                                ctypes.byref(handle)), "nxs_batch_create_planned")
+        else:
+            mode = BATCH_FUSED if fused else (BATCH_TUNED if autotune else BATCH_DEFAULT)
+            _check(lib.nxs_batch_create(arr, len(pairs), mode, ctypes.byref(handle)),
+                   "nxs_batch_create")
         self.lib, self.handle = lib, handle
 
     def run(self, stream: int = 0) -> None:

@@ -2615,8 +2615,12 @@ struct BatchLaunch {
 constexpr int MAX_BRANCHES = 4;
 
 inline std::string plan_text(const BatchPlan &p) {
-    if (p.fused)
+    if (p.fused && p.branches <= 1)
         return "every class in one launch per kernel, checkpoints every " +
+               std::to_string(p.fused) + " steps";
+    if (p.fused)
+        return "the classes on " + std::to_string(p.branches) +
+               " branches, each branch's in one launch per kernel, checkpoints every " +
                std::to_string(p.fused) + " steps";
     return std::to_string(p.branches) + " branches, " +
            (p.chunk ? std::to_string(p.chunk) + " pairs a launch" : std::string("a class a launch"));
@@ -2687,9 +2691,38 @@ struct nxs_batch {
                      const std::vector<nxs::BatchLaunch> &ls) {
         using namespace nxs;
         if (p.fused) {
-            std::vector<MixedMember> members;
-            for (const auto &c : classes) members.push_back(MixedMember{c.p, c.first, c.count});
-            HIP_TRY(launch_match_mixed(members.data(), (int)members.size(), tables, p.fused, s));
+            // Whole classes dealt largest first (by cost evaluations) to the least loaded
+            // branch; each branch's classes in one launch per kernel.
+            const int nb = std::max(1, std::min(p.branches, (int)classes.size()));
+            std::vector<int> order(classes.size());
+            for (size_t c = 0; c < order.size(); c++) order[c] = (int)c;
+            auto work = [&](int c) {
+                const MatchParams &q = classes[c].p;
+                return (double)q.width * q.height * q.disparities * classes[c].count;
+            };
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int a, int b) { return work(a) > work(b); });
+            std::vector<MixedMember> groups[MAX_BRANCHES];
+            double load[MAX_BRANCHES] = {};
+            for (int c : order) {
+                int best = 0;
+                for (int b = 1; b < nb; b++)
+                    if (load[b] < load[best]) best = b;
+                groups[best].push_back(MixedMember{classes[c].p, classes[c].first, classes[c].count});
+                load[best] += work(c);
+            }
+            if (nb == 1) {
+                HIP_TRY(launch_match_mixed(groups[0].data(), (int)groups[0].size(), tables, p.fused, s));
+                return;
+            }
+            HIP_TRY(hipEventRecord(fork, s));
+            for (int b = 0; b < nb; b++) {
+                HIP_TRY(hipStreamWaitEvent(branch[b], fork, 0));
+                HIP_TRY(launch_match_mixed(groups[b].data(), (int)groups[b].size(), tables,
+                                           p.fused, branch[b]));
+                HIP_TRY(hipEventRecord(join[b], branch[b]));
+                HIP_TRY(hipStreamWaitEvent(s, join[b], 0));
+            }
             return;
         }
         if (p.branches <= 1) {
@@ -2720,7 +2753,8 @@ struct nxs_batch {
         for (int branches : {1, 2, 4})
             for (int chunk : {0, 8, 2}) plans.push_back(BatchPlan{branches, chunk, 0});
         if (classes.size() <= (size_t)MAX_MIXED_CLASSES)
-            for (int segment : {4, 8}) plans.push_back(BatchPlan{1, 0, segment});
+            for (int branches : {1, 2})
+                for (int segment : {4, 8}) plans.push_back(BatchPlan{branches, 0, segment});
         for (const BatchPlan &p : plans) {
             const std::vector<BatchLaunch> ls = deal(p);
             enqueue_all(stream, p, ls);
@@ -2792,12 +2826,15 @@ struct nxs_batch {
 
 extern "C" {
 
-int nxs_batch_create(const nxs_pair *pairs, int count, int mode, nxs_batch **out) {
-    using namespace nxs;
-    if (!pairs || count < 1 || !out) return fail(NXS_EINVAL, "no pairs or output pointer");
-    if (mode < NXS_BATCH_DEFAULT || mode > NXS_BATCH_FUSED)
-        return fail(NXS_EINVAL, "mode must be NXS_BATCH_DEFAULT, NXS_BATCH_TUNED or NXS_BATCH_FUSED");
-    const bool autotune = mode == NXS_BATCH_TUNED;
+}  // extern "C"
+
+namespace nxs {
+
+// nxs_batch_create in `mode`, or with the plan `pinned` (untuned) where given.
+static int create_batch(const nxs_pair *pairs, int count, int mode, const BatchPlan *pinned,
+                        nxs_batch **out) {
+    const bool autotune = mode == NXS_BATCH_TUNED && !pinned;
+    const bool fused = pinned ? pinned->fused != 0 : mode == NXS_BATCH_FUSED;
     auto batch = std::make_unique<nxs_batch>();
     try {
         // Size classes in order of first appearance; each pair's workspace in one allocation.
@@ -2822,7 +2859,7 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int mode, nxs_batch **out
             }
             members[c].push_back(i);
         }
-        if (mode == NXS_BATCH_FUSED && batch->classes.size() > (size_t)MAX_MIXED_CLASSES)
+        if (fused && batch->classes.size() > (size_t)MAX_MIXED_CLASSES)
             throw Error{"a fused batch takes at most " + std::to_string(MAX_MIXED_CLASSES) +
                         " size classes, got " + std::to_string(batch->classes.size())};
         size_t total = 0;
@@ -2870,9 +2907,9 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int mode, nxs_batch **out
             }
         // Without tuning: the classes in two branches, a class a launch; or fused, checkpoints
         // every 8 steps.
-        batch->plan = mode == NXS_BATCH_FUSED
-                          ? BatchPlan{1, 0, 8}
-                          : BatchPlan{std::min(2, (int)batch->classes.size()), 0, 0};
+        batch->plan = pinned ? *pinned
+                             : fused ? BatchPlan{1, 0, 8}
+                                     : BatchPlan{std::min(2, (int)batch->classes.size()), 0, 0};
         if (autotune) batch->tune_plan();
         batch->launches = batch->deal(batch->plan);
         // One graph for the whole batch.
@@ -2895,7 +2932,9 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int mode, nxs_batch **out
         HIP_TRY(hipGraphInstantiate(&batch->exec, batch->graph, nullptr, nullptr, 0));
         std::ostringstream text;
         text << "plan: " << plan_text(batch->plan) << ", "
-             << (batch->plan.fused ? std::string("4 launches")
+             << (batch->plan.fused
+                     ? std::to_string(4 * std::min(batch->plan.branches,
+                                                   (int)batch->classes.size())) + " launches"
                                    : std::to_string(batch->launches.size()) + " launch chains")
              << "\n";
         for (auto &c : batch->classes) {
@@ -2922,6 +2961,32 @@ int nxs_batch_create(const nxs_pair *pairs, int count, int mode, nxs_batch **out
     }
     *out = batch.release();
     return NXS_OK;
+}
+
+}  // namespace nxs
+
+extern "C" {
+
+int nxs_batch_create(const nxs_pair *pairs, int count, int mode, nxs_batch **out) {
+    using namespace nxs;
+    if (!pairs || count < 1 || !out) return fail(NXS_EINVAL, "no pairs or output pointer");
+    if (mode < NXS_BATCH_DEFAULT || mode > NXS_BATCH_FUSED)
+        return fail(NXS_EINVAL, "mode must be NXS_BATCH_DEFAULT, NXS_BATCH_TUNED or NXS_BATCH_FUSED");
+    return create_batch(pairs, count, mode, nullptr, out);
+}
+
+int nxs_batch_create_planned(const nxs_pair *pairs, int count, const nxs_batch_plan *plan,
+                             nxs_batch **out) {
+    using namespace nxs;
+    if (!pairs || count < 1 || !out || !plan)
+        return fail(NXS_EINVAL, "no pairs, plan or output pointer");
+    if (plan->branches != 1 && plan->branches != 2 && plan->branches != 4)
+        return fail(NXS_EINVAL, "branches must be 1, 2 or 4");
+    if (plan->chunk < 0) return fail(NXS_EINVAL, "chunk must be 0 or more pairs");
+    if (plan->fused != 0 && plan->fused != 4 && plan->fused != 8)
+        return fail(NXS_EINVAL, "fused must be 0, 4 or 8");
+    const BatchPlan pinned{plan->branches, plan->fused ? 0 : plan->chunk, plan->fused};
+    return create_batch(pairs, count, NXS_BATCH_DEFAULT, &pinned, out);
 }
 
 int nxs_schedule_set(int width, int height, int disparities, int paths, const nxs_schedule *s) {

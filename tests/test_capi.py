@@ -116,6 +116,18 @@ def test_a_batch_refuses_an_unknown_mode_and_a_fused_batch_of_nine_classes_befor
     assert lib.nxs_batch_create(pairs, 9, _native.BATCH_FUSED, ctypes.byref(handle)) != _native.NXS_OK
     assert "at most 8 size classes, got 9" in _native.last_error()
     assert not handle.value
+    # A pinned plan: its fields are checked, and a fused one holds at most 8 classes too.
+    for plan, reason in (((3, 0, 0), "branches must be 1, 2 or 4"),
+                         ((1, -1, 0), "chunk must be 0 or more"),
+                         ((2, 0, 5), "fused must be 0, 4 or 8")):
+        rc = lib.nxs_batch_create_planned(pairs, 9, ctypes.byref(_native.NxsBatchPlan(*plan)),
+                                          ctypes.byref(handle))
+        assert rc == _native.NXS_EINVAL and reason in _native.last_error(), plan
+    rc = lib.nxs_batch_create_planned(pairs, 9, ctypes.byref(_native.NxsBatchPlan(2, 0, 8)),
+                                      ctypes.byref(handle))
+    assert rc != _native.NXS_OK and "at most 8 size classes, got 9" in _native.last_error()
+    assert lib.nxs_batch_create_planned(pairs, 9, None, ctypes.byref(handle)) == _native.NXS_EINVAL
+    assert not handle.value
 
 
 def test_the_nif_calls_only_entry_points_the_header_declares():

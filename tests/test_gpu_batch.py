@@ -53,6 +53,40 @@ def test_batch_matches_the_oracle_on_every_replay(fused):
     batch.close()
 
 
+# (branches, pairs a launch, fused checkpoints): the tuned plans, pinned one by one.
+PLANS = [(1, 0, 0), (1, 2, 0), (2, 0, 0), (2, 8, 0), (4, 2, 0), (1, 0, 4), (1, 0, 8), (2, 0, 4),
+         (2, 0, 8), (4, 0, 8)]
+
+
+def test_every_plan_gives_the_oracle_results():
+    """nxs_batch_create_planned: every plan the tuner times, pinned, on the mixed batch
+    (four classes, 64 to 192 disparities, one at 3 paths), two replays each."""
+    pairs = make_pairs(2)
+    bufs = [(hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * cfg[0] * cfg[1]))
+            for left, right, cfg in pairs]
+    wants = [oracle.sgm(left, right, *cfg[2:]) for left, right, cfg in pairs]
+    stream = hip.Stream()
+    try:
+        for plan in PLANS:
+            print("plan", plan, flush=True)
+            batch = _native.Batch([(l.ptr, r.ptr, o.ptr, *cfg)
+                                   for (l, r, o), (_, _, cfg) in zip(bufs, pairs)], plan=plan)
+            for replay in range(2):
+                for _l, _r, o in bufs:
+                    o.upload(np.zeros(o.nbytes // 4, np.float32))
+                batch.run(stream.handle)
+                stream.synchronize()
+                for (left, right, (w, h, d, *_)), (_l, _r, o), want in zip(pairs, bufs, wants):
+                    got = o.download((h, w), np.float32)
+                    assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (plan, replay, w, h, d)
+            batch.close()
+    finally:
+        stream.destroy()
+        for b in bufs:
+            for x in b:
+                x.free()
+
+
 LANES = {64: (16, 32), 128: (16, 32, 64), 192: (16,), 256: (16, 32, 64)}
 WAVES = {64: (2, 4), 128: (2, 4, 8), 192: (2,), 256: (2, 4, 8)}  # more than one wave per line
 
