@@ -87,6 +87,50 @@ def test_every_plan_gives_the_oracle_results():
                 x.free()
 
 
+def test_a_fused_batch_of_eight_classes_up_to_256_disparities():
+    """The most classes one fused launch holds: 64, 128, 192 and 256 disparities at 8 and 3
+    paths (so the launches hold every disparity count's body, up to 16 a lane), two pairs
+    a class, odd sizes, on two replays with new images between them."""
+    classes = [(w, h, d, paths, 8, 64, 1.0526316)
+               for (w, h), d in (((97, 31), 64), ((130, 37), 128), ((201, 29), 192),
+                                 ((300, 41), 256))
+               for paths in (8, 3)]
+    assert len(classes) == 8
+
+    def images(seed):
+        return [scenes.planar_pair(w, h, seed=seed + 10 * k + j, low=4, high=min(d - 8, 80))[:2]
+                for k, (w, h, d, *_r) in enumerate(classes) for j in range(2)]
+
+    configs = [cfg for cfg in classes for _ in range(2)]
+    pairs = images(500)
+    bufs = [(hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * cfg[0] * cfg[1]))
+            for (left, right), cfg in zip(pairs, configs)]
+    batch = _native.Batch([(l.ptr, r.ptr, o.ptr, *cfg) for (l, r, o), cfg in zip(bufs, configs)],
+                          autotune=False, fused=True)
+    print(batch.describe(), flush=True)
+    stream = hip.Stream()
+    try:
+        for replay in range(2):
+            if replay:
+                pairs = images(700)
+                for (left, right), (l, r, _o) in zip(pairs, bufs):
+                    l.upload(left)
+                    r.upload(right)
+            batch.run(stream.handle)
+            stream.synchronize()
+            for (left, right), cfg, (_l, _r, o) in zip(pairs, configs, bufs):
+                w, h = cfg[:2]
+                got = o.download((h, w), np.float32)
+                want = oracle.sgm(left, right, *cfg[2:])
+                assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (replay, cfg)
+    finally:
+        batch.close()
+        stream.destroy()
+        for b in bufs:
+            for x in b:
+                x.free()
+
+
 LANES = {64: (16, 32), 128: (16, 32, 64), 192: (16,), 256: (16, 32, 64)}
 WAVES = {64: (2, 4), 128: (2, 4, 8), 192: (2,), 256: (2, 4, 8)}  # more than one wave per line
 
