@@ -98,6 +98,9 @@ constexpr int MAX_MIXED_CLASSES = 8;
 // keep_stages and positive are the first class's.
 hipError_t launch_match_mixed(const MixedMember *classes, int n, const PairDesc *table,
                               int segment, hipStream_t stream);
+// Whether launch_match_mixed takes these classes: at most MAX_MIXED_CLASSES of sizes the
+// matcher takes, and each of its one-dimensional grids under 2^31 threads.
+bool mixed_launch_fits(const MixedMember *classes, int n);
 
 // The left-right check and 3x3 median of a selected map and its keys (the last kernel of
 // every matching); `checked` receives the checked map when keep_checked.

@@ -2752,7 +2752,9 @@ struct nxs_batch {
         std::vector<BatchPlan> plans;
         for (int branches : {1, 2, 4})
             for (int chunk : {0, 8, 2}) plans.push_back(BatchPlan{branches, chunk, 0});
-        if (classes.size() <= (size_t)MAX_MIXED_CLASSES)
+        std::vector<MixedMember> all;
+        for (const auto &c : classes) all.push_back(MixedMember{c.p, c.first, c.count});
+        if (mixed_launch_fits(all.data(), (int)all.size()))  // at most 8 classes, and room
             for (int branches : {1, 2})
                 for (int segment : {4, 8}) plans.push_back(BatchPlan{branches, 0, segment});
         for (const BatchPlan &p : plans) {
@@ -2862,6 +2864,13 @@ static int create_batch(const nxs_pair *pairs, int count, int mode, const BatchP
         if (fused && batch->classes.size() > (size_t)MAX_MIXED_CLASSES)
             throw Error{"a fused batch takes at most " + std::to_string(MAX_MIXED_CLASSES) +
                         " size classes, got " + std::to_string(batch->classes.size())};
+        if (fused) {
+            std::vector<MixedMember> all;
+            for (size_t c = 0; c < batch->classes.size(); c++)
+                all.push_back(MixedMember{batch->classes[c].p, 0, (int)members[c].size()});
+            if (!mixed_launch_fits(all.data(), (int)all.size()))
+                throw Error{"a fused batch's launches would pass 2^31 threads: too many pairs"};
+        }
         size_t total = 0;
         for (size_t c = 0; c < batch->classes.size(); c++) {
             const MatchParams &p = batch->classes[c].p;

@@ -1997,17 +1997,52 @@ static void launch_mixed_aggregation(const MixedArgs &m, int path_blocks, int se
     profile_end(slot, stream);
 }
 
-hipError_t launch_match_mixed(const MixedMember *members, int n, const PairDesc *table,
-                              int segment, hipStream_t stream) {
-    if (n < 1 || n > MAX_MIXED_CLASSES || !table || (segment != 4 && segment != 8))
-        return hipErrorInvalidValue;
+// One pair's blocks in the fused path launch, and its select blocks a row: as
+// launch_paths_and_select lays out a pair with two waves of 16 lanes a line, four lines a
+// block (eight down-only columns at 3 paths), and select with 16 disparities a lane (12 at
+// 192).
+static void mixed_pair_geometry(const MatchParams &p, int block_start[4], int *path_blocks,
+                                int *select_x) {
+    const int W = p.width, H = p.height;
+    const int lines[4] = {H, W, W + H - 1, W + H - 1};
+    int blocks = 0;
+    for (int o = 0; o < 4; o++) {
+        block_start[o] = blocks;
+        const int per_block = o == 1 && p.paths != 8 ? 8 : 4;
+        if (o < orientations_of(p.paths)) blocks += (lines[o] + per_block - 1) / per_block;
+    }
+    *path_blocks = blocks;
+    const int K = p.disparities / 16;
+    const int spl = K == 12 ? 16 : K;
+    *select_x = (W + 256 / spl * SELECT_GROUPS - 1) / (256 / spl * SELECT_GROUPS);
+}
+
+bool mixed_launch_fits(const MixedMember *members, int n) {
+    if (n < 1 || n > MAX_MIXED_CLASSES) return false;
+    // Each launch's grid is one dimension: its threads stay under 2^31.
+    const int64_t threads[4] = {1024, TWO_BLOCK, 256, CHECK_BLOCK};
+    int64_t blocks[4] = {0, 0, 0, 0};
     for (int i = 0; i < n; i++) {
         const MatchParams &p = members[i].p;
         if (p.width < 1 || p.width > MAX_WIDTH || p.height < 1 || members[i].count < 1 ||
             (p.disparities != 64 && p.disparities != 128 && p.disparities != 192 &&
              p.disparities != 256))
-            return hipErrorInvalidValue;
+            return false;
+        int block_start[4], path_blocks, select_x;
+        mixed_pair_geometry(p, block_start, &path_blocks, &select_x);
+        const int64_t per_pair[4] = {p.height, path_blocks, (int64_t)select_x * p.height,
+                                     p.height};
+        for (int w = 0; w < 4; w++) blocks[w] += per_pair[w] * members[i].count;
     }
+    for (int w = 0; w < 4; w++)
+        if (blocks[w] * threads[w] > INT32_MAX) return false;
+    return true;
+}
+
+hipError_t launch_match_mixed(const MixedMember *members, int n, const PairDesc *table,
+                              int segment, hipStream_t stream) {
+    if (!table || (segment != 4 && segment != 8) || !mixed_launch_fits(members, n))
+        return hipErrorInvalidValue;
     // The classes with the most work a pair first: their long lines start first.
     int order[MAX_MIXED_CLASSES];
     for (int i = 0; i < n; i++) order[i] = i;
@@ -2042,19 +2077,8 @@ hipError_t launch_match_mixed(const MixedMember *members, int n, const PairDesc 
         a.first = 0;
         a.last = c.orientations - 1;
         a.longest_first = p.paths == 8;
-        // As launch_paths_and_select lays out a pair: two waves of 16 lanes take four lines
-        // a block (eight down-only columns at 3 paths).
-        const int lines[4] = {H, W, W + H - 1, W + H - 1};
-        int blocks = 0;
-        for (int o = 0; o < 4; o++) {
-            a.block_start[o] = blocks;
-            const int per_block = o == 1 && !a.backward_columns ? 8 : 4;
-            if (o < c.orientations) blocks += (lines[o] + per_block - 1) / per_block;
-        }
-        c.path_blocks = blocks;
+        mixed_pair_geometry(p, a.block_start, &c.path_blocks, &c.select_x);
         const int K = p.disparities / 16;
-        const int spl = K == 12 ? 16 : K;  // 16 disparities a select lane, 12 at 192
-        c.select_x = (W + 256 / spl * SELECT_GROUPS - 1) / (256 / spl * SELECT_GROUPS);
         const int per_pair[4] = {H, c.path_blocks, c.select_x * H, H};
         for (int w = 0; w < 4; w++) {
             c.start[w] = starts[w];

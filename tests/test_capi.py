@@ -128,6 +128,15 @@ def test_a_batch_refuses_an_unknown_mode_and_a_fused_batch_of_nine_classes_befor
     assert rc != _native.NXS_OK and "at most 8 size classes, got 9" in _native.last_error()
     assert lib.nxs_batch_create_planned(pairs, 9, None, ctypes.byref(handle)) == _native.NXS_EINVAL
     assert not handle.value
+    # One class of 2^18 pairs of 64x8: the fused census launch, a 1024-thread block a row
+    # in one grid dimension, would hold 2^31 threads.
+    many = 1 << 18
+    pairs = (_native.NxsPair * many)()
+    for k in range(many):
+        pairs[k] = _native.NxsPair(fake, fake, fake, 64, 8, 64, 8, 8, 64, 1.0526316)
+    assert lib.nxs_batch_create(pairs, many, _native.BATCH_FUSED, ctypes.byref(handle)) != _native.NXS_OK
+    assert "would pass 2^31 threads" in _native.last_error()
+    assert not handle.value
 
 
 def test_the_nif_calls_only_entry_points_the_header_declares():
