@@ -131,6 +131,38 @@ def test_a_fused_batch_of_eight_classes_up_to_256_disparities():
                 x.free()
 
 
+def test_a_fused_batch_of_extreme_shapes():
+    """Classes whose blocks a pair differ by hundreds of times in each fused launch: 4096-wide
+    rows, 700-tall columns, a 9x7 pair (lines shorter than a checkpoint segment), 3 paths
+    (eight down-only columns a block), one to three pairs a class."""
+    classes = [((4096, 8, 64, 8), 1), ((16, 700, 64, 8), 2), ((9, 7, 64, 3), 3),
+               ((33, 17, 128, 8), 2), ((250, 60, 192, 3), 1), ((70, 90, 256, 8), 2)]
+    configs = [(w, h, d, paths, 8, 64, 1.0526316) for (w, h, d, paths), n in classes
+               for _ in range(n)]
+    pairs = [scenes.planar_pair(w, h, seed=900 + k, low=1, high=min(d - 8, 40, max(w - 2, 2)))[:2]
+             for k, (w, h, d, *_r) in enumerate(configs)]
+    bufs = [(hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * cfg[0] * cfg[1]))
+            for (left, right), cfg in zip(pairs, configs)]
+    batch = _native.Batch([(l.ptr, r.ptr, o.ptr, *cfg) for (l, r, o), cfg in zip(bufs, configs)],
+                          autotune=False, fused=True)
+    print(batch.describe(), flush=True)
+    stream = hip.Stream()
+    try:
+        batch.run(stream.handle)
+        stream.synchronize()
+        for (left, right), cfg, (_l, _r, o) in zip(pairs, configs, bufs):
+            w, h = cfg[:2]
+            got = o.download((h, w), np.float32)
+            want = oracle.sgm(left, right, *cfg[2:])
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), cfg
+    finally:
+        batch.close()
+        stream.destroy()
+        for b in bufs:
+            for x in b:
+                x.free()
+
+
 LANES = {64: (16, 32), 128: (16, 32, 64), 192: (16,), 256: (16, 32, 64)}
 WAVES = {64: (2, 4), 128: (2, 4, 8), 192: (2,), 256: (2, 4, 8)}  # more than one wave per line
 
