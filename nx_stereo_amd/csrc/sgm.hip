@@ -489,13 +489,17 @@ struct BlockLines {
 
 __device__ __forceinline__ BlockLines block_lines(const PathArgs &a, int b, int per_block) {
     if (!a.longest_first) {
-        int o = a.last;
-        for (int k = a.first; k < a.last; k++)
-            if (b < a.block_start[k + 1]) {
-                o = k;
-                break;
-            }
-        return {o, (b - a.block_start[o]) * per_block};
+        // The last launched orientation whose first block is at or before b. Constant indices
+        // only: `a` may be a kernel argument reached through a reference (the fused batch
+        // launch's), where a run-time index would load it into vector registers.
+        int o = a.first, start = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool here = k >= a.first && k <= a.last && b >= a.block_start[k];
+            o = here ? k : o;
+            start = here ? a.block_start[k] : start;
+        }
+        return {o, (b - start) * per_block};
     }
     const int rows = a.block_start[1] - a.block_start[0];
     const int columns = a.block_start[2] - a.block_start[1];
@@ -1936,7 +1940,7 @@ __global__ __launch_bounds__(TWO_BLOCK) void path_rc_mixed_kernel(MixedArgs m) {
     const MixedClass &c = m.c[mixed_class(m, MIX_PATH, blockIdx.x)];
     const int local = blockIdx.x - c.start[MIX_PATH];
     const int pair = local / c.path_blocks, b = local - pair * c.path_blocks;
-    const PathArgs &a = c.path;
+    const PathArgs a = c.path;  // in scalar registers: the class index is a run-time one
     const int index = c.first + pair;
     switch (c.D) {
         case 64: path_rc_block<4, S, 16, STREAM>(a, m.table, index, b); break;
