@@ -199,7 +199,9 @@ int nxs_coarse_to_fine_host(const uint8_t *left, const uint8_t *right, int width
  *     for the batch (single-pair launches of that size keep their own
  *     schedule); then the plans (one, two or four branches, each class in
  *     one launch or in launches of 8 or 2 pairs, and the fused plan below
- *     with checkpoints every 4 or 8 steps) are timed and the fastest kept;
+ *     with checkpoints every 4 or 8 steps, on one or two branches, its
+ *     aggregation in one block per line group or in persistent blocks on a
+ *     device work queue) are timed and the fastest kept;
  *   NXS_BATCH_FUSED: every class in one launch per kernel (census and
  *     costs, aggregation, selection, check and median: four launches for the
  *     batch), the aggregation as two waves of 16 lanes a line meeting by
@@ -257,6 +259,9 @@ typedef struct nxs_batch_plan {
     int fused;    /* 0; or 4 or 8: each branch's classes (whole classes, the largest first to
                      the least loaded branch; at most 8 classes) in one launch per kernel,
                      checkpoints every `fused` steps */
+    int queue;    /* with fused: 1 = the aggregation launch in persistent blocks (as many as
+                     the device holds at once) taking its line groups from a device work
+                     queue, 0 = one block per line group */
 } nxs_batch_plan;
 
 /* nxs_batch_create with the plan pinned, untuned, and each class's schedule its

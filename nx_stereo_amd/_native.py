@@ -59,9 +59,11 @@ class NxsSchedule(ctypes.Structure):
 
 
 class NxsBatchPlan(ctypes.Structure):
-    """struct nxs_batch_plan: a batch's branches, pairs a launch and fused checkpoints."""
+    """struct nxs_batch_plan: a batch's branches, pairs a launch, fused checkpoints and
+    whether the fused aggregation runs in persistent blocks on a work queue."""
 
-    _fields_ = [("branches", ctypes.c_int), ("chunk", ctypes.c_int), ("fused", ctypes.c_int)]
+    _fields_ = [("branches", ctypes.c_int), ("chunk", ctypes.c_int), ("fused", ctypes.c_int),
+                ("queue", ctypes.c_int)]
 
 
 class NativeError(RuntimeError):
@@ -333,8 +335,8 @@ class Batch:
     """nxs_batch: pairs (left, right, out device addresses; width, height, disparities,
     paths, p1, p2, uniqueness) captured into one graph; run() replays it on a stream.
     autotune: the class schedules and plans timed on the batch (NXS_BATCH_TUNED); fused:
-    every class in one launch per kernel (NXS_BATCH_FUSED); plan: (branches, chunk, fused)
-    pinned, untuned (nxs_batch_create_planned)."""
+    every class in one launch per kernel (NXS_BATCH_FUSED); plan: (branches, chunk, fused
+    [, queue]) pinned, untuned (nxs_batch_create_planned)."""
 
     def __init__(self, pairs, autotune: bool = True, fused: bool = False, plan=None):
         lib = load()

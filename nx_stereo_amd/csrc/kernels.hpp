@@ -95,9 +95,12 @@ constexpr int MAX_MIXED_CLASSES = 8;
 // Every class of a batch in one launch per kernel (census and costs, the aggregation,
 // selection, check and median), the aggregation as two waves of 16 lanes a line meeting by
 // recomputation every `segment` (4 or 8) steps: up to MAX_MIXED_CLASSES classes, whose
-// keep_stages and positive are the first class's.
+// keep_stages and positive are the first class's. With `queue` (two u32 counters in device
+// memory, zero before the first launch; each launch leaves them at zero), the aggregation
+// runs in persistent blocks that take its line groups from the queue; launches sharing one
+// queue must not overlap.
 hipError_t launch_match_mixed(const MixedMember *classes, int n, const PairDesc *table,
-                              int segment, hipStream_t stream);
+                              int segment, uint32_t *queue, hipStream_t stream);
 // Whether launch_match_mixed takes these classes: at most MAX_MIXED_CLASSES of sizes the
 // matcher takes, and each of its one-dimensional grids under 2^31 threads.
 bool mixed_launch_fits(const MixedMember *classes, int n);

@@ -2605,6 +2605,8 @@ struct BatchPlan {
     int branches = 1;
     int chunk = 0;
     int fused = 0;
+    int queue = 0;  // fused: 1 = the aggregation in persistent blocks taking line groups from a
+                    // device work queue (one per branch)
 };
 
 // One launch chain of the batch: pairs tables[first .. first+count) of class `cls`.
@@ -2615,13 +2617,14 @@ struct BatchLaunch {
 constexpr int MAX_BRANCHES = 4;
 
 inline std::string plan_text(const BatchPlan &p) {
+    const std::string queue = p.queue ? ", the aggregation's blocks persistent on a work queue" : "";
     if (p.fused && p.branches <= 1)
         return "every class in one launch per kernel, checkpoints every " +
-               std::to_string(p.fused) + " steps";
+               std::to_string(p.fused) + " steps" + queue;
     if (p.fused)
         return "the classes on " + std::to_string(p.branches) +
                " branches, each branch's in one launch per kernel, checkpoints every " +
-               std::to_string(p.fused) + " steps";
+               std::to_string(p.fused) + " steps" + queue;
     return std::to_string(p.branches) + " branches, " +
            (p.chunk ? std::to_string(p.chunk) + " pairs a launch" : std::string("a class a launch"));
 }
@@ -2632,6 +2635,7 @@ struct nxs_batch {
     std::vector<nxs::BatchClass> classes;
     void *workspace = nullptr;
     nxs::PairDesc *tables = nullptr;
+    uint32_t *queues = nullptr;  // two counters a branch, for fused plans with a work queue
     hipStream_t stream = nullptr;
     // The graph's parallel branches, with the events that fork them from `stream` and join
     // them back (all at the default priority).
@@ -2650,6 +2654,7 @@ struct nxs_batch {
         for (auto b : branch)
             if (b) (void)hipStreamSynchronize(b);
         if (tables) (void)hipFree(tables);
+        if (queues) (void)hipFree(queues);
         if (workspace) (void)hipFree(workspace);
         if (stream) (void)hipStreamDestroy(stream);
         for (auto b : branch)
@@ -2711,15 +2716,17 @@ struct nxs_batch {
                 groups[best].push_back(MixedMember{classes[c].p, classes[c].first, classes[c].count});
                 load[best] += work(c);
             }
+            auto queue = [&](int b) { return p.queue ? queues + 2 * b : nullptr; };
             if (nb == 1) {
-                HIP_TRY(launch_match_mixed(groups[0].data(), (int)groups[0].size(), tables, p.fused, s));
+                HIP_TRY(launch_match_mixed(groups[0].data(), (int)groups[0].size(), tables, p.fused,
+                                           queue(0), s));
                 return;
             }
             HIP_TRY(hipEventRecord(fork, s));
             for (int b = 0; b < nb; b++) {
                 HIP_TRY(hipStreamWaitEvent(branch[b], fork, 0));
                 HIP_TRY(launch_match_mixed(groups[b].data(), (int)groups[b].size(), tables,
-                                           p.fused, branch[b]));
+                                           p.fused, queue(b), branch[b]));
                 HIP_TRY(hipEventRecord(join[b], branch[b]));
                 HIP_TRY(hipStreamWaitEvent(s, join[b], 0));
             }
@@ -2755,8 +2762,10 @@ struct nxs_batch {
         std::vector<MixedMember> all;
         for (const auto &c : classes) all.push_back(MixedMember{c.p, c.first, c.count});
         if (mixed_launch_fits(all.data(), (int)all.size()))  // at most 8 classes, and room
-            for (int branches : {1, 2})
-                for (int segment : {4, 8}) plans.push_back(BatchPlan{branches, 0, segment});
+            for (int branches : {1, 2}) {
+                for (int segment : {4, 8}) plans.push_back(BatchPlan{branches, 0, segment, 0});
+                plans.push_back(BatchPlan{branches, 0, 8, 1});
+            }
         for (const BatchPlan &p : plans) {
             const std::vector<BatchLaunch> ls = deal(p);
             enqueue_all(stream, p, ls);
@@ -2878,6 +2887,8 @@ static int create_batch(const nxs_pair *pairs, int count, int mode, const BatchP
         }
         HIP_TRY(hipMalloc(&batch->workspace, total));
         HIP_TRY(hipMalloc((void **)&batch->tables, sizeof(PairDesc) * count));
+        HIP_TRY(hipMalloc((void **)&batch->queues, sizeof(uint32_t) * 2 * MAX_BRANCHES));
+        HIP_TRY(hipMemset(batch->queues, 0, sizeof(uint32_t) * 2 * MAX_BRANCHES));
         HIP_TRY(hipStreamCreateWithFlags(&batch->stream, hipStreamNonBlocking));
         for (int b = 0; b < MAX_BRANCHES; b++) {
             HIP_TRY(hipStreamCreateWithFlags(&batch->branch[b], hipStreamNonBlocking));
@@ -2917,8 +2928,8 @@ static int create_batch(const nxs_pair *pairs, int count, int mode, const BatchP
         // Without tuning: the classes in two branches, a class a launch; or fused, checkpoints
         // every 8 steps.
         batch->plan = pinned ? *pinned
-                             : fused ? BatchPlan{1, 0, 8}
-                                     : BatchPlan{std::min(2, (int)batch->classes.size()), 0, 0};
+                             : fused ? BatchPlan{1, 0, 8, 0}
+                                     : BatchPlan{std::min(2, (int)batch->classes.size()), 0, 0, 0};
         if (autotune) batch->tune_plan();
         batch->launches = batch->deal(batch->plan);
         // One graph for the whole batch.
@@ -2994,7 +3005,10 @@ int nxs_batch_create_planned(const nxs_pair *pairs, int count, const nxs_batch_p
     if (plan->chunk < 0) return fail(NXS_EINVAL, "chunk must be 0 or more pairs");
     if (plan->fused != 0 && plan->fused != 4 && plan->fused != 8)
         return fail(NXS_EINVAL, "fused must be 0, 4 or 8");
-    const BatchPlan pinned{plan->branches, plan->fused ? 0 : plan->chunk, plan->fused};
+    if (plan->queue != 0 && plan->queue != 1) return fail(NXS_EINVAL, "queue must be 0 or 1");
+    if (plan->queue && !plan->fused) return fail(NXS_EINVAL, "a work queue takes a fused plan");
+    const BatchPlan pinned{plan->branches, plan->fused ? 0 : plan->chunk, plan->fused,
+                           plan->queue};
     return create_batch(pairs, count, NXS_BATCH_DEFAULT, &pinned, out);
 }
 

@@ -1935,10 +1935,11 @@ __global__ __launch_bounds__(COST_BLOCK) void census_cost_mixed_kernel(MixedArgs
 
 // Two waves of 16 lanes a line meeting by recomputation (path_rc_kernel), disparity counts up
 // to 16 KMAX.
+// Item `item` of the fused path launch: the block of that index in the launch's grid.
 template <int KMAX, int S, bool STREAM>
-__global__ __launch_bounds__(TWO_BLOCK) void path_rc_mixed_kernel(MixedArgs m) {
-    const MixedClass &c = m.c[mixed_class(m, MIX_PATH, blockIdx.x)];
-    const int local = blockIdx.x - c.start[MIX_PATH];
+__device__ __forceinline__ void mixed_path_item(const MixedArgs &m, int item) {
+    const MixedClass &c = m.c[mixed_class(m, MIX_PATH, item)];
+    const int local = item - c.start[MIX_PATH];
     const int pair = local / c.path_blocks, b = local - pair * c.path_blocks;
     const PathArgs a = c.path;  // in scalar registers: the class index is a run-time one
     const int index = c.first + pair;
@@ -1947,6 +1948,34 @@ __global__ __launch_bounds__(TWO_BLOCK) void path_rc_mixed_kernel(MixedArgs m) {
         case 128: if constexpr (KMAX >= 8) path_rc_block<8, S, 16, STREAM>(a, m.table, index, b); break;
         case 192: if constexpr (KMAX >= 12) path_rc_block<12, S, 16, STREAM>(a, m.table, index, b); break;
         default: if constexpr (KMAX >= 16) path_rc_block<16, S, 16, STREAM>(a, m.table, index, b); break;
+    }
+}
+
+template <int KMAX, int S, bool STREAM>
+__global__ __launch_bounds__(TWO_BLOCK) void path_rc_mixed_kernel(MixedArgs m) {
+    mixed_path_item<KMAX, S, STREAM>(m, blockIdx.x);
+}
+
+// The same items with persistent blocks: as many blocks as the device holds at once, each
+// taking the next item (in the grid's order) from queue[0] until none is left. An item's
+// two waves meet at most once, at one s_barrier both reach (or neither: a block whose lines
+// are all empty), so the item loop's own barriers stay in step. The last block to leave
+// (queue[1] counts them) sets both counters back to zero for the next launch.
+template <int KMAX, int S, bool STREAM>
+__global__ __launch_bounds__(TWO_BLOCK) void path_rc_queue_kernel(MixedArgs m, uint32_t *queue,
+                                                                  int items) {
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = (int)atomicAdd(&queue[0], 1u);
+        __syncthreads();
+        const int item = next;
+        __syncthreads();  // every thread has read `next` before it is written again
+        if (item >= items) break;
+        mixed_path_item<KMAX, S, STREAM>(m, item);
+    }
+    if (threadIdx.x == 0 && atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+        atomicExch(&queue[0], 0u);
+        atomicExch(&queue[1], 0u);
     }
 }
 
@@ -1977,7 +2006,24 @@ __global__ __launch_bounds__(CHECK_BLOCK) void check_median_mixed_kernel(MixedAr
 }
 
 template <int KMAX, bool STREAM>
-static void launch_mixed_paths(const MixedArgs &m, int blocks, int segment, hipStream_t stream) {
+static void launch_mixed_paths(const MixedArgs &m, int blocks, int segment, uint32_t *queue,
+                               hipStream_t stream) {
+    if (queue) {
+        // Persistent blocks: what the device holds of this kernel at once.
+        const void *kernel = segment == 4
+                                 ? reinterpret_cast<const void *>(path_rc_queue_kernel<KMAX, 4, STREAM>)
+                                 : reinterpret_cast<const void *>(path_rc_queue_kernel<KMAX, 8, STREAM>);
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, TWO_BLOCK, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        const int grid = std::min(blocks, per_cu * compute_unit_count());
+        if (segment == 4)
+            hipLaunchKernelGGL((path_rc_queue_kernel<KMAX, 4, STREAM>), dim3(grid), dim3(TWO_BLOCK), 0, stream, m, queue, blocks);
+        else
+            hipLaunchKernelGGL((path_rc_queue_kernel<KMAX, 8, STREAM>), dim3(grid), dim3(TWO_BLOCK), 0, stream, m, queue, blocks);
+        return;
+    }
     if (segment == 4)
         hipLaunchKernelGGL((path_rc_mixed_kernel<KMAX, 4, STREAM>), dim3(blocks), dim3(TWO_BLOCK), 0, stream, m);
     else
@@ -1986,12 +2032,13 @@ static void launch_mixed_paths(const MixedArgs &m, int blocks, int segment, hipS
 
 template <int KMAX>
 static void launch_mixed_aggregation(const MixedArgs &m, int path_blocks, int select_blocks,
-                                     int segment, bool stream_volumes, hipStream_t stream) {
+                                     int segment, uint32_t *queue, bool stream_volumes,
+                                     hipStream_t stream) {
     int slot = profile_begin(PK_PATH, stream);
     if (stream_volumes)
-        launch_mixed_paths<KMAX, true>(m, path_blocks, segment, stream);
+        launch_mixed_paths<KMAX, true>(m, path_blocks, segment, queue, stream);
     else
-        launch_mixed_paths<KMAX, false>(m, path_blocks, segment, stream);
+        launch_mixed_paths<KMAX, false>(m, path_blocks, segment, queue, stream);
     profile_end(slot, stream);
     slot = profile_begin(PK_SELECT, stream);
     if (stream_volumes)
@@ -2044,7 +2091,7 @@ bool mixed_launch_fits(const MixedMember *members, int n) {
 }
 
 hipError_t launch_match_mixed(const MixedMember *members, int n, const PairDesc *table,
-                              int segment, hipStream_t stream) {
+                              int segment, uint32_t *queue, hipStream_t stream) {
     if (!table || (segment != 4 && segment != 8) || !mixed_launch_fits(members, n))
         return hipErrorInvalidValue;
     // The classes with the most work a pair first: their long lines start first.
@@ -2119,10 +2166,10 @@ hipError_t launch_match_mixed(const MixedMember *members, int n, const PairDesc 
     profile_end(slot, stream);
     const bool stream_volumes = volumes > ((size_t)256 << 20);
     switch (kmax) {
-        case 4: launch_mixed_aggregation<4>(m, starts[MIX_PATH], starts[MIX_SELECT], segment, stream_volumes, stream); break;
-        case 8: launch_mixed_aggregation<8>(m, starts[MIX_PATH], starts[MIX_SELECT], segment, stream_volumes, stream); break;
-        case 12: launch_mixed_aggregation<12>(m, starts[MIX_PATH], starts[MIX_SELECT], segment, stream_volumes, stream); break;
-        default: launch_mixed_aggregation<16>(m, starts[MIX_PATH], starts[MIX_SELECT], segment, stream_volumes, stream); break;
+        case 4: launch_mixed_aggregation<4>(m, starts[MIX_PATH], starts[MIX_SELECT], segment, queue, stream_volumes, stream); break;
+        case 8: launch_mixed_aggregation<8>(m, starts[MIX_PATH], starts[MIX_SELECT], segment, queue, stream_volumes, stream); break;
+        case 12: launch_mixed_aggregation<12>(m, starts[MIX_PATH], starts[MIX_SELECT], segment, queue, stream_volumes, stream); break;
+        default: launch_mixed_aggregation<16>(m, starts[MIX_PATH], starts[MIX_SELECT], segment, queue, stream_volumes, stream); break;
     }
     slot = profile_begin(PK_CHECK_MEDIAN, stream);
     hipLaunchKernelGGL(check_median_mixed_kernel, dim3(starts[MIX_CHECK]), dim3(CHECK_BLOCK),

@@ -119,7 +119,9 @@ def test_a_batch_refuses_an_unknown_mode_and_a_fused_batch_of_nine_classes_befor
     # A pinned plan: its fields are checked, and a fused one holds at most 8 classes too.
     for plan, reason in (((3, 0, 0), "branches must be 1, 2 or 4"),
                          ((1, -1, 0), "chunk must be 0 or more"),
-                         ((2, 0, 5), "fused must be 0, 4 or 8")):
+                         ((2, 0, 5), "fused must be 0, 4 or 8"),
+                         ((1, 0, 8, 2), "queue must be 0 or 1"),
+                         ((1, 0, 0, 1), "a work queue takes a fused plan")):
         rc = lib.nxs_batch_create_planned(pairs, 9, ctypes.byref(_native.NxsBatchPlan(*plan)),
                                           ctypes.byref(handle))
         assert rc == _native.NXS_EINVAL and reason in _native.last_error(), plan

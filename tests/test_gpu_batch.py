@@ -53,14 +53,17 @@ def test_batch_matches_the_oracle_on_every_replay(fused):
     batch.close()
 
 
-# (branches, pairs a launch, fused checkpoints): the tuned plans, pinned one by one.
-PLANS = [(1, 0, 0), (1, 2, 0), (2, 0, 0), (2, 8, 0), (4, 2, 0), (1, 0, 4), (1, 0, 8), (2, 0, 4),
-         (2, 0, 8), (4, 0, 8)]
+# (branches, pairs a launch, fused checkpoints, work queue): the tuned plans, pinned one by
+# one.
+PLANS = [(1, 0, 0, 0), (1, 2, 0, 0), (2, 0, 0, 0), (2, 8, 0, 0), (4, 2, 0, 0), (1, 0, 4, 0),
+         (1, 0, 8, 0), (2, 0, 4, 0), (2, 0, 8, 0), (4, 0, 8, 0), (1, 0, 8, 1), (1, 0, 4, 1),
+         (2, 0, 8, 1)]
 
 
 def test_every_plan_gives_the_oracle_results():
     """nxs_batch_create_planned: every plan the tuner times, pinned, on the mixed batch
-    (four classes, 64 to 192 disparities, one at 3 paths), two replays each."""
+    (four classes, 64 to 192 disparities, one at 3 paths), two replays each (the second
+    replay of a work-queue plan starts from the counters the first left at zero)."""
     pairs = make_pairs(2)
     bufs = [(hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * cfg[0] * cfg[1]))
             for left, right, cfg in pairs]

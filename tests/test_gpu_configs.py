@@ -66,19 +66,21 @@ def batch_images(variant):
     return pairs
 
 
-@pytest.mark.parametrize("fused", [False, True], ids=["per_class", "fused"])
-def test_batch_of_96_pairs_at_the_benchmark_classes(fused):
+@pytest.mark.parametrize("form", ["per_class", "fused", "fused_queue"])
+def test_batch_of_96_pairs_at_the_benchmark_classes(form):
     """nxs_batch over the batched leg's classes, every pair checked after a replay, then
-    again with new images uploaded before the next replay: launches per class, and every
-    class in one launch per kernel."""
+    again with new images uploaded before the next replay: launches per class, every class
+    in one launch per kernel, and that with the aggregation's persistent blocks taking their
+    line groups from a work queue."""
     pairs = batch_images(0)
     bufs = [(hip.Buffer.of(left), hip.Buffer.of(right), hip.Buffer(4 * w * h))
             for left, right, (w, h, d) in pairs]
-    print(f"batch: 96 pairs, 24 x 480x270/64, 48 x 728x544/128, 24 x 1280x720/192, "
-          f"fused {fused}", flush=True)
+    print(f"batch: 96 pairs, 24 x 480x270/64, 48 x 728x544/128, 24 x 1280x720/192, {form}",
+          flush=True)
     batch = _native.Batch([(l.ptr, r.ptr, o.ptr, w, h, d, 8, 8, 64, UNIQUENESS)
                            for (l, r, o), (_, _, (w, h, d)) in zip(bufs, pairs)], autotune=False,
-                          fused=fused)
+                          fused=form == "fused",
+                          plan=(1, 0, 8, 1) if form == "fused_queue" else None)
     stream = hip.Stream()
     try:
         for variant in range(2):
